@@ -1,0 +1,314 @@
+"""MI355X-native DD-alpha-AMG for the 2-D Schwinger-model Wilson-Dirac operator.
+
+Python access to libamgs.so (include/amgs.h) through ctypes.  The library is
+the product: HIP kernels for gfx950 plus a C++ host runtime.  This module only
+marshals arrays; every numerical operation runs in libamgs on the GPU (the
+generators and file readers run in libamgs on the host).  There is no CPU
+fallback: if libamgs.so is missing or no HIP device is present, the calls
+raise.
+
+The class interface mirrors the reference's (Fabian2598/AMG_SchwingerModel):
+``FGMRES_AMG(dim1, dim2, m, restarts, tol, GConf, m0)`` runs the set-up in its
+constructor (include/amg.h:101-119) and ``fgmres(phi, x0, x)`` returns the
+iteration count (src/fgmres.cpp:5-88).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamgs.so")
+MAXL = 6
+
+AMGS_OK, AMGS_NOT_CONVERGED = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class AmgsError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nt", ctypes.c_int), ("levels", ctypes.c_int),
+        ("bx", ctypes.c_int * MAXL), ("bt", ctypes.c_int * MAXL), ("ntest", ctypes.c_int * MAXL),
+        ("sapx", ctypes.c_int * MAXL), ("sapt", ctypes.c_int * MAXL),
+        ("m0", ctypes.c_double),
+        ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
+        ("sap_m", ctypes.c_int), ("sap_restarts", ctypes.c_int), ("sap_gmres_tol", ctypes.c_double),
+        ("sap_tol", ctypes.c_double),
+        ("kc_m", ctypes.c_int), ("kc_restarts", ctypes.c_int), ("kc_tol", ctypes.c_double),
+        ("crs_m", ctypes.c_int), ("crs_restarts", ctypes.c_int), ("crs_tol", ctypes.c_double),
+        ("out_m", ctypes.c_int), ("out_restarts", ctypes.c_int), ("out_tol", ctypes.c_double),
+        ("tv_sap_iters", ctypes.c_int),
+        ("tv_seed", ctypes.c_uint32),
+        ("precision", ctypes.c_int),
+        ("max_rhs", ctypes.c_int),
+    ]
+
+
+def build(arch="gfx950"):
+    """Compile libamgs.so in place with hipcc (no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE, f"ARCH={arch}", "-B"], check=True)
+
+
+_lib = None
+
+# exported symbols and their argument types (include/amgs.h)
+_P = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_V = ctypes.c_void_p
+SIGNATURES = {
+    "amgs_default_params": ([ctypes.POINTER(Params)], ctypes.c_int),
+    "amgs_read_parameters": ([ctypes.c_char_p, ctypes.POINTER(Params)], ctypes.c_int),
+    "amgs_create": ([ctypes.POINTER(Params), ctypes.POINTER(_V)], ctypes.c_int),
+    "amgs_destroy": ([_V], None),
+    "amgs_last_error": ([_V], ctypes.c_char_p),
+    "amgs_set_gauge": ([_V, _P], ctypes.c_int),
+    "amgs_setup": ([_V], ctypes.c_int),
+    "amgs_setup_from_tv": ([_V, ctypes.POINTER(_P), ctypes.c_int], ctypes.c_int),
+    "amgs_solve": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
+    "amgs_solve_dev": ([_V, ctypes.c_int, _V, _V, _I, _P], ctypes.c_int),
+    "amgs_apply_D": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
+    "amgs_apply_D_dev": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _V, _V], ctypes.c_int),
+    "amgs_apply_P": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
+    "amgs_apply_Pt": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
+    "amgs_sap": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int], ctypes.c_int),
+    "amgs_kcycle": ([_V, ctypes.c_int, _P, _P], ctypes.c_int),
+    "amgs_get_tv": ([_V, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_get_links": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_gauge_metropolis": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64, _P, _P],
+                              ctypes.c_int),
+    "amgs_gaussian": ([ctypes.c_int64, ctypes.c_uint64, _P], ctypes.c_int),
+    "amgs_read_conf": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_write_conf": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_read_rhs": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_write_rhs": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_write_residuals": ([ctypes.c_char_p, _P, ctypes.c_int], ctypes.c_int),
+    "amgs_time_kernel": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "amgs_last_timings": ([_V, _P], ctypes.c_int),
+}
+
+
+def lib():
+    """Load libamgs.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AmgsError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def default_params(nx, nt, rows, m0=None, **kw):
+    """rows: parameters.dat rows (level, bx, bt, ntest, sapx, sapt), block COUNTS."""
+    p = Params()
+    lib().amgs_default_params(ctypes.byref(p))
+    p.nx, p.nt = nx, nt
+    p.levels = len(rows) + 1
+    for (l, bx, bt, nv, sx, st) in rows:
+        p.bx[l], p.bt[l], p.ntest[l], p.sapx[l], p.sapt[l] = bx, bt, nv, sx, st
+    if m0 is not None:
+        p.m0 = m0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def read_parameters(path, p=None):
+    p = p or Params()
+    if p.nx == 0:
+        lib().amgs_default_params(ctypes.byref(p))
+    rc = lib().amgs_read_parameters(path.encode(), ctypes.byref(p))
+    if rc != AMGS_OK:
+        raise AmgsError(f"amgs_read_parameters({path}) failed: {rc}")
+    return p
+
+
+def gauge_metropolis(nx, nt, beta, sweeps, seed):
+    """Quenched U(1) field from a cold start; returns (U [nx*nt, 2], plaquette)."""
+    U = np.zeros((nx * nt, 2), np.complex128)
+    plaq = ctypes.c_double(0)
+    rc = lib().amgs_gauge_metropolis(nx, nt, beta, sweeps, seed, _p(U), ctypes.byref(plaq))
+    if rc != AMGS_OK:
+        raise AmgsError(f"amgs_gauge_metropolis failed: {rc}")
+    return U, plaq.value
+
+
+def gaussian(n, seed):
+    out = np.zeros(n, np.complex128)
+    rc = lib().amgs_gaussian(n, seed, _p(out))
+    if rc != AMGS_OK:
+        raise AmgsError(f"amgs_gaussian failed: {rc}")
+    return out
+
+
+class Solver:
+    """One hierarchy on the GPU (one amgs context)."""
+
+    def __init__(self, params, U=None):
+        self.p = params
+        h = _V()
+        rc = lib().amgs_create(ctypes.byref(params), ctypes.byref(h))
+        if rc != AMGS_OK:
+            raise AmgsError(f"amgs_create failed ({rc})")
+        self.h = h
+        self.L = params.levels
+        self.shapes = [(params.nx * params.nt, 2)]
+        for l in range(self.L - 1):
+            self.shapes.append((params.bx[l] * params.bt[l], 2 * params.ntest[l]))
+        if U is not None:
+            self.set_gauge(U)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().amgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what, ok=(AMGS_OK,)):
+        if rc not in ok:
+            msg = lib().amgs_last_error(self.h)
+            raise AmgsError(f"{what}: {rc} {msg.decode() if msg else ''}")
+        return rc
+
+    def size(self, l):
+        return self.shapes[l][0] * self.shapes[l][1]
+
+    def set_gauge(self, U):
+        self.U = _c(U).reshape(-1)
+        self._chk(lib().amgs_set_gauge(self.h, _p(self.U)), "amgs_set_gauge")
+
+    def setup(self):
+        self._chk(lib().amgs_setup(self.h), "amgs_setup")
+
+    def setup_from_tv(self, tvs, smooth=True):
+        arrs = [_c(t).reshape(-1) for t in tvs]
+        ptrs = (_P * len(arrs))(*[_p(a) for a in arrs])
+        self._chk(lib().amgs_setup_from_tv(self.h, ptrs, 1 if smooth else 0), "amgs_setup_from_tv")
+
+    def solve(self, b, hist_cap=1000):
+        """b: [nrhs, n0] (or [n0]); returns x, iterations, true relres, histories."""
+        b = _c(b)
+        single = b.ndim == 1
+        b = b.reshape(-1, self.size(0))
+        nrhs = b.shape[0]
+        x = np.zeros_like(b)
+        it = np.zeros(nrhs, np.int32)
+        rr = np.zeros(nrhs)
+        hist = np.zeros((nrhs, hist_cap))
+        nh = np.zeros(nrhs, np.int32)
+        rc = lib().amgs_solve(self.h, nrhs, _p(b), _p(x), it.ctypes.data_as(_I), _p(rr), _p(hist), hist_cap,
+                              nh.ctypes.data_as(_I))
+        self._chk(rc, "amgs_solve", ok=(AMGS_OK, AMGS_NOT_CONVERGED))
+        hists = [hist[r, : nh[r]].copy() for r in range(nrhs)]
+        if single:
+            return x[0], int(it[0]), float(rr[0]), hists[0]
+        return x, it, rr, hists
+
+    def _apply(self, fn, nin, nout, x, *args, nrhs=1):
+        x = _c(x).reshape(-1)
+        out = np.zeros(nout * nrhs, np.complex128)
+        self._chk(fn(self.h, *args, _p(x), _p(out)), fn.__name__)
+        return out
+
+    def apply_D(self, level, x, dagger=False, nrhs=1):
+        n = self.size(level)
+        return self._apply(lib().amgs_apply_D, n, n, x, level, 1 if dagger else 0, nrhs, nrhs=nrhs)
+
+    def apply_P(self, level, xc, nrhs=1):
+        return self._apply(lib().amgs_apply_P, self.size(level + 1), self.size(level), xc, level, nrhs, nrhs=nrhs)
+
+    def apply_Pt(self, level, xf, nrhs=1):
+        return self._apply(lib().amgs_apply_Pt, self.size(level), self.size(level + 1), xf, level, nrhs, nrhs=nrhs)
+
+    def sap(self, level, v, x, nu, nrhs=1):
+        v = _c(v).reshape(-1)
+        x = _c(x).reshape(-1).copy()
+        self._chk(lib().amgs_sap(self.h, level, nrhs, _p(v), _p(x), nu), "amgs_sap")
+        return x
+
+    def kcycle(self, eta, nrhs=1):
+        return self._apply(lib().amgs_kcycle, self.size(0), self.size(0), eta, nrhs, nrhs=nrhs)
+
+    def tv(self, level, i):
+        out = np.zeros(self.size(level), np.complex128)
+        self._chk(lib().amgs_get_tv(self.h, level, i, _p(out)), "amgs_get_tv")
+        return out
+
+    def links(self, level, which, mu=0):
+        ns, d = self.shapes[level]
+        out = np.zeros(ns * d * d, np.complex128)
+        self._chk(lib().amgs_get_links(self.h, level, which, mu, _p(out)), "amgs_get_links")
+        return out.reshape(ns, d, d)
+
+    def time_kernel(self, kernel, nrhs=1, reps=50):
+        ms = ctypes.c_double(0)
+        self._chk(lib().amgs_time_kernel(self.h, kernel, nrhs, reps, ctypes.byref(ms)), "amgs_time_kernel")
+        return ms.value
+
+    def last_timings(self):
+        a = np.zeros(3)
+        lib().amgs_last_timings(self.h, _p(a))
+        return a
+
+
+class GaugeConf:
+    """Gauge field holder (include/gauge_conf.h:12-49): Conf[n][mu], n = x*Nt + t."""
+
+    def __init__(self, Nspace, Ntime):
+        self.Nx, self.Nt = Nspace, Ntime
+        self.Conf = np.ones((Nspace * Ntime, 2), np.complex128)
+
+    def read_conf(self, name):
+        rc = lib().amgs_read_conf(name.encode(), self.Nx, self.Nt, _p(self.Conf))
+        if rc != AMGS_OK:
+            raise AmgsError(f"read_conf({name}) failed: {rc}")
+
+    def setGconf(self, conf):
+        self.Conf = _c(conf).reshape(self.Nx * self.Nt, 2).copy()
+
+
+class FGMRES_AMG:
+    """FGMRES with the K-cycle preconditioner (include/amg.h:99-145).
+
+    The constructor builds the hierarchy (set-up phase) on the GPU;
+    ``fgmres(phi, x0, x)`` solves D x = phi from x0 = 0 and returns the
+    reference's iteration count.  ``params`` carries parameters.dat's rows.
+    """
+
+    def __init__(self, dim1, dim2, m, restarts, tol, GConf, m0, params, precision=PREC_FP64):
+        p = params
+        p.out_m, p.out_restarts, p.out_tol, p.m0 = m, restarts, tol, m0
+        p.precision = precision
+        assert dim1 == GConf.Nx * GConf.Nt and dim2 == 2
+        self.solver = Solver(p, GConf.Conf)
+        self.solver.setup()
+        self.residuals = []
+
+    def fgmres(self, phi, x0, x, print_message=False, save_res=False):
+        if np.any(np.asarray(x0) != 0):
+            raise AmgsError("the GPU solve starts from x0 = 0 as the reference's driver does (src/tests.cpp:69)")
+        sol, it, rr, hist = self.solver.solve(phi)
+        x[...] = sol.reshape(np.shape(x))
+        self.residuals = list(hist)
+        if print_message:
+            print(f"FGMRES finished with {it} iterations, true relative residual {rr:.3e}")
+        return it

@@ -1,0 +1,1014 @@
+// libamgs: context, device memory, orchestration and the C ABI (include/amgs.h).
+//
+// Execution model (one HIP device, one stream per context):
+//   - outer FGMRES (FP64) on the fine lattice: kernels per Arnoldi step, scalar
+//     work in small per-rhs kernels, one host synchronisation per step for the
+//     break test (src/fgmres.cpp:5-88);
+//   - preconditioner K-cycle(0): restriction, one persistent coarse kernel
+//     (levels >= 1, whole nested recursion on the device), prolongation, SAP on
+//     the fine lattice as a fixed sequence of predicated kernels;
+//   - every right-hand side of a batch follows its own control flow exactly as
+//     a single solve would; finished right-hand sides are skipped.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/amgs.h"
+#include "coarse.hpp"
+#include "common.hpp"
+#include "fine.hpp"
+#include "host_util.hpp"
+#include "outer.hpp"
+#include "setup.hpp"
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t e, const char* what, const char* file, int line)
+        : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")"),
+          code(e) {}
+};
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+namespace amgs {
+
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
+        bytes = std::max<size_t>(b, 256);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+static inline unsigned cdiv_u(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+struct Ctx;
+
+// ---------------------------------------------------------------------------
+// per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments
+// ---------------------------------------------------------------------------
+template <typename R> struct Engine {
+    Ctx* c = nullptr;
+    int cap = 0;
+    // hierarchy in precision R
+    const cplx<R>* U = nullptr;
+    const cplx<R>* tv[MAXL] = {};
+    const cplx<R>* links[MAXL] = {};
+    // fine SAP workspace
+    DevBuf Vt, xl, r, st_mem, skip, part_r, part_v, zero;
+    SapState st{};
+    int nchunk_max = 0;
+    // K-cycle(0) temporaries
+    DevBuf eta1, psi1, res0;
+    // coarse workspace
+    DevBuf cws;
+    CArgs<R> ca{};
+
+    void init(Ctx* ctx, int capacity);
+    void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
+    void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi);
+    void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
+                int nu = 0);
+    void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
+};
+
+struct Ctx {
+    amgs_params P{};
+    int L = 0;
+    Geo geo[MAXL]{};
+    std::vector<int> red[MAXL], black[MAXL];
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    bool gauge_set = false, setup_done = false;
+    // FP64 hierarchy (always), FP32 copy in mixed mode
+    DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], redl[MAXL], blackl[MAXL];
+    Engine<double> e64;
+    Engine<float> e32;
+    // outer FGMRES
+    DevBuf oV, oZ, ox, ob, or_, ow, ost_mem, oeta32, opsi32;
+    OuterState ost{};
+    int ocap = 0;
+    // io staging
+    DevBuf io_a, io_b;
+    double last_ms[3] = {0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    size_t nvec(int l) const { return (size_t)geo[l].ns * geo[l].dof; }
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+// ---------------------------------------------------------------------------
+// geometry (include/params.h:30-65, include/level.h:147-155, include/sap.h:64-78)
+// ---------------------------------------------------------------------------
+static void build_geometry(Ctx& c) {
+    const amgs_params& p = c.P;
+    if (p.levels < 2 || p.levels > MAXL) throw ApiError(AMGS_ERR_GEOMETRY, "levels must be in [2, 6]");
+    if (p.nx < 2 || p.nt < 2) throw ApiError(AMGS_ERR_GEOMETRY, "lattice too small");
+    c.L = p.levels;
+    for (int l = 0; l < c.L; l++) {
+        Geo& g = c.geo[l];
+        g.nx = l == 0 ? p.nx : p.bx[l - 1];
+        g.nt = l == 0 ? p.nt : p.bt[l - 1];
+        g.ns = g.nx * g.nt;
+        g.colors = l == 0 ? 1 : p.ntest[l - 1];
+        g.dof = 2 * g.colors;
+        if (l < c.L - 1) {
+            g.bx = p.bx[l];
+            g.bt = p.bt[l];
+            g.ntest = p.ntest[l];
+            g.sbx = p.sapx[l];
+            g.sbt = p.sapt[l];
+            if (g.ntest < 1 || g.ntest > 24) throw ApiError(AMGS_ERR_GEOMETRY, "ntest must be in [1, 24]");
+        } else {
+            g.bx = g.bt = g.ntest = g.sbx = g.sbt = 1;
+        }
+        if (g.nx < 1 || g.nt < 1 || g.bx < 1 || g.bt < 1 || g.sbx < 1 || g.sbt < 1 || g.nx % g.bx || g.nt % g.bt ||
+            g.nx % g.sbx || g.nt % g.sbt)
+            throw ApiError(AMGS_ERR_GEOMETRY, "level " + std::to_string(l) + ": block counts must divide the lattice");
+        g.nagg = 2 * g.bx * g.bt;
+        g.xel = g.nx / g.bx;
+        g.tel = g.nt / g.bt;
+        g.spb = g.xel * g.tel;
+        g.sxel = g.nx / g.sbx;
+        g.stel = g.nt / g.sbt;
+        g.sspb = g.sxel * g.stel;
+        g.nsb = g.sbx * g.sbt;
+        g.ncol = g.nsb != 1 ? g.nsb / 2 : 1;
+        // red/black lists (src/sap.cpp:26-36): red iff (X + T) even; block b
+        // goes to slot b / 2; a single block serves as both colours
+        c.red[l].assign(g.ncol, 0);
+        c.black[l].assign(g.ncol, 0);
+        for (int X = 0; X < g.sbx; X++)
+            for (int T = 0; T < g.sbt; T++) {
+                const int blk = X * g.sbt + T;
+                bool red = (g.sbt % 2 == 0) ? ((X % 2 == 0) ? blk % 2 == 0 : blk % 2 != 0) : blk % 2 == 0;
+                if (blk / 2 < g.ncol) (red ? c.red[l] : c.black[l])[blk / 2] = blk;
+            }
+    }
+    if (p.sap_m < 1 || p.sap_m > SAP_MAXM || p.kc_m < 1 || p.kc_m > KC_MAXM || p.crs_m < 1 || p.crs_m > CRS_MAXM ||
+        p.out_m < 1 || p.out_m > OUT_MAXM)
+        throw ApiError(AMGS_ERR_ARG, "restart lengths exceed the compiled limits (sap 8, kc 8, coarsest 24, outer 32)");
+    if (p.sap_restarts < 1 || p.kc_restarts < 1 || p.crs_restarts < 1 || p.out_restarts < 1)
+        throw ApiError(AMGS_ERR_ARG, "restart counts must be >= 1");
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
+    c = ctx;
+    cap = capacity;
+    const Geo& g0 = c->geo[0];
+    const size_t n0 = c->nvec(0);
+    const int m = c->P.sap_m;
+    Vt.alloc(sizeof(cplx<R>) * (m + 1) * cap * n0);
+    xl.alloc(sizeof(cplx<R>) * cap * n0);
+    r.alloc(sizeof(cplx<R>) * cap * n0);
+    res0.alloc(sizeof(cplx<R>) * cap * n0);
+    size_t nmax = n0;
+    for (int l = 1; l < c->L; l++) nmax = std::max(nmax, c->nvec(l));
+    zero.alloc(sizeof(cplx<R>) * cap * nmax);
+    HIP_CHECK(hipMemsetAsync(zero.p, 0, zero.bytes, c->stream));
+    nchunk_max = (int)cdiv_u(g0.sspb, FT);
+    const size_t items = (size_t)g0.ncol * cap, M = m;
+    // state layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_maxIt = take(4 * items), o_jb = take(4 * items), o_done = take(4 * items);
+    const size_t o_nb = take(8 * items), o_s = take(8 * items * (M + 1));
+    const size_t o_Hr = take(16 * items * (M + 1) * M), o_Ho = take(16 * items * (M + 1) * M);
+    const size_t o_G = take(16 * items * (M + 1) * (M + 1));
+    const size_t o_gp = take(16 * items * (M + 1)), o_gq = take(16 * items * (M + 1));
+    const size_t o_cn = take(16 * items * M), o_sn = take(16 * items * M);
+    const size_t o_qa = take(16 * items * nchunk_max * 2 * M), o_qb = take(8 * items * nchunk_max);
+    const size_t o_qr = take(16 * items * nchunk_max);
+    st_mem.alloc(off);
+    char* b = st_mem.as<char>();
+    st.maxIt = (int*)(b + o_maxIt);
+    st.jbreak = (int*)(b + o_jb);
+    st.done = (int*)(b + o_done);
+    st.normb = (double*)(b + o_nb);
+    st.s = (double*)(b + o_s);
+    st.Hraw = (c64*)(b + o_Hr);
+    st.Hrot = (c64*)(b + o_Ho);
+    st.G = (c64*)(b + o_G);
+    st.gpre = (c64*)(b + o_gp);
+    st.gpost = (c64*)(b + o_gq);
+    st.cn = (c64*)(b + o_cn);
+    st.sn = (c64*)(b + o_sn);
+    st.qa = (c64*)(b + o_qa);
+    st.qb = (double*)(b + o_qb);
+    st.qr = (double*)(b + o_qr);
+    skip.alloc(sizeof(int) * cap);
+    const unsigned ntile = cdiv_u(g0.ns, FT);
+    part_r.alloc(sizeof(double) * cap * ntile);
+    part_v.alloc(sizeof(double) * cap * ntile);
+    // K-cycle(0) and coarse workspace
+    const size_t n1 = c->nvec(1);
+    eta1.alloc(sizeof(cplx<R>) * cap * n1);
+    psi1.alloc(sizeof(cplx<R>) * cap * n1);
+    ca = CArgs<R>{};
+    ca.L = c->L;
+    ca.kc_m = c->P.kc_m;
+    ca.kc_restarts = c->P.kc_restarts;
+    ca.kc_tol = c->P.kc_tol;
+    ca.crs_m = c->P.crs_m;
+    ca.crs_restarts = c->P.crs_restarts;
+    ca.crs_tol = c->P.crs_tol;
+    ca.sap_m = c->P.sap_m;
+    ca.sap_restarts = c->P.sap_restarts;
+    ca.sap_gtol = c->P.sap_gmres_tol;
+    ca.sap_tol = c->P.sap_tol;
+    ca.nu1 = c->P.nu1;
+    ca.nu2 = c->P.nu2;
+    size_t wo = 0;
+    for (int l = 1; l < c->L; l++) {
+        CLevel<R>& lv = ca.lv[l];
+        const Geo& g = c->geo[l];
+        const size_t n = c->nvec(l);
+        lv.g = g;
+        lv.red = c->redl[l].as<int>();
+        lv.black = c->blackl[l].as<int>();
+        lv.kcV = wo; wo += (ca.kc_m + 1) * n;
+        lv.kcZ = wo; wo += ca.kc_m * n;
+        lv.kcR = wo; wo += n;
+        lv.kcW = wo; wo += n;
+        lv.kcDx = wo; wo += n;
+        if (l == c->L - 1) {
+            lv.crV = wo; wo += (ca.crs_m + 1) * n;
+            lv.crR = wo; wo += n;
+            lv.crW = wo; wo += n;
+            lv.crDx = wo; wo += n;
+        } else {
+            const size_t nc = c->nvec(l + 1);
+            lv.kDpsi = wo; wo += n;
+            lv.kR = wo; wo += n;
+            lv.kEta1 = wo; wo += nc;
+            lv.kPsi1 = wo; wo += nc;
+            lv.sR = wo; wo += n;
+            lv.sDx = wo; wo += n;
+            lv.blkStride = (size_t)(ca.sap_m + 3) * g.sspb * g.dof;
+            lv.blk = wo; wo += CW * lv.blkStride;
+        }
+    }
+    ca.ws_per_rhs = (wo + 63) & ~size_t(63);
+    cws.alloc(sizeof(cplx<R>) * ca.ws_per_rhs * cap);
+    ca.ws = cws.as<cplx<R>>();
+}
+
+template <typename R>
+void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out,
+                       size_t out_stride, int nu) {
+    if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
+    CArgs<R> a = ca;
+    a.mm = R(c->P.m0 + 2.0);
+    for (int l = 1; l < c->L; l++) {
+        a.lv[l].links = links[l];
+        a.lv[l].w = l < c->L - 1 ? tv[l] : nullptr;
+    }
+    a.mode = mode;
+    a.level = level;
+    a.nu = nu;
+    a.in = in;
+    a.out = out;
+    a.in_stride = in_stride;
+    a.out_stride = out_stride;
+    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), 0, c->stream, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out) {
+    const Geo& g = c->geo[0];
+    const Geo& g1 = c->geo[1];
+    const dim3 grid(g1.ns, nrhs);
+    const int nv = g.ntest;
+    if (nv <= 8)
+        hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+    else if (nv <= 16)
+        hipLaunchKernelGGL((k_restrict0<R, 16>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+    else
+        hipLaunchKernelGGL((k_restrict0<R, 24>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+    HIP_CHECK(hipGetLastError());
+}
+
+// SAP_C::SAP at level 0 (src/sap.cpp:64-174) for nrhs right-hand sides.
+template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu) {
+    if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
+    const Geo& g = c->geo[0];
+    const R mm = R(c->P.m0 + 2.0);
+    const unsigned ntile = cdiv_u(g.ns, FT);
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipMemsetAsync(skip.p, 0, sizeof(int) * nrhs, s));
+    if (nu <= 0) return;
+    hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
+    hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
+                       (double*)nullptr, skip.as<int>());
+    const int nchunk = (int)cdiv_u(g.sspb, FT);
+    SapArgs<R> a{};
+    a.g = g;
+    a.mm = mm;
+    a.U = U;
+    a.ncol = g.ncol;
+    a.nchunk = nchunk;
+    a.nrhs = nrhs;
+    a.m = c->P.sap_m;
+    a.tol = c->P.sap_gmres_tol;
+    a.rglob = r.as<cplx<R>>();
+    a.Vt = Vt.as<cplx<R>>();
+    a.xl = xl.as<cplx<R>>();
+    a.skip = skip.as<int>();
+    a.st = st;
+    const dim3 grid(g.ncol * nchunk, nrhs);
+    for (int it = 0; it < nu; it++) {
+        for (int colour = 0; colour < 2; colour++) {
+            a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
+            hipLaunchKernelGGL(k_sap_init<R>, grid, dim3(FT), 0, s, a);
+            for (int k = 0; k < c->P.sap_restarts; k++) {
+                for (int j = 0; j < a.m; j++) {
+                    hipLaunchKernelGGL(k_sap_arnoldi_a<R>, grid, dim3(FT), 0, s, a, j, k);
+                    hipLaunchKernelGGL(k_sap_arnoldi_b<R>, grid, dim3(FT), 0, s, a, j);
+                }
+                hipLaunchKernelGGL(k_sap_update<R>, grid, dim3(FT), 0, s, a);
+                if (k + 1 < c->P.sap_restarts) hipLaunchKernelGGL(k_sap_resid<R>, grid, dim3(FT), 0, s, a);
+            }
+            hipLaunchKernelGGL(k_sap_accum<R>, grid, dim3(FT), 0, s, a, x);
+            hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
+                               colour == 1 ? part_r.as<double>() : (double*)nullptr, skip.as<int>());
+        }
+        hipLaunchKernelGGL(k_sap_check, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, (const double*)part_r.as<double>(),
+                           (const double*)part_v.as<double>(), (int)ntile, nrhs, c->P.sap_tol, skip.as<int>());
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// One application psi = K(eta) at level 0 from psi = 0 (include/amg.h:134-144,
+// src/amg.cpp:135-178).
+template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi) {
+    const Geo& g = c->geo[0];
+    const size_t n0 = c->nvec(0), n1 = c->nvec(1);
+    hipStream_t s = c->stream;
+    const unsigned ntile = cdiv_u(g.ns, FT);
+    hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(nrhs * n0, 256)), dim3(256), 0, s, nrhs * n0, psi);
+    const cplx<R>* rr = eta;  // r = eta - D psi = eta for psi = 0
+    if (c->P.nu1 > 0) {
+        sap0(nrhs, eta, psi, c->P.nu1);
+        hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, R(c->P.m0 + 2.0), U, eta, psi,
+                           res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr);
+        rr = res0.as<cplx<R>>();
+    }
+    restrict0(nrhs, tv[0], rr, eta1.as<cplx<R>>());
+    coarse(CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
+    hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
+                       (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, 1);
+    if (c->P.nu2 > 0) sap0(nrhs, eta, psi, c->P.nu2);
+    HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy set-up (src/amg.cpp:3-46)
+// ---------------------------------------------------------------------------
+template <typename R> static void bind_engine(Ctx& c, Engine<R>& e, DevBuf& U, DevBuf* tv, DevBuf* links) {
+    e.U = U.as<cplx<R>>();
+    for (int l = 0; l < c.L; l++) {
+        e.tv[l] = tv[l].as<cplx<R>>();
+        e.links[l] = links[l].as<cplx<R>>();
+    }
+}
+
+static void convert64to32(Ctx& c, DevBuf& src, DevBuf& dst, size_t ncomplex) {
+    dst.alloc(sizeof(c32) * ncomplex);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(ncomplex, 256)), dim3(256), 0, c.stream, ncomplex,
+                       src.as<c64>(), dst.as<c32>());
+    HIP_CHECK(hipGetLastError());
+}
+
+static void smooth_level(Ctx& c, int l) {
+    Engine<double>& e = c.e64;
+    const Geo& g = c.geo[l];
+    const size_t n = c.nvec(l);
+    if (c.P.tv_sap_iters <= 0) return;
+    if (l == 0) {
+        e.sap0(g.ntest, e.zero.as<c64>(), c.tv64[0].as<c64>(), c.P.tv_sap_iters);
+    } else {
+        e.coarse(CM_SAP, l, g.ntest, e.zero.as<c64>(), n, c.tv64[l].as<c64>(), n, c.P.tv_sap_iters);
+    }
+}
+
+static void setup_common(Ctx& c, bool smooth) {
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    for (int l = 0; l < c.L - 1; l++) {
+        if (smooth) smooth_level(c, l);
+        const Geo& g = c.geo[l];
+        hipLaunchKernelGGL(k_orthonormalize, dim3(g.nagg), dim3(ST), 0, c.stream, g, c.tv64[l].as<c64>());
+        const Geo& gc = c.geo[l + 1];
+        LinkSrc src{g, l == 0, c.U64.as<c64>(), l == 0 ? nullptr : c.links64[l].as<c64>()};
+        hipLaunchKernelGGL(k_galerkin, dim3(gc.ns, 4), dim3(ST), 0, c.stream, src, (const c64*)c.tv64[l].as<c64>(),
+                           gc.ns, gc, c.links64[l + 1].as<c64>());
+        HIP_CHECK(hipGetLastError());
+    }
+    if (c.P.precision == AMGS_PREC_MIXED) {
+        convert64to32(c, c.U64, c.U32, c.nvec(0));
+        for (int l = 0; l < c.L; l++) {
+            if (l < c.L - 1) convert64to32(c, c.tv64[l], c.tv32[l], (size_t)c.geo[l].ntest * c.nvec(l));
+            if (l >= 1) convert64to32(c, c.links64[l], c.links32[l], (size_t)5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
+        }
+        bind_engine(c, c.e32, c.U32, c.tv32, c.links32);
+    }
+    c.sync();
+    c.setup_done = true;
+}
+
+static void setup_random(Ctx& c) {
+    TvStream rng(c.P.tv_seed);
+    std::vector<double> host;
+    const Geo& g0 = c.geo[0];
+    host.resize((size_t)2 * g0.ntest * c.nvec(0));
+    rng.fill(host.data(), (size_t)g0.ntest * c.nvec(0));
+    HIP_CHECK(hipMemcpyAsync(c.tv64[0].p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    // levels 1..L-2: P^dagger of the raw level-(l-1) vectors, then fresh draws
+    for (int l = 1; l < c.L - 1; l++) {
+        const Geo& g = c.geo[l];
+        const Geo& gp = c.geo[l - 1];
+        const size_t n = c.nvec(l), np = c.nvec(l - 1);
+        const int nr = std::min(g.ntest, gp.ntest);
+        if (l == 1)
+            c.e64.restrict0(nr, c.tv64[0].as<c64>(), c.tv64[0].as<c64>(), c.tv64[1].as<c64>());
+        else
+            c.e64.coarse(CM_RESTRICT, l - 1, nr, c.tv64[l - 1].as<c64>(), np, c.tv64[l].as<c64>(), n);
+        if (g.ntest > gp.ntest) {
+            host.assign((size_t)2 * (g.ntest - gp.ntest) * n, 0.0);
+            rng.fill(host.data(), (size_t)(g.ntest - gp.ntest) * n);
+            HIP_CHECK(hipMemcpyAsync(c.tv64[l].as<c64>() + (size_t)gp.ntest * n, host.data(),
+                                     host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        }
+        c.sync();  // host buffer reuse
+    }
+    setup_common(c, true);
+}
+
+// ---------------------------------------------------------------------------
+// outer FGMRES (FP64)
+// ---------------------------------------------------------------------------
+static void outer_alloc(Ctx& c, int cap) {
+    const size_t n0 = c.nvec(0);
+    const int M = c.P.out_m;
+    c.ocap = cap;
+    c.oV.alloc(sizeof(c64) * (M + 1) * cap * n0);
+    c.oZ.alloc(sizeof(c64) * M * cap * n0);
+    c.ox.alloc(sizeof(c64) * cap * n0);
+    c.ob.alloc(sizeof(c64) * cap * n0);
+    c.or_.alloc(sizeof(c64) * cap * n0);
+    c.ow.alloc(sizeof(c64) * cap * n0);
+    if (c.P.precision == AMGS_PREC_MIXED) {
+        c.oeta32.alloc(sizeof(c32) * cap * n0);
+        c.opsi32.alloc(sizeof(c32) * cap * n0);
+    }
+    const int ntile = (int)cdiv_u(c.geo[0].ns, FT);
+    const int hcap = M * c.P.out_restarts;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_i = take(4 * cap * 6), o_nb = take(8 * cap), o_s = take(8 * cap * (M + 1)),
+                 o_h = take(8 * (size_t)cap * hcap), o_R = take(16 * (size_t)cap * M * M),
+                 o_G = take(16 * (size_t)cap * (M + 1) * (M + 1)), o_g = take(16 * cap * (M + 1)),
+                 o_cn = take(16 * cap * M), o_sn = take(16 * cap * M), o_eta = take(16 * cap * M),
+                 o_qa = take(16 * (size_t)cap * ntile * (2 * M + 1)), o_qb = take(8 * (size_t)cap * ntile),
+                 o_qr = take(16 * (size_t)cap * ntile);
+    c.ost_mem.alloc(off);
+    char* b = c.ost_mem.as<char>();
+    OuterState& s = c.ost;
+    int* ib = (int*)(b + o_i);
+    s.maxIt = ib;
+    s.jbreak = ib + cap;
+    s.done = ib + 2 * cap;
+    s.active = ib + 3 * cap;
+    s.iters = ib + 4 * cap;
+    s.nhist = ib + 5 * cap;
+    s.normb = (double*)(b + o_nb);
+    s.s = (double*)(b + o_s);
+    s.hist = (double*)(b + o_h);
+    s.R = (c64*)(b + o_R);
+    s.G = (c64*)(b + o_G);
+    s.g = (c64*)(b + o_g);
+    s.cn = (c64*)(b + o_cn);
+    s.sn = (c64*)(b + o_sn);
+    s.eta = (c64*)(b + o_eta);
+    s.qa = (c64*)(b + o_qa);
+    s.qb = (double*)(b + o_qb);
+    s.qr = (double*)(b + o_qr);
+    s.M = M;
+    s.ntile = ntile;
+    s.hcap = hcap;
+}
+
+// preconditioner Z_j = K(V_j) for all rhs of the batch
+static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
+    const size_t n0 = c.nvec(0), tot = nrhs * n0;
+    if (c.P.precision == AMGS_PREC_MIXED) {
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
+                           c.oeta32.as<c32>());
+        c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                           (const c32*)c.opsi32.as<c32>(), zj);
+    } else {
+        c.e64.kcycle0(nrhs, vj, zj);
+    }
+}
+
+// FGMRES_AMG::fgmres(b, 0, x) on device buffers ob -> ox (src/fgmres.cpp:5-88)
+static void outer_solve(Ctx& c, int nrhs) {
+    const Geo& g = c.geo[0];
+    const size_t n0 = c.nvec(0), vstride = (size_t)c.ocap * n0;
+    const int M = c.P.out_m;
+    const double mm = c.P.m0 + 2.0, tol = c.P.out_tol;
+    const unsigned ntile = cdiv_u(g.ns, FT);
+    hipStream_t s = c.stream;
+    OuterState& st = c.ost;
+    c64* V = c.oV.as<c64>();
+    c64* Z = c.oZ.as<c64>();
+    c64* x = c.ox.as<c64>();
+    c64* b = c.ob.as<c64>();
+    c64* r = c.or_.as<c64>();
+    c64* w = c.ow.as<c64>();
+    HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nrhs * n0, s));
+    HIP_CHECK(hipMemsetAsync(st.done, 0, sizeof(int) * nrhs, s));
+    HIP_CHECK(hipMemsetAsync(st.nhist, 0, sizeof(int) * nrhs, s));
+    HIP_CHECK(hipMemsetAsync(st.iters, 0, sizeof(int) * nrhs, s));
+    const dim3 g2(ntile, nrhs);
+    std::vector<int> flags(nrhs);
+    hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
+                       (const c64*)x, r, st);
+    for (int k = 0; k < c.P.out_restarts; k++) {
+        hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k, tol, M);
+        HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
+        c.sync();
+        if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
+        hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0);
+        for (int j = 0; j < M; j++) {
+            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride);
+            hipLaunchKernelGGL(k_outer_a, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(),
+                               (const c64*)(Z + (size_t)j * vstride), (const c64*)V, w, st, j, vstride);
+            hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
+            hipLaunchKernelGGL(k_outer_b, g2, dim3(FT), 0, s, g.ns, (const c64*)V, w, st, j, vstride);
+            hipLaunchKernelGGL(k_outer_fin, dim3(nrhs), dim3(FT), 0, s, st, j, tol);
+            if (j + 1 < M)
+                hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)w, V + (size_t)(j + 1) * vstride,
+                                   st, j + 1);
+            HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
+            c.sync();
+            if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
+        }
+        hipLaunchKernelGGL(k_outer_backsolve, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs);
+        hipLaunchKernelGGL(k_outer_update, g2, dim3(FT), 0, s, g.ns, (const c64*)Z, x, st, vstride);
+        hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
+                           (const c64*)x, r, st);
+        if (k + 1 == c.P.out_restarts)
+            hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k + 1, tol, M);
+    }
+    hipLaunchKernelGGL(k_outer_final, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs, c.P.out_restarts, M);
+    HIP_CHECK(hipGetLastError());
+}
+
+// true relative residual |b - D x| / |b| in FP64
+static void true_relres(Ctx& c, int nrhs, double* out) {
+    const Geo& g = c.geo[0];
+    const size_t n0 = c.nvec(0);
+    const unsigned ntile = cdiv_u(g.ns, FT);
+    DevBuf pr, pb;
+    pr.alloc(sizeof(double) * nrhs * ntile);
+    pb.alloc(sizeof(double) * nrhs * ntile);
+    hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                       (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), (const c64*)c.ox.as<c64>(),
+                       c.ow.as<c64>(), pr.as<double>(), (const int*)nullptr);
+    hipLaunchKernelGGL(k_norm_part<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g.ns,
+                       (const c64*)c.ob.as<c64>(), pb.as<double>());
+    std::vector<double> hr((size_t)nrhs * ntile), hb((size_t)nrhs * ntile);
+    HIP_CHECK(hipMemcpyAsync(hr.data(), pr.p, hr.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hb.data(), pb.p, hb.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    (void)n0;
+    for (int r = 0; r < nrhs; r++) {
+        double a = 0, bb = 0;
+        for (unsigned t = 0; t < ntile; t++) {
+            a += hr[(size_t)r * ntile + t];
+            bb += hb[(size_t)r * ntile + t];
+        }
+        out[r] = bb > 0 ? std::sqrt(a / bb) : 0.0;
+    }
+}
+
+}  // namespace amgs
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace amgs;
+
+struct amgs_ctx {
+    Ctx c;
+};
+
+#define API_BEGIN try {
+#define API_END                                             \
+    }                                                       \
+    catch (const ApiError& e) {                             \
+        ctx->c.err = e.what();                              \
+        return e.code;                                      \
+    }                                                       \
+    catch (const HipError& e) {                             \
+        ctx->c.err = e.what();                              \
+        return AMGS_ERR_HIP;                                \
+    }                                                       \
+    catch (const std::exception& e) {                       \
+        ctx->c.err = e.what();                              \
+        return AMGS_ERR_STATE;                              \
+    }
+
+static void need(bool cond, int code, const char* msg) {
+    if (!cond) throw ApiError(code, msg);
+}
+
+static void h2d(Ctx& c, DevBuf& dst, const double* src, size_t ncomplex) {
+    dst.alloc(ncomplex * sizeof(c64));
+    HIP_CHECK(hipMemcpyAsync(dst.p, src, ncomplex * sizeof(c64), hipMemcpyHostToDevice, c.stream));
+}
+
+extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
+    if (!p || !out) return AMGS_ERR_ARG;
+    *out = nullptr;
+    auto* ctx = new amgs_ctx();
+    try {
+        Ctx& c = ctx->c;
+        c.P = *p;
+        if (c.P.max_rhs < 1) c.P.max_rhs = 1;
+        build_geometry(c);
+        HIP_CHECK(hipGetDevice(&c.device));
+        HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&c.ev0));
+        HIP_CHECK(hipEventCreate(&c.ev1));
+        const size_t n0 = c.nvec(0);
+        c.U64.alloc(sizeof(c64) * n0);
+        for (int l = 0; l < c.L; l++) {
+            c.redl[l].alloc(sizeof(int) * c.red[l].size());
+            c.blackl[l].alloc(sizeof(int) * c.black[l].size());
+            HIP_CHECK(hipMemcpyAsync(c.redl[l].p, c.red[l].data(), sizeof(int) * c.red[l].size(), hipMemcpyHostToDevice, c.stream));
+            HIP_CHECK(hipMemcpyAsync(c.blackl[l].p, c.black[l].data(), sizeof(int) * c.black[l].size(),
+                                     hipMemcpyHostToDevice, c.stream));
+            if (l < c.L - 1) c.tv64[l].alloc(sizeof(c64) * c.geo[l].ntest * c.nvec(l));
+            if (l >= 1) c.links64[l].alloc(sizeof(c64) * 5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
+        }
+        int ntest_max = 1;
+        for (int l = 0; l < c.L - 1; l++) ntest_max = std::max(ntest_max, c.geo[l].ntest);
+        const bool mixed = c.P.precision == AMGS_PREC_MIXED;
+        c.e64.init(&c, mixed ? ntest_max : std::max(ntest_max, c.P.max_rhs));
+        if (mixed) c.e32.init(&c, c.P.max_rhs);
+        outer_alloc(c, c.P.max_rhs);
+        c.sync();
+    } catch (const ApiError& e) {
+        std::fprintf(stderr, "amgs_create: %s\n", e.what());
+        int code = e.code;
+        delete ctx;
+        return code;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "amgs_create: %s\n", e.what());
+        delete ctx;
+        return AMGS_ERR_HIP;
+    }
+    *out = ctx;
+    return AMGS_OK;
+}
+
+extern "C" void amgs_destroy(amgs_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    Ctx& c = ctx->c;
+    hipStream_t s = c.stream;
+    hipEvent_t a = c.ev0, b = c.ev1;
+    delete ctx;  // frees device buffers
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" const char* amgs_last_error(const amgs_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+extern "C" int amgs_set_gauge(amgs_ctx* ctx, const double* U) {
+    if (!ctx || !U) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    const size_t n = c.nvec(0);
+    for (size_t i = 0; i < 2 * n; i++) need(std::isfinite(U[i]), AMGS_ERR_NAN, "gauge field holds NaN/Inf");
+    h2d(c, c.U64, U, n);
+    c.sync();
+    c.gauge_set = true;
+    c.setup_done = false;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_setup(amgs_ctx* ctx) {
+    if (!ctx) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    auto t0 = std::chrono::steady_clock::now();
+    setup_random(c);
+    c.last_ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_setup_from_tv(amgs_ctx* ctx, const double* const* tv, int smooth) {
+    if (!ctx || !tv) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    for (int l = 0; l < c.L - 1; l++) {
+        need(tv[l] != nullptr, AMGS_ERR_ARG, "missing test vectors");
+        HIP_CHECK(hipMemcpyAsync(c.tv64[l].p, tv[l], sizeof(c64) * c.geo[l].ntest * c.nvec(l), hipMemcpyHostToDevice,
+                                 c.stream));
+    }
+    setup_common(c, smooth != 0);
+    return AMGS_OK;
+    API_END
+}
+
+static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist, int hist_cap, int* nhist) {
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
+    auto t0 = std::chrono::steady_clock::now();
+    outer_solve(c, nrhs);
+    c.sync();
+    c.last_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c.last_ms[0] = c.last_ms[1] + c.last_ms[2];
+    std::vector<int> it(nrhs), nh(nrhs);
+    HIP_CHECK(hipMemcpy(it.data(), c.ost.iters, sizeof(int) * nrhs, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(nh.data(), c.ost.nhist, sizeof(int) * nrhs, hipMemcpyDeviceToHost));
+    if (iters) std::copy(it.begin(), it.end(), iters);
+    if (hist && hist_cap > 0) {
+        std::vector<double> h((size_t)nrhs * c.ost.hcap);
+        HIP_CHECK(hipMemcpy(h.data(), c.ost.hist, h.size() * 8, hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrhs; r++)
+            for (int i = 0; i < std::min(hist_cap, std::min(nh[r], c.ost.hcap)); i++)
+                hist[(size_t)r * hist_cap + i] = h[(size_t)r * c.ost.hcap + i];
+    }
+    if (nhist)
+        for (int r = 0; r < nrhs; r++) nhist[r] = std::min(nh[r], hist_cap);
+    std::vector<double> rr(nrhs);
+    true_relres(c, nrhs, rr.data());
+    if (relres) std::copy(rr.begin(), rr.end(), relres);
+    bool all = true;
+    for (int r = 0; r < nrhs; r++) all = all && it[r] < c.P.out_restarts * c.P.out_m;
+    // an exact solve of iteration count restarts*m cannot be told apart; use the residual
+    all = true;
+    for (int r = 0; r < nrhs; r++) all = all && rr[r] < c.P.out_tol * 1.0000001;
+    return all ? AMGS_OK : AMGS_NOT_CONVERGED;
+}
+
+extern "C" int amgs_solve(amgs_ctx* ctx, int nrhs, const double* b, double* x, int* iters, double* relres, double* hist,
+                          int hist_cap, int* nhist) {
+    if (!ctx || !b || !x) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
+    const size_t n0 = c.nvec(0);
+    for (size_t i = 0; i < 2 * n0 * nrhs; i++) need(std::isfinite(b[i]), AMGS_ERR_NAN, "right-hand side holds NaN/Inf");
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, b, sizeof(c64) * n0 * nrhs, hipMemcpyHostToDevice, c.stream));
+    int rc = solve_impl(c, nrhs, iters, relres, hist, hist_cap, nhist);
+    HIP_CHECK(hipMemcpy(x, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToHost));
+    return rc;
+    API_END
+}
+
+extern "C" int amgs_solve_dev(amgs_ctx* ctx, int nrhs, const void* b_dev, void* x_dev, int* iters, double* relres) {
+    if (!ctx || !b_dev || !x_dev) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
+    const size_t n0 = c.nvec(0);
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, b_dev, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToDevice, c.stream));
+    int rc = solve_impl(c, nrhs, iters, relres, nullptr, 0, nullptr);
+    HIP_CHECK(hipMemcpyAsync(x_dev, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToDevice, c.stream));
+    c.sync();
+    return rc;
+    API_END
+}
+
+// ---- component entry points (FP64 engine) ----
+static size_t level_n(Ctx& c, int l) { return c.nvec(l); }
+
+extern "C" int amgs_apply_D_dev(amgs_ctx* ctx, int level, int dagger, int nrhs, const void* in, void* out) {
+    if (!ctx || !in || !out) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(level >= 0 && level < c.L, AMGS_ERR_ARG, "level out of range");
+    need(nrhs >= 1, AMGS_ERR_ARG, "nrhs must be >= 1");
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    const Geo& g = c.geo[level];
+    if (level == 0) {
+        const dim3 grid(cdiv_u(g.ns, FT), nrhs);
+        if (dagger)
+            hipLaunchKernelGGL((k_wilson<double, true>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out);
+        else
+            hipLaunchKernelGGL((k_wilson<double, false>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out);
+        HIP_CHECK(hipGetLastError());
+    } else {
+        need(!dagger, AMGS_ERR_ARG, "D^dagger only on the fine level");
+        need(c.setup_done, AMGS_ERR_STATE, "coarse operators need amgs_setup");
+        need(nrhs <= c.e64.cap, AMGS_ERR_ARG, "nrhs exceeds capacity");
+        const size_t n = c.nvec(level);
+        c.e64.coarse(CM_DOP, level, nrhs, (const c64*)in, n, (c64*)out, n);
+    }
+    c.sync();
+    return AMGS_OK;
+    API_END
+}
+
+// host-pointer wrapper: copy in, run, copy out
+template <typename F>
+static int host_wrap(amgs_ctx* ctx, size_t nin, const double* in, size_t nout, double* out, F&& f, double* inout = nullptr) {
+    Ctx& c = ctx->c;
+    c.io_a.alloc(sizeof(c64) * std::max<size_t>(nin, 1));
+    c.io_b.alloc(sizeof(c64) * std::max<size_t>(nout, 1));
+    if (in) HIP_CHECK(hipMemcpyAsync(c.io_a.p, in, sizeof(c64) * nin, hipMemcpyHostToDevice, c.stream));
+    if (inout) HIP_CHECK(hipMemcpyAsync(c.io_b.p, inout, sizeof(c64) * nout, hipMemcpyHostToDevice, c.stream));
+    f(c.io_a.as<c64>(), c.io_b.as<c64>());
+    HIP_CHECK(hipMemcpyAsync(out ? out : inout, c.io_b.p, sizeof(c64) * nout, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return AMGS_OK;
+}
+
+extern "C" int amgs_apply_D(amgs_ctx* ctx, int level, int dagger, int nrhs, const double* in, double* out) {
+    if (!ctx || !in || !out) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(level >= 0 && level < c.L && nrhs >= 1, AMGS_ERR_ARG, "bad level / nrhs");
+    const size_t n = level_n(c, level) * nrhs;
+    int rc = AMGS_OK;
+    host_wrap(ctx, n, in, n, out, [&](c64* a, c64* b) { rc = amgs_apply_D_dev(ctx, level, dagger, nrhs, a, b); });
+    return rc;
+    API_END
+}
+
+extern "C" int amgs_apply_P(amgs_ctx* ctx, int level, int nrhs, const double* coarse, double* fine) {
+    if (!ctx || !coarse || !fine) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
+    const size_t nf = c.nvec(level), nc = c.nvec(level + 1);
+    return host_wrap(ctx, nc * nrhs, coarse, nf * nrhs, fine, [&](c64* a, c64* b) {
+        if (level == 0) {
+            const Geo& g = c.geo[0];
+            hipLaunchKernelGGL(k_prolong0<double>, dim3(cdiv_u(g.ns, FT), nrhs), dim3(FT), 0, c.stream, g, g.ntest,
+                               (const c64*)c.tv64[0].as<c64>(), (const c64*)a, b, c.geo[1].dof, c.geo[1].ns, 0);
+            HIP_CHECK(hipGetLastError());
+        } else {
+            c.e64.coarse(CM_PROLONG, level, nrhs, a, nc, b, nf);
+        }
+    });
+    API_END
+}
+
+extern "C" int amgs_apply_Pt(amgs_ctx* ctx, int level, int nrhs, const double* fine, double* coarse) {
+    if (!ctx || !coarse || !fine) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
+    const size_t nf = c.nvec(level), nc = c.nvec(level + 1);
+    return host_wrap(ctx, nf * nrhs, fine, nc * nrhs, coarse, [&](c64* a, c64* b) {
+        if (level == 0) c.e64.restrict0(nrhs, c.tv64[0].as<c64>(), a, b);
+        else c.e64.coarse(CM_RESTRICT, level, nrhs, a, nf, b, nc);
+    });
+    API_END
+}
+
+extern "C" int amgs_sap(amgs_ctx* ctx, int level, int nrhs, const double* v, double* x, int nu) {
+    if (!ctx || !v || !x) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
+    need(level == 0 || c.setup_done, AMGS_ERR_STATE, "coarse SAP needs amgs_setup");
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    const size_t n = c.nvec(level) * nrhs;
+    return host_wrap(ctx, n, v, n, nullptr, [&](c64* a, c64* b) {
+        if (level == 0) c.e64.sap0(nrhs, a, b, nu);
+        else c.e64.coarse(CM_SAP, level, nrhs, a, c.nvec(level), b, c.nvec(level), nu);
+    }, x);
+    API_END
+}
+
+extern "C" int amgs_kcycle(amgs_ctx* ctx, int nrhs, const double* eta, double* psi) {
+    if (!ctx || !eta || !psi) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad nrhs");
+    const size_t n = c.nvec(0) * nrhs;
+    return host_wrap(ctx, n, eta, n, psi, [&](c64* a, c64* b) { c.e64.kcycle0(nrhs, a, b); });
+    API_END
+}
+
+extern "C" int amgs_get_tv(amgs_ctx* ctx, int level, int i, double* out) {
+    if (!ctx || !out) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(level >= 0 && level < c.L - 1 && i >= 0 && i < c.geo[level].ntest, AMGS_ERR_ARG, "bad level / index");
+    const size_t n = c.nvec(level);
+    HIP_CHECK(hipMemcpy(out, c.tv64[level].as<c64>() + (size_t)i * n, sizeof(c64) * n, hipMemcpyDeviceToHost));
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_get_links(amgs_ctx* ctx, int level, int which, int mu, double* out) {
+    if (!ctx || !out) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(level >= 1 && level < c.L && which >= 0 && which <= 2 && mu >= 0 && mu <= 1, AMGS_ERR_ARG, "bad arguments");
+    const Geo& g = c.geo[level];
+    const int D = g.dof;
+    const int k = which == 0 ? 0 : (which == 1 ? 1 + mu : 3 + mu);
+    std::vector<c64> h((size_t)g.ns * D * D);
+    HIP_CHECK(hipMemcpy(h.data(), c.links64[level].as<c64>() + (size_t)k * g.ns * D * D, h.size() * sizeof(c64),
+                        hipMemcpyDeviceToHost));
+    // device layout [x][col][row] -> [x][row][col]
+    for (int x = 0; x < g.ns; x++)
+        for (int r = 0; r < D; r++)
+            for (int col = 0; col < D; col++) {
+                const c64 v = h[((size_t)x * D + col) * D + r];
+                out[2 * (((size_t)x * D + r) * D + col)] = v.re;
+                out[2 * (((size_t)x * D + r) * D + col) + 1] = v.im;
+            }
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_time_kernel(amgs_ctx* ctx, int kernel, int nrhs, int reps, double* avg_ms) {
+    if (!ctx || !avg_ms || reps < 1) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    need(nrhs >= 1 && nrhs <= c.ocap, AMGS_ERR_ARG, "nrhs exceeds capacity");
+    const Geo& g = c.geo[0];
+    const dim3 grid(cdiv_u(g.ns, FT), nrhs);
+    auto launch = [&]() {
+        if (kernel == 0)
+            hipLaunchKernelGGL((k_wilson<double, false>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                               (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), c.ow.as<c64>());
+        else if (kernel == 1) {
+            need(c.P.precision == AMGS_PREC_MIXED && c.setup_done, AMGS_ERR_STATE, "FP32 operator needs mixed setup");
+            hipLaunchKernelGGL((k_wilson<float, false>), grid, dim3(FT), 0, c.stream, g, float(c.P.m0 + 2.0),
+                               (const c32*)c.U32.as<c32>(), (const c32*)c.oeta32.as<c32>(), c.opsi32.as<c32>());
+        } else
+            throw ApiError(AMGS_ERR_ARG, "unknown kernel id");
+    };
+    launch();
+    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < reps; i++) launch();
+    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    HIP_CHECK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    *avg_ms = ms / reps;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_last_timings(amgs_ctx* ctx, double* ms3) {
+    if (!ctx || !ms3) return AMGS_ERR_ARG;
+    for (int i = 0; i < 3; i++) ms3[i] = ctx->c.last_ms[i];
+    return AMGS_OK;
+}

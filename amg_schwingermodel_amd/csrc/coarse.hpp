@@ -1,0 +1,552 @@
+// Coarse levels (l >= 1): one persistent workgroup per right-hand side runs the
+// whole nested solve of the reference on the device (src/amg.cpp:135-178,
+// include/amg.h:19-40, src/fgmres.cpp:5-115, src/sap.cpp:64-174,
+// src/level.cpp:161-237,304-368): K-cycle FGMRES, K-cycle recursion, SAP with
+// per-block GMRES, coarsest GMRES.  Control flow never returns to the host.
+//
+// Layout: a level-l vector is [ns][dof]; coarse links are [5][ns][col][row]
+// (k = A, B_0, B_1, C_0, C_1) so that the lanes of one site (consecutive rows)
+// read consecutive addresses.  Test vectors are [ntest][ns*dof].
+//
+// Workgroup-level reductions are deterministic (fixed lane butterfly, waves in
+// order).  SAP blocks of one colour run concurrently, one wavefront per block,
+// each with its own control flow and wave-level reductions (no block barrier).
+#pragma once
+#include "common.hpp"
+
+namespace amgs {
+
+constexpr int CT = 512;        // threads per coarse workgroup
+constexpr int CW = CT / 64;    // wavefronts per coarse workgroup
+constexpr int MAXL = 6;        // maximum number of levels
+constexpr int KC_MAXM = 8;     // max restart length of K-cycle FGMRES and SAP GMRES
+constexpr int CRS_MAXM = 24;   // max restart length of the coarsest GMRES
+
+enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4, CM_KCYCLE = 5 };
+
+template <typename R> struct CLevel {
+    Geo g;
+    const cplx<R>* links;  // [5][ns][dof][dof], column-major per site
+    const cplx<R>* w;      // test vectors [ntest][ns*dof]  (levels < L-1)
+    const int* red;        // SAP colour lists [ncol]
+    const int* black;
+    // per-rhs workspace offsets (complex elements)
+    size_t kcV, kcZ, kcR, kcW, kcDx;   // FGMRES of the K-cycle at this level
+    size_t crV, crR, crW, crDx;        // coarsest GMRES (l == L-1)
+    size_t kDpsi, kR, kEta1, kPsi1;    // k_cycle temporaries (l < L-1)
+    size_t sR, sDx;                    // SAP residual / D x
+    size_t blk;                        // CW per-wave block workspaces
+    size_t blkStride;                  // (sap_m + 3) * sspb * dof
+};
+
+template <typename R> struct CArgs {
+    int L;
+    R mm;
+    CLevel<R> lv[MAXL];
+    int kc_m, kc_restarts;
+    double kc_tol;
+    int crs_m, crs_restarts;
+    double crs_tol;
+    int sap_m, sap_restarts;
+    double sap_gtol, sap_tol;
+    int nu1, nu2;
+    cplx<R>* ws;
+    size_t ws_per_rhs;
+    int mode, level, nu;
+    const cplx<R>* in;
+    cplx<R>* out;
+    size_t in_stride, out_stride;
+};
+
+template <int M> struct FgScal {
+    c64 R[M][M];        // rotated Hessenberg (upper triangle)
+    c64 G[M + 1][M + 1];  // G[i][k] = <V_i, V_k>, k < i
+    c64 g[M + 1], cn[M], sn[M], eta[M];
+    c64 col[M + 1], dq[2 * M + 1];
+    int brk;
+};
+struct WaveScal {
+    c64 R[KC_MAXM][KC_MAXM];
+    c64 g[KC_MAXM + 1], cn[KC_MAXM], sn[KC_MAXM], eta[KC_MAXM];
+};
+struct CLds {
+    c64 red[CT];
+    FgScal<KC_MAXM> kc[MAXL];
+    FgScal<CRS_MAXM> crs;
+    WaveScal wv[CW];
+};
+
+// ---------------------------------------------------------------------------
+// workgroup-level vector primitives (all threads call them uniformly)
+// ---------------------------------------------------------------------------
+template <typename R> struct CK {
+    const CArgs<R>& A;
+    cplx<R>* W;  // this rhs's workspace
+    CLds& S;
+};
+
+template <typename R> __device__ __forceinline__ c64 c_dot(CLds& S, const cplx<R>* a, const cplx<R>* b, int n) {
+    c64 q[1] = {{0, 0}};
+    for (int e = threadIdx.x; e < n; e += CT) q[0] += mulc(cvt<double>(a[e]), cvt<double>(b[e]));
+    __syncthreads();
+    block_sum<1>(q, S.red);
+    return q[0];
+}
+
+// out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))
+template <typename R> __device__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out) {
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int D = g.dof, n = g.ns * D;
+    const size_t msz = (size_t)D * D, ksz = (size_t)g.ns * msz;
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += CT) {
+        const int x = e / D, r = e - x * D;
+        cplx<R> acc = A.mm * in[e];
+        const cplx<R>* a0 = L.links + x * msz + r;
+        const cplx<R>* v0 = in + (size_t)x * D;
+        for (int c = 0; c < D; c++) acc -= a0[(size_t)c * D] * v0[c];
+        for (int mu = 0; mu < 2; mu++) {
+            const int xr = nb_right(x, mu, g.nx, g.nt), xl = nb_left(x, mu, g.nx, g.nt);
+            const cplx<R>* bm = L.links + (1 + mu) * ksz + x * msz + r;
+            const cplx<R>* cm = L.links + (3 + mu) * ksz + x * msz + r;
+            const cplx<R>* vr = in + (size_t)xr * D;
+            const cplx<R>* vl = in + (size_t)xl * D;
+            cplx<R> sr = czero<R>(), sl = czero<R>();
+            for (int c = 0; c < D; c++) {
+                sr += bm[(size_t)c * D] * vr[c];
+                sl += cm[(size_t)c * D] * vl[c];
+            }
+            if (wraps_right(x, mu, g.nt)) sr = -sr;
+            if (wraps_left(x, mu, g.nt)) sl = -sl;
+            acc -= sr + sl;
+        }
+        out[e] = acc;
+    }
+    __syncthreads();
+}
+
+// out = P^dagger_l vf (src/level.cpp:212-237); one thread per coarse entry,
+// summed over the aggregate in the reference order (x, t, colour).
+template <typename R> __device__ void c_restrict(const CArgs<R>& A, int l, const cplx<R>* vf, cplx<R>* out) {
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int Dc = A.lv[l + 1].g.dof, nc = A.lv[l + 1].g.ns * Dc, D = g.dof;
+    const size_t nvec = (size_t)g.ns * D;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc; e += CT) {
+        const int X = e / Dc, q = e - X * Dc, cc = q >> 1, s = q & 1;
+        c64 acc = {0, 0};
+        const cplx<R>* w = L.w + cc * nvec;
+        for (int m = 0; m < g.spb; m++) {
+            const int n = agg_site(g, X, m);
+            for (int c = 0; c < g.colors; c++) {
+                const size_t i = (size_t)n * D + 2 * c + s;
+                acc += mulc(cvt<double>(vf[i]), cvt<double>(w[i]));
+            }
+        }
+        out[e] = cvt<R>(acc);
+    }
+    __syncthreads();
+}
+
+// out (+)= P_l vc (src/level.cpp:182-209)
+template <typename R>
+__device__ void c_prolong(const CArgs<R>& A, int l, const cplx<R>* vc, cplx<R>* out, bool accumulate) {
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int D = g.dof, n = g.ns * D, Dc = A.lv[l + 1].g.dof;
+    const size_t nvec = (size_t)n;
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += CT) {
+        const int x = e / D, d = e - x * D, s = d & 1;
+        const cplx<R>* c = vc + (size_t)agg_block_of(g, x) * Dc;
+        cplx<R> o = czero<R>();
+        for (int cc = 0; cc < g.ntest; cc++) o += L.w[cc * nvec + e] * c[2 * cc + s];
+        out[e] = accumulate ? out[e] + o : o;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// wavefront-level pieces for one Schwarz block (src/sap.cpp:44-62,
+// src/level.cpp:304-368).  Block vectors are [sspb][dof], local index
+// m = mx * stel + mt.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename R> __device__ __forceinline__ c64 w_dot(const cplx<R>* a, const cplx<R>* b, int n) {
+    const int lane = threadIdx.x & 63;
+    c64 s = {0, 0};
+    for (int e = lane; e < n; e += 64) s += mulc(cvt<double>(a[e]), cvt<double>(b[e]));
+    return wave_sum(s);
+}
+
+// block-local stencil, neighbours outside block blk read as zero
+template <typename R>
+__device__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63;
+    const size_t msz = (size_t)D * D, ksz = (size_t)g.ns * msz;
+    for (int e = lane; e < nb; e += 64) {
+        const int m = e / D, r = e - m * D;
+        const int n = sap_site(g, blk, m);
+        cplx<R> acc = A.mm * in[e];
+        const cplx<R>* a0 = L.links + n * msz + r;
+        const cplx<R>* v0 = in + (size_t)m * D;
+        for (int c = 0; c < D; c++) acc -= a0[(size_t)c * D] * v0[c];
+        for (int mu = 0; mu < 2; mu++) {
+            const int xr = nb_right(n, mu, g.nx, g.nt), xl = nb_left(n, mu, g.nx, g.nt);
+            if (sap_block_of(g, xr) == blk) {
+                const cplx<R>* bm = L.links + (1 + mu) * ksz + n * msz + r;
+                const cplx<R>* vr = in + (size_t)sap_local_of(g, xr) * D;
+                cplx<R> sr = czero<R>();
+                for (int c = 0; c < D; c++) sr += bm[(size_t)c * D] * vr[c];
+                acc -= wraps_right(n, mu, g.nt) ? -sr : sr;
+            }
+            if (sap_block_of(g, xl) == blk) {
+                const cplx<R>* cm = L.links + (3 + mu) * ksz + n * msz + r;
+                const cplx<R>* vl = in + (size_t)sap_local_of(g, xl) * D;
+                cplx<R> sl = czero<R>();
+                for (int c = 0; c < D; c++) sl += cm[(size_t)c * D] * vl[c];
+                acc -= wraps_left(n, mu, g.nt) ? -sl : sl;
+            }
+        }
+        out[e] = acc;
+    }
+    wave_sync();
+}
+
+// Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
+// returns cn_j, sn_j; col[j] becomes the diagonal of R.
+__device__ __forceinline__ void givens_col(c64* col, const c64* cn, const c64* sn, int j, c64& cnj, c64& snj) {
+    for (int i = 0; i < j; i++) {
+        c64 t = conj(cn[i]) * col[i] + conj(sn[i]) * col[i + 1];
+        col[i + 1] = -(sn[i] * col[i]) + cn[i] * col[i + 1];
+        col[i] = t;
+    }
+    const double den = sqrt(norm2(col[j]) + norm2(col[j + 1]));
+    snj = c64{col[j + 1].re / den, col[j + 1].im / den};
+    cnj = c64{col[j].re / den, col[j].im / den};
+    col[j] = conj(cnj) * col[j] + conj(snj) * col[j + 1];
+}
+
+// GMRES(m, restarts, tol) on D_B for one Schwarz block, x0 = b_B; then
+// x[B] += x_B.  Executed by one wavefront.  True modified Gram-Schmidt.
+template <typename R>
+__device__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob, cplx<R>* x) {
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = A.sap_m;
+    WaveScal& S = K.S.wv[wv];
+    cplx<R>* base = K.W + L.blk + (size_t)wv * L.blkStride;
+    auto V = [&](int j) { return base + (size_t)j * nb; };
+    cplx<R>* xb = base + (size_t)(m + 1) * nb;
+    cplx<R>* bb = base + (size_t)(m + 2) * nb;
+    // gather b_B; x_B = b_B
+    for (int e = lane; e < nb; e += 64) {
+        const int mm = e / D, d = e - mm * D;
+        const cplx<R> v = rglob[(size_t)sap_site(g, blk, mm) * D + d];
+        bb[e] = v;
+        xb[e] = v;
+    }
+    wave_sync();
+    w_dlocal(A, l, blk, xb, V(1));
+    for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
+    wave_sync();
+    const double normb = sqrt(w_dot(bb, bb, nb).re);
+    double err = sqrt(w_dot(V(0), V(0), nb).re);
+    if (normb == 0.0) return;  // zero block right-hand side: x_B = 0
+    int maxIt = m;
+    for (int k = 0; k < A.sap_restarts; k++) {
+        {
+            const R ib = R(1.0 / err);
+            for (int e = lane; e < nb; e += 64) V(0)[e] = ib * V(0)[e];
+        }
+        c64 g0 = {err, 0.0};
+        if (lane == 0) S.g[0] = g0;
+        wave_sync();
+        for (int j = 0; j < m; j++) {
+            w_dlocal(A, l, blk, V(j), V(j + 1));
+            cplx<R>* w = V(j + 1);
+            c64 col[KC_MAXM + 1];
+            for (int i = 0; i <= j; i++) {
+                const c64 h = w_dot(w, V(i), nb);
+                col[i] = h;
+                const cplx<R> hr = cvt<R>(h);
+                const cplx<R>* vi = V(i);
+                for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
+                wave_sync();
+            }
+            const double hn = sqrt(w_dot(w, w, nb).re);
+            col[j + 1] = c64{hn, 0.0};
+            if (hn > 0) {
+                const R ih = R(1.0 / hn);
+                for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
+            }
+            c64 cnj, snj;
+            givens_col(col, S.cn, S.sn, j, cnj, snj);
+            const c64 gj = S.g[j];
+            const c64 gn = -(snj * gj);
+            wave_sync();
+            if (lane == 0) {
+                for (int i = 0; i <= j; i++) S.R[i][j] = col[i];
+                S.cn[j] = cnj;
+                S.sn[j] = snj;
+                S.g[j] = conj(cnj) * gj;
+                S.g[j + 1] = gn;
+            }
+            wave_sync();
+            if (sqrt(norm2(gn)) < A.sap_gtol * normb) {
+                maxIt = j + 1;
+                break;
+            }
+        }
+        if (lane == 0) {
+            for (int i = maxIt - 1; i >= 0; i--) {
+                c64 e = S.g[i];
+                for (int q = i + 1; q < maxIt; q++) e -= S.R[i][q] * S.eta[q];
+                S.eta[i] = cdiv(e, S.R[i][i]);
+            }
+        }
+        wave_sync();
+        for (int e = lane; e < nb; e += 64) {
+            c64 s = cvt<double>(xb[e]);
+            for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(V(j)[e]);
+            xb[e] = cvt<R>(s);
+        }
+        wave_sync();
+        w_dlocal(A, l, blk, xb, V(1));
+        for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
+        wave_sync();
+        err = sqrt(w_dot(V(0), V(0), nb).re);
+        if (err < A.sap_gtol * normb) break;
+    }
+    for (int e = lane; e < nb; e += 64) {
+        const int mm = e / D, d = e - mm * D;
+        x[(size_t)sap_site(g, blk, mm) * D + d] += xb[e];
+    }
+}
+
+// SAP_C::SAP (src/sap.cpp:64-174): red then black blocks, residual after each colour.
+template <typename R> __device__ void c_sap(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* v, cplx<R>* x, int nu) {
+    const CLevel<R>& L = A.lv[l];
+    const int n = L.g.ns * L.g.dof;
+    cplx<R>* r = K.W + L.sR;
+    cplx<R>* Dx = K.W + L.sDx;
+    const double vn = sqrt(c_dot(K.S, v, v, n).re);
+    c_dop(A, l, x, Dx);
+    for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+    for (int it = 0; it < nu; it++) {
+        for (int colour = 0; colour < 2; colour++) {
+            const int* list = colour == 0 ? L.red : L.black;
+            __syncthreads();
+            for (int bi = threadIdx.x >> 6; bi < L.g.ncol; bi += CW) w_block_gmres(A, K, l, list[bi], r, x);
+            __syncthreads();
+            c_dop(A, l, x, Dx);
+            for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+        }
+        const double err = sqrt(c_dot(K.S, r, r, n).re);
+        if (err < A.sap_tol * vn) return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// workgroup FGMRES (src/fgmres.cpp:5-88) and the K-cycle recursion
+// ---------------------------------------------------------------------------
+template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi);
+
+// Restarted FGMRES from x0 = b.  crs: the coarsest GMRES (identity
+// preconditioner, GMRES_level_l); otherwise the K-cycle FGMRES whose
+// preconditioner is k_cycle(LV) started from the previous cycle's Z_j
+// (include/amg.h:37-39).  One-pass MGS with the Gram correction:
+//   H_ij = <w, V_i> - sum_{k<i} H_kj <V_k, V_i>.
+// Scalars (Hessenberg column, Givens, residual estimate) are computed by
+// thread 0 in FP64 and broadcast through LDS.
+template <typename R, int LV, int M>
+__device__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, const cplx<R>* b, cplx<R>* x) {
+    const CLevel<R>& L = A.lv[LV];
+    const int n = L.g.ns * L.g.dof;
+    const int m = crs ? A.crs_m : A.kc_m;
+    const int restarts = crs ? A.crs_restarts : A.kc_restarts;
+    const double tol = crs ? A.crs_tol : A.kc_tol;
+    cplx<R>* Vb = K.W + (crs ? L.crV : L.kcV);
+    cplx<R>* Zb = crs ? Vb : K.W + L.kcZ;
+    cplx<R>* r = K.W + (crs ? L.crR : L.kcR);
+    cplx<R>* w = K.W + (crs ? L.crW : L.kcW);
+    cplx<R>* Dx = K.W + (crs ? L.crDx : L.kcDx);
+    auto V = [&](int j) { return Vb + (size_t)j * n; };
+    auto Z = [&](int j) { return Zb + (size_t)j * n; };
+    // setZeros (include/fgmres.h:93-120): Z_j start from zero in the first cycle
+    __syncthreads();
+    if (!crs)
+        for (int e = threadIdx.x; e < m * n; e += CT) Zb[e] = czero<R>();
+    for (int e = threadIdx.x; e < n; e += CT) x[e] = b[e];
+    c_dop(A, LV, x, Dx);
+    for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
+    const double nb = sqrt(c_dot(K.S, b, b, n).re);
+    double err = sqrt(c_dot(K.S, r, r, n).re);
+    if (err == 0.0) return;
+    int maxIt = m;
+    for (int k = 0; k < restarts; k++) {
+        const R ib = R(1.0 / err);
+        for (int e = threadIdx.x; e < n; e += CT) V(0)[e] = ib * r[e];
+        if (threadIdx.x == 0) S.g[0] = c64{err, 0.0};
+        for (int j = 0; j < m; j++) {
+            cplx<R>* zj = Z(j);
+            if (!crs) {
+                if constexpr (LV < MAXL) c_kcycle<R, LV>(A, K, V(j), zj);
+            }
+            c_dop(A, LV, zj, w);
+            // one pass: <w, V_i> (i <= j) and <V_j, V_k> (k < j); nq dots,
+            // CT/nq threads per dot, partial sums combined in fixed order
+            const int nq = 2 * j + 1, per = CT / nq;
+            {
+                const int q = threadIdx.x / per, sub = threadIdx.x - q * per;
+                c64 acc = {0, 0};
+                if (q < nq) {
+                    const cplx<R>* a = q <= j ? w : V(j);
+                    const cplx<R>* bv = q <= j ? V(q) : V(q - j - 1);
+                    for (int e = sub; e < n; e += per) acc += mulc(cvt<double>(a[e]), cvt<double>(bv[e]));
+                }
+                K.S.red[threadIdx.x] = acc;
+                __syncthreads();
+                if (threadIdx.x < nq) {
+                    c64 s = K.S.red[threadIdx.x * per];
+                    for (int i = 1; i < per; i++) s += K.S.red[threadIdx.x * per + i];
+                    S.dq[threadIdx.x] = s;
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                for (int i = 0; i <= j; i++) {
+                    c64 h = S.dq[i];
+                    for (int kk = 0; kk < i; kk++) {
+                        const c64 Gik = (i == j) ? S.dq[j + 1 + kk] : S.G[i][kk];
+                        h -= S.col[kk] * conj(Gik);
+                    }
+                    S.col[i] = h;
+                }
+                for (int kk = 0; kk < j; kk++) S.G[j][kk] = S.dq[j + 1 + kk];
+            }
+            __syncthreads();
+            c64 nq1[1] = {{0, 0}};
+            for (int e = threadIdx.x; e < n; e += CT) {
+                c64 we = cvt<double>(w[e]);
+                for (int i = 0; i <= j; i++) we -= S.col[i] * cvt<double>(V(i)[e]);
+                const cplx<R> wr = cvt<R>(we);
+                w[e] = wr;
+                nq1[0] += mulc(cvt<double>(wr), cvt<double>(wr));
+            }
+            block_sum<1>(nq1, K.S.red);
+            const double hn = sqrt(nq1[0].re);
+            if (hn > 0) {
+                const R ih = R(1.0 / hn);
+                for (int e = threadIdx.x; e < n; e += CT) V(j + 1)[e] = ih * w[e];
+            }
+            if (threadIdx.x == 0) {
+                c64 col[M + 1];
+                for (int i = 0; i <= j; i++) col[i] = S.col[i];
+                col[j + 1] = c64{hn, 0.0};
+                c64 cnj, snj;
+                givens_col(col, S.cn, S.sn, j, cnj, snj);
+                const c64 gj = S.g[j];
+                const c64 gn = -(snj * gj);
+                for (int i = 0; i <= j; i++) S.R[i][j] = col[i];
+                S.cn[j] = cnj;
+                S.sn[j] = snj;
+                S.g[j] = conj(cnj) * gj;
+                S.g[j + 1] = gn;
+                S.brk = sqrt(norm2(gn)) < tol * nb;
+            }
+            __syncthreads();
+            if (S.brk) {
+                maxIt = j + 1;
+                break;
+            }
+        }
+        if (threadIdx.x == 0) {
+            for (int i = maxIt - 1; i >= 0; i--) {
+                c64 e = S.g[i];
+                for (int q = i + 1; q < maxIt; q++) e -= S.R[i][q] * S.eta[q];
+                S.eta[i] = cdiv(e, S.R[i][i]);
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += CT) {
+            c64 s = cvt<double>(x[e]);
+            for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(Z(j)[e]);
+            x[e] = cvt<R>(s);
+        }
+        c_dop(A, LV, x, Dx);
+        for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
+        err = sqrt(c_dot(K.S, r, r, n).re);
+        if (err < tol * nb) return;
+    }
+}
+
+// k_cycle(l, eta, psi) (src/amg.cpp:135-178)
+template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi) {
+    if (LV == A.L - 1) {
+        c_fgmres<R, LV, CRS_MAXM>(A, K, K.S.crs, true, eta, psi);
+        return;
+    }
+    if constexpr (LV + 1 < MAXL) {
+        const CLevel<R>& L = A.lv[LV];
+        const int n = L.g.ns * L.g.dof;
+        if (A.nu1 > 0) c_sap(A, K, LV, eta, psi, A.nu1);
+        cplx<R>* Dpsi = K.W + L.kDpsi;
+        cplx<R>* r = K.W + L.kR;
+        cplx<R>* eta1 = K.W + L.kEta1;
+        cplx<R>* psi1 = K.W + L.kPsi1;
+        c_dop(A, LV, psi, Dpsi);
+        for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
+        c_restrict(A, LV, r, eta1);
+        c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
+        c_prolong(A, LV, psi1, psi, true);
+        if (A.nu2 > 0) c_sap(A, K, LV, eta, psi, A.nu2);
+    }
+}
+
+template <typename R> __device__ void c_kcycle_dispatch(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* eta, cplx<R>* psi) {
+    switch (l) {
+        case 1: c_kcycle<R, 1>(A, K, eta, psi); break;
+        case 2: c_kcycle<R, 2>(A, K, eta, psi); break;
+        case 3: c_kcycle<R, 3>(A, K, eta, psi); break;
+        case 4: c_kcycle<R, 4>(A, K, eta, psi); break;
+        default: c_kcycle<R, 5>(A, K, eta, psi); break;
+    }
+}
+
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse(CArgs<R> A) {
+    __shared__ CLds S;
+    const int rhs = blockIdx.x;
+    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S};
+    const cplx<R>* in = A.in + (size_t)rhs * A.in_stride;
+    cplx<R>* out = A.out + (size_t)rhs * A.out_stride;
+    const int l = A.level;
+    switch (A.mode) {
+        case CM_KC_SOLVE:  // FGMRES_k_cycle[l].fgmres(eta, eta, psi) (src/amg.cpp:161)
+            switch (l) {
+                case 1: c_fgmres<R, 1, KC_MAXM>(A, K, S.kc[1], false, in, out); break;
+                case 2: c_fgmres<R, 2, KC_MAXM>(A, K, S.kc[2], false, in, out); break;
+                case 3: c_fgmres<R, 3, KC_MAXM>(A, K, S.kc[3], false, in, out); break;
+                case 4: c_fgmres<R, 4, KC_MAXM>(A, K, S.kc[4], false, in, out); break;
+                default: c_fgmres<R, 5, KC_MAXM>(A, K, S.kc[5], false, in, out); break;
+            }
+            break;
+        case CM_SAP: c_sap(A, K, l, in, out, A.nu); break;
+        case CM_DOP: c_dop(A, l, in, out); break;
+        case CM_RESTRICT: c_restrict(A, l, in, out); break;
+        case CM_PROLONG: c_prolong(A, l, in, out, false); break;
+        case CM_KCYCLE: c_kcycle_dispatch(A, K, l, in, out); break;
+    }
+}
+
+}  // namespace amgs

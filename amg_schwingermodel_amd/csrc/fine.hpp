@@ -1,0 +1,595 @@
+// Level-0 (fine lattice, 2 spin components) kernels.
+//
+// Vectors are [nrhs][ns][2] complex; site n = x*nt + t.  One thread per site,
+// 256 threads per workgroup; the right-hand side is blockIdx.y.
+#pragma once
+#include "common.hpp"
+
+namespace amgs {
+
+constexpr int FT = 256;       // threads per workgroup for fine-lattice kernels
+constexpr int SAP_MAXM = 8;   // max restart length of the SAP block GMRES
+constexpr int OUT_MAXM = 32;  // max restart length of the outer FGMRES
+
+// Workgroup -> tile map that keeps consecutive tiles on one XCD: the hardware
+// deals workgroups round-robin over the 8 XCDs, so tile = (wg % 8) * per + wg / 8
+// gives each XCD a contiguous run of lattice rows (its x-neighbours stay in
+// the same L2).  Identity when the grid is not a multiple of 8.
+__device__ __forceinline__ int xcd_tile(int wg, int nwg) {
+    if (nwg & 7) return wg;
+    return (wg & 7) * (nwg >> 3) + (wg >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Wilson-Dirac stencil at one site (PDF eq.6; src/dirac_operator.cpp:3-54)
+//   D:  out = (m0+2) psi - 1/2 sum_mu [(1-s_mu) U_mu(n) psi(n+mu) + (1+s_mu) U_mu(n-mu)^* psi(n-mu)]
+//   D^dagger swaps the projectors.  s_0 = sigma_x (time), s_1 = sigma_y (space),
+//   antiperiodic in time.  With spin projection every hop costs one complex
+//   product:  A = f0 (p0-p1)(r0), B = f1 (p0+ip1)(r1), C = b0 (p0+p1)(l0),
+//   E = b1 (p0-ip1)(l1);  out0 = mm p0 - (A+B+C+E)/2,  out1 = mm p1 - (-A-iB+C+iE)/2
+//   (for D^dagger the same with the forward/backward hops exchanged).
+// MASK(n') says whether neighbour n' contributes (zero Dirichlet for SAP blocks).
+// ---------------------------------------------------------------------------
+template <typename R, bool DAG, typename Mask>
+__device__ __forceinline__ void wilson_site(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
+                                            int n, int nx, int nt, R mm, Mask mask, cplx<R>& o0, cplx<R>& o1) {
+    const int r0 = nb_right(n, 0, nx, nt), r1 = nb_right(n, 1, nx, nt);
+    const int l0 = nb_left(n, 0, nx, nt), l1 = nb_left(n, 1, nx, nt);
+    const cplx<R> c0 = in[2 * n], c1 = in[2 * n + 1];
+    cplx<R> f0 = U[2 * n], f1 = U[2 * n + 1];
+    cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
+    if (wraps_right(n, 0, nt)) f0 = -f0;
+    if (wraps_left(n, 0, nt)) b0 = -b0;
+    cplx<R> A = czero<R>(), B = czero<R>(), C = czero<R>(), E = czero<R>();
+    // forward/backward neighbour reads
+    if (!DAG) {
+        if (mask(r0)) { cplx<R> p0 = in[2 * r0], p1 = in[2 * r0 + 1]; A = f0 * (p0 - p1); }
+        if (mask(r1)) { cplx<R> p0 = in[2 * r1], p1 = in[2 * r1 + 1]; B = f1 * (p0 + mul_i(p1)); }
+        if (mask(l0)) { cplx<R> p0 = in[2 * l0], p1 = in[2 * l0 + 1]; C = b0 * (p0 + p1); }
+        if (mask(l1)) { cplx<R> p0 = in[2 * l1], p1 = in[2 * l1 + 1]; E = b1 * (p0 - mul_i(p1)); }
+    } else {
+        if (mask(l0)) { cplx<R> p0 = in[2 * l0], p1 = in[2 * l0 + 1]; A = b0 * (p0 - p1); }
+        if (mask(l1)) { cplx<R> p0 = in[2 * l1], p1 = in[2 * l1 + 1]; B = b1 * (p0 + mul_i(p1)); }
+        if (mask(r0)) { cplx<R> p0 = in[2 * r0], p1 = in[2 * r0 + 1]; C = f0 * (p0 + p1); }
+        if (mask(r1)) { cplx<R> p0 = in[2 * r1], p1 = in[2 * r1 + 1]; E = f1 * (p0 - mul_i(p1)); }
+    }
+    const R h = R(0.5);
+    o0 = mm * c0 - h * (A + B + C + E);
+    o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+}
+
+struct AllSites {
+    __device__ bool operator()(int) const { return true; }
+};
+struct InBlock {
+    Geo g;
+    int blk;
+    __device__ bool operator()(int n) const { return sap_block_of(g, n) == blk; }
+};
+
+// out = D in  (or D^dagger), all sites, all right-hand sides
+template <typename R, bool DAG>
+__global__ void __launch_bounds__(FT) k_wilson(Geo g, R mm, const cplx<R>* __restrict__ U,
+                                               const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int n = tile * FT + threadIdx.x;
+    if (n >= g.ns) return;
+    const size_t off = (size_t)blockIdx.y * g.ns * 2;
+    cplx<R> o0, o1;
+    wilson_site<R, DAG>(U, in + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
+    out[off + 2 * n] = o0;
+    out[off + 2 * n + 1] = o1;
+}
+
+// r = v - D x; optional per-workgroup partial sums of |r|^2 (for SAP's
+// convergence check) into part[rhs][tile].  skip[rhs] != 0 leaves r unchanged.
+template <typename R>
+__global__ void __launch_bounds__(FT) k_resid(Geo g, R mm, const cplx<R>* __restrict__ U,
+                                              const cplx<R>* __restrict__ v, const cplx<R>* __restrict__ x,
+                                              cplx<R>* __restrict__ r, double* __restrict__ part,
+                                              const int* __restrict__ skip) {
+    __shared__ c64 red[4];
+    const int rhs = blockIdx.y;
+    if (skip && skip[rhs]) return;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int n = tile * FT + threadIdx.x;
+    const size_t off = (size_t)rhs * g.ns * 2;
+    double nn = 0;
+    if (n < g.ns) {
+        cplx<R> o0, o1;
+        wilson_site<R, false>(U, x + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
+        cplx<R> a = v[off + 2 * n] - o0, b = v[off + 2 * n + 1] - o1;
+        r[off + 2 * n] = a;
+        r[off + 2 * n + 1] = b;
+        nn = double(norm2(a)) + double(norm2(b));
+    }
+    if (part) {
+        c64 q[1] = {{nn, 0.0}};
+        block_sum<1>(q, red);
+        if (threadIdx.x == 0) part[(size_t)rhs * gridDim.x + tile] = q[0].re;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SAP on the fine lattice (src/sap.cpp:44-174, include/sap.h:27-47).
+//
+// All Schwarz blocks of one colour and all right-hand sides are solved at once:
+// an item is (block of the colour, rhs); each item runs its own restarted
+// GMRES(m) with the reference's control flow (x0 = b_B, |g_{j+1}| < tol ||b_B||
+// breaks a cycle, true residual after each cycle, maxIt carried across cycles).
+// Vectors keep the full-lattice layout: the colour's blocks are disjoint, so
+// block-local vectors of all items of a colour share one lattice-sized buffer.
+//
+// An item covers nchunk workgroups of FT sites each.  Scalars (Hessenberg
+// column, Givens rotations, residual estimate) are recomputed identically by
+// every workgroup of an item from the previous kernel's per-workgroup partial
+// sums (fixed summation order), and stored by chunk 0 for later kernels; a
+// kernel never reads state that the same kernel writes.
+//
+// Modified Gram-Schmidt is evaluated in one pass: with G_ki = <V_k, V_i>,
+//   H_ij = <w, V_i> - sum_{k<i} H_kj G_ki
+// equals the sequential MGS coefficient <w - sum_{k<i} H_kj V_k, V_i> exactly
+// in exact arithmetic.  Basis vectors are stored unnormalised (Vt_j) with
+// V_j = s_j Vt_j, s_j = 1/H_{j,j-1}, so no separate normalisation pass runs.
+// ---------------------------------------------------------------------------
+struct SapState {
+    int *maxIt, *jbreak, *done;
+    double *normb, *s;          // s: [item][M+1]
+    c64 *Hraw, *Hrot;           // [item][(M+1)*M]  (row i, column j at i*M+j)
+    c64 *G;                     // [item][(M+1)*(M+1)]
+    c64 *gpre, *gpost, *cn, *sn;
+    c64 *qa;                    // [item][nchunk][2M]   <w,Vt_i>, <Vt_j,Vt_k>
+    double *qb;                 // [item][nchunk]       |w|^2
+    double *qr;                 // [item][nchunk][2]    |r|^2, |b|^2
+};
+
+template <typename R> struct SapArgs {
+    Geo g;
+    R mm;
+    const cplx<R>* U;
+    const int* blocks;  // block ids of this colour [ncol]
+    int ncol, nchunk, nrhs, m;
+    double tol;
+    const cplx<R>* rglob;  // b of each item = global residual restricted to the block
+    cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]
+    cplx<R>* xl;           // [nrhs][ns][2] block solutions
+    const int* skip;       // per-rhs: SAP already converged
+    SapState st;
+};
+
+struct ItemIdx {
+    int item, chunk, rhs, blk, m_lo, m_hi;
+};
+
+template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R>& a) {
+    ItemIdx t;
+    const int i = blockIdx.x / a.nchunk;
+    t.chunk = blockIdx.x - i * a.nchunk;
+    t.rhs = blockIdx.y;
+    t.item = t.rhs * a.ncol + i;
+    t.blk = a.blocks[i];
+    t.m_lo = t.chunk * FT;
+    t.m_hi = min(a.g.sspb, t.m_lo + FT);
+    return t;
+}
+
+template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
+    return a.Vt + ((size_t)j * a.nrhs + rhs) * a.g.ns * 2;
+}
+
+// Per-item scalar state held in LDS during a kernel.
+struct ItemLds {
+    int active, maxIt, broke;
+    double s[SAP_MAXM + 1];
+    c64 H[SAP_MAXM + 1];   // working column
+    c64 R[SAP_MAXM][SAP_MAXM];  // rotated (upper-triangular) Hessenberg
+    c64 g[SAP_MAXM + 1];
+    c64 eta[SAP_MAXM];
+    c64 red[8 * 2 * SAP_MAXM];
+};
+
+// Finalise Arnoldi step j (src/fgmres.cpp:35-49): H_{j+1,j} from the |w|^2
+// partials, Givens rotation of column j, residual estimate, break test.
+// Executed by thread 0; chunk 0 stores.  Returns through L.
+__device__ void sap_finalize_step(const SapState& st, int item, int nchunk, int M, int j, double tol,
+                                  bool store, ItemLds& L) {
+    double hn2 = 0;
+    for (int c = 0; c < nchunk; c++) hn2 += st.qb[(size_t)item * nchunk + c];
+    const double hn = sqrt(hn2);
+    const c64* Hr = st.Hraw + (size_t)item * (M + 1) * M;
+    c64 col[SAP_MAXM + 1];
+    for (int i = 0; i <= j; i++) col[i] = Hr[i * M + j];
+    col[j + 1] = c64{hn, 0.0};
+    const c64* cn = st.cn + (size_t)item * M;
+    const c64* sn = st.sn + (size_t)item * M;
+    for (int i = 0; i < j; i++) {
+        c64 t = conj(cn[i]) * col[i] + conj(sn[i]) * col[i + 1];
+        col[i + 1] = -(sn[i] * col[i]) + cn[i] * col[i + 1];
+        col[i] = t;
+    }
+    const double den = sqrt(norm2(col[j]) + norm2(col[j + 1]));
+    const c64 snj = {col[j + 1].re / den, col[j + 1].im / den};
+    const c64 cnj = {col[j].re / den, col[j].im / den};
+    col[j] = conj(cnj) * col[j] + conj(snj) * col[j + 1];
+    const c64 gj = st.gpre[(size_t)item * (M + 1) + j];
+    const c64 gn = -(snj * gj);
+    const c64 gp = conj(cnj) * gj;
+    const bool broke = sqrt(norm2(gn)) < tol * st.normb[item];
+    for (int i = 0; i <= j; i++) L.R[i][j] = col[i];
+    L.g[j] = gp;
+    L.s[j + 1] = hn > 0 ? 1.0 / hn : 0.0;
+    L.broke = broke;
+    if (store) {
+        c64* Ro = st.Hrot + (size_t)item * (M + 1) * M;
+        for (int i = 0; i <= j; i++) Ro[i * M + j] = col[i];
+        st.cn[(size_t)item * M + j] = cnj;
+        st.sn[(size_t)item * M + j] = snj;
+        st.gpost[(size_t)item * (M + 1) + j] = gp;
+        st.gpre[(size_t)item * (M + 1) + j + 1] = gn;
+        st.s[(size_t)item * (M + 1) + j + 1] = L.s[j + 1];
+        if (broke) {
+            st.jbreak[item] = j;
+            st.maxIt[item] = j + 1;
+        }
+    }
+}
+
+// init: x_B = b_B (x0 = b, src/sap.cpp:61), Vt_0 = b_B - D_B b_B; partials |r|^2, |b|^2
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R> a) {
+    __shared__ c64 red[8];
+    const ItemIdx t = item_of(a);
+    if (a.skip && a.skip[t.rhs]) return;
+    const int m = t.m_lo + threadIdx.x;
+    const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    c64 q[2] = {{0, 0}, {0, 0}};
+    if (m < t.m_hi) {
+        const int n = sap_site(a.g, t.blk, m);
+        cplx<R> o0, o1;
+        wilson_site<R, false>(a.U, a.rglob + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        const cplx<R> b0 = a.rglob[off + 2 * n], b1 = a.rglob[off + 2 * n + 1];
+        a.xl[off + 2 * n] = b0;
+        a.xl[off + 2 * n + 1] = b1;
+        const cplx<R> r0 = b0 - o0, r1 = b1 - o1;
+        cplx<R>* v0 = slot(a, 0, t.rhs);
+        v0[2 * n] = r0;
+        v0[2 * n + 1] = r1;
+        q[0].re = double(norm2(r0)) + double(norm2(r1));
+        q[1].re = double(norm2(b0)) + double(norm2(b1));
+    }
+    block_sum<2>(q, red);
+    if (threadIdx.x == 0) {
+        a.st.qr[((size_t)t.item * a.nchunk + t.chunk) * 2] = q[0].re;
+        a.st.qr[((size_t)t.item * a.nchunk + t.chunk) * 2 + 1] = q[1].re;
+        if (t.chunk == 0) {
+            a.st.maxIt[t.item] = a.m;
+            a.st.done[t.item] = 0;
+            a.st.jbreak[t.item] = -1;
+        }
+    }
+}
+
+// Arnoldi step j, part A: prologue finalises step j-1 (or starts cycle k),
+// body w = s_j D_B Vt_j -> Vt_{j+1}; partials <w, Vt_i> (i <= j), <Vt_j, Vt_k> (k < j).
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int j, int k) {
+    __shared__ ItemLds L;
+    const ItemIdx t = item_of(a);
+    const int M = a.m;
+    if (a.skip && a.skip[t.rhs]) return;
+    if (threadIdx.x == 0) {
+        const SapState& st = a.st;
+        const bool store = t.chunk == 0;
+        int active = !st.done[t.item];
+        if (active && j == 0) {
+            // cycle start (src/fgmres.cpp:8-17, 61-65)
+            double e2 = 0, b2 = 0;
+            for (int c = 0; c < a.nchunk; c++) {
+                e2 += st.qr[((size_t)t.item * a.nchunk + c) * 2];
+                b2 += st.qr[((size_t)t.item * a.nchunk + c) * 2 + 1];
+            }
+            const double err = sqrt(e2);
+            double nb = k == 0 ? sqrt(b2) : st.normb[t.item];
+            bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
+            if (done) {
+                active = 0;
+                if (store) st.done[t.item] = 1;
+            } else {
+                L.s[0] = 1.0 / err;
+                if (store) {
+                    if (k == 0) st.normb[t.item] = nb;
+                    st.s[(size_t)t.item * (M + 1)] = L.s[0];
+                    st.gpre[(size_t)t.item * (M + 1)] = c64{err, 0.0};
+                    st.jbreak[t.item] = -1;
+                }
+            }
+        } else if (active) {
+            if (st.jbreak[t.item] >= 0) active = 0;
+            else {
+                sap_finalize_step(st, t.item, a.nchunk, M, j - 1, a.tol, store, L);
+                if (L.broke) active = 0;
+            }
+        }
+        if (active && j > 0) L.s[j] = L.s[j];  // s_j from the finalisation above
+        L.active = active;
+    }
+    __syncthreads();
+    if (!L.active) return;
+    const double sj = L.s[j];
+    const int m = t.m_lo + threadIdx.x;
+    const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    const cplx<R>* vj = slot(a, j, t.rhs);
+    cplx<R>* w = slot(a, j + 1, t.rhs);
+    c64 q[2 * SAP_MAXM];
+    for (int i = 0; i < 2 * SAP_MAXM; i++) q[i] = c64{0, 0};
+    if (m < t.m_hi) {
+        const int n = sap_site(a.g, t.blk, m);
+        cplx<R> o0, o1;
+        wilson_site<R, false>(a.U, vj, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        o0 = R(sj) * o0;
+        o1 = R(sj) * o1;
+        w[2 * n] = o0;
+        w[2 * n + 1] = o1;
+        const c64 w0 = cvt<double>(o0), w1 = cvt<double>(o1);
+        const c64 vj0 = cvt<double>(vj[2 * n]), vj1 = cvt<double>(vj[2 * n + 1]);
+        for (int i = 0; i <= j; i++) {
+            const cplx<R>* vi = slot(a, i, t.rhs);
+            const c64 a0 = cvt<double>(vi[2 * n]), a1 = cvt<double>(vi[2 * n + 1]);
+            q[i] = mulc(w0, a0) + mulc(w1, a1);
+            if (i < j) q[j + 1 + i] = mulc(vj0, a0) + mulc(vj1, a1);
+        }
+    }
+    (void)off;
+    block_sum<2 * SAP_MAXM>(q, L.red, 2 * j + 1);
+    if (threadIdx.x < 2 * j + 1)
+        a.st.qa[((size_t)t.item * a.nchunk + t.chunk) * 2 * M + threadIdx.x] = q[threadIdx.x];
+}
+
+// Arnoldi step j, part B: prologue builds Hessenberg column j (MGS form),
+// body w -= sum_i H_ij V_i (in place in Vt_{j+1}); partial |w|^2.
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j) {
+    __shared__ ItemLds L;
+    const ItemIdx t = item_of(a);
+    const int M = a.m;
+    if (a.skip && a.skip[t.rhs]) return;
+    if (threadIdx.x == 0) {
+        const SapState& st = a.st;
+        int active = !st.done[t.item] && st.jbreak[t.item] < 0;
+        if (active) {
+            c64 d[2 * SAP_MAXM];
+            for (int q = 0; q < 2 * j + 1; q++) {
+                c64 s = {0, 0};
+                for (int c = 0; c < a.nchunk; c++) s += st.qa[((size_t)t.item * a.nchunk + c) * 2 * M + q];
+                d[q] = s;
+            }
+            const double* sv = st.s + (size_t)t.item * (M + 1);
+            double s[SAP_MAXM + 1];
+            for (int i = 0; i <= j; i++) s[i] = sv[i];
+            // Gram row j: G_jk = s_j s_k <Vt_j, Vt_k>,  k < j
+            c64* Gm = st.G + (size_t)t.item * (M + 1) * (M + 1);
+            c64 gj[SAP_MAXM];
+            for (int kk = 0; kk < j; kk++) gj[kk] = (s[j] * s[kk]) * d[j + 1 + kk];
+            c64 H[SAP_MAXM + 1];
+            for (int i = 0; i <= j; i++) {
+                c64 h = s[i] * d[i];
+                for (int kk = 0; kk < i; kk++) {
+                    // G_ki = <V_k, V_i> = conj(G_ik); rows < j come from memory
+                    c64 Gik = (i == j) ? gj[kk] : Gm[i * (M + 1) + kk];
+                    h -= H[kk] * conj(Gik);
+                }
+                H[i] = h;
+                L.H[i] = h;
+                L.s[i] = s[i];
+            }
+            if (t.chunk == 0) {
+                c64* Hr = st.Hraw + (size_t)t.item * (M + 1) * M;
+                for (int i = 0; i <= j; i++) Hr[i * M + j] = H[i];
+                for (int kk = 0; kk < j; kk++) Gm[j * (M + 1) + kk] = gj[kk];
+            }
+        }
+        L.active = active;
+    }
+    __syncthreads();
+    if (!L.active) return;
+    const int m = t.m_lo + threadIdx.x;
+    double nn = 0;
+    if (m < t.m_hi) {
+        const int n = sap_site(a.g, t.blk, m);
+        cplx<R>* w = slot(a, j + 1, t.rhs);
+        c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
+        for (int i = 0; i <= j; i++) {
+            const cplx<R>* vi = slot(a, i, t.rhs);
+            const c64 c = L.s[i] * L.H[i];
+            w0 -= c * cvt<double>(vi[2 * n]);
+            w1 -= c * cvt<double>(vi[2 * n + 1]);
+        }
+        const cplx<R> r0 = cvt<R>(w0), r1 = cvt<R>(w1);
+        w[2 * n] = r0;
+        w[2 * n + 1] = r1;
+        nn = double(norm2(r0)) + double(norm2(r1));
+    }
+    c64 q[1] = {{nn, 0.0}};
+    block_sum<1>(q, L.red);
+    if (threadIdx.x == 0) a.st.qb[(size_t)t.item * a.nchunk + t.chunk] = q[0].re;
+}
+
+// End of cycle: finalise the last step if the cycle ran to m, back-substitute
+// (src/fgmres.cpp:52-59, 107-115) and update x_B += sum_j eta_j V_j.
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a) {
+    __shared__ ItemLds L;
+    const ItemIdx t = item_of(a);
+    const int M = a.m;
+    if (a.skip && a.skip[t.rhs]) return;
+    if (threadIdx.x == 0) {
+        const SapState& st = a.st;
+        int active = !st.done[t.item];
+        if (active) {
+            const int jb = st.jbreak[t.item];
+            int maxIt = st.maxIt[t.item];
+            const c64* Ro = st.Hrot + (size_t)t.item * (M + 1) * M;
+            const int ncols = jb >= 0 ? jb + 1 : M;
+            for (int jj = 0; jj < ncols; jj++)
+                for (int i = 0; i <= jj; i++) L.R[i][jj] = Ro[i * M + jj];
+            for (int jj = 0; jj < ncols; jj++) L.g[jj] = st.gpost[(size_t)t.item * (M + 1) + jj];
+            const double* sv = st.s + (size_t)t.item * (M + 1);
+            for (int i = 0; i <= M; i++) L.s[i] = sv[i];
+            if (jb < 0) {
+                sap_finalize_step(st, t.item, a.nchunk, M, M - 1, a.tol, t.chunk == 0, L);
+                if (L.broke) maxIt = M;
+            }
+            for (int i = maxIt - 1; i >= 0; i--) {
+                c64 e = L.g[i];
+                for (int q = i + 1; q < maxIt; q++) e -= L.R[i][q] * L.eta[q];
+                L.eta[i] = cdiv(e, L.R[i][i]);
+            }
+            L.maxIt = maxIt;
+        }
+        L.active = active;
+    }
+    __syncthreads();
+    if (!L.active) return;
+    const int m = t.m_lo + threadIdx.x;
+    if (m < t.m_hi) {
+        const int n = sap_site(a.g, t.blk, m);
+        const size_t off = (size_t)t.rhs * a.g.ns * 2;
+        c64 x0 = cvt<double>(a.xl[off + 2 * n]), x1 = cvt<double>(a.xl[off + 2 * n + 1]);
+        for (int jj = 0; jj < L.maxIt; jj++) {
+            const cplx<R>* vj = slot(a, jj, t.rhs);
+            const c64 c = L.s[jj] * L.eta[jj];
+            x0 += c * cvt<double>(vj[2 * n]);
+            x1 += c * cvt<double>(vj[2 * n + 1]);
+        }
+        a.xl[off + 2 * n] = cvt<R>(x0);
+        a.xl[off + 2 * n + 1] = cvt<R>(x1);
+    }
+}
+
+// True block residual after a cycle: Vt_0 = b_B - D_B x_B, partial |r|^2.
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<R> a) {
+    __shared__ c64 red[8];
+    __shared__ int act;
+    const ItemIdx t = item_of(a);
+    if (a.skip && a.skip[t.rhs]) return;
+    if (threadIdx.x == 0) act = !a.st.done[t.item];
+    __syncthreads();
+    if (!act) return;
+    const int m = t.m_lo + threadIdx.x;
+    const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    c64 q[1] = {{0, 0}};
+    if (m < t.m_hi) {
+        const int n = sap_site(a.g, t.blk, m);
+        cplx<R> o0, o1;
+        wilson_site<R, false>(a.U, a.xl + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        const cplx<R> r0 = a.rglob[off + 2 * n] - o0, r1 = a.rglob[off + 2 * n + 1] - o1;
+        cplx<R>* v0 = slot(a, 0, t.rhs);
+        v0[2 * n] = r0;
+        v0[2 * n + 1] = r1;
+        q[0].re = double(norm2(r0)) + double(norm2(r1));
+    }
+    block_sum<1>(q, red);
+    if (threadIdx.x == 0) a.st.qr[((size_t)t.item * a.nchunk + t.chunk) * 2] = q[0].re;
+}
+
+// x += x_B on the sites of this colour's blocks (src/sap.cpp:117-125)
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_accum(SapArgs<R> a, cplx<R>* x) {
+    const ItemIdx t = item_of(a);
+    if (a.skip && a.skip[t.rhs]) return;
+    const int m = t.m_lo + threadIdx.x;
+    if (m >= t.m_hi) return;
+    const int n = sap_site(a.g, t.blk, m);
+    const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    x[off + 2 * n] += a.xl[off + 2 * n];
+    x[off + 2 * n + 1] += a.xl[off + 2 * n + 1];
+}
+
+// per-rhs SAP convergence |r| < tol |v| after a sweep (src/sap.cpp:159-168)
+__global__ void k_sap_check(const double* part_r, const double* part_v, int ntile, int nrhs, double tol, int* skip) {
+    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rhs >= nrhs || skip[rhs]) return;
+    double e = 0, v = 0;
+    for (int i = 0; i < ntile; i++) {
+        e += part_r[(size_t)rhs * ntile + i];
+        v += part_v[(size_t)rhs * ntile + i];
+    }
+    if (sqrt(e) < tol * sqrt(v)) skip[rhs] = 1;
+}
+
+// |v|^2 per tile (for SAP's ||v||)
+template <typename R>
+__global__ void __launch_bounds__(FT) k_norm_part(int ns, const cplx<R>* __restrict__ v, double* part) {
+    __shared__ c64 red[4];
+    const int tile = blockIdx.x, n = tile * FT + threadIdx.x;
+    const size_t off = (size_t)blockIdx.y * ns * 2;
+    double nn = 0;
+    if (n < ns) nn = double(norm2(v[off + 2 * n])) + double(norm2(v[off + 2 * n + 1]));
+    c64 q[1] = {{nn, 0.0}};
+    block_sum<1>(q, red);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + tile] = q[0].re;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation between level 0 and level 1 (src/level.cpp:182-237).
+// P columns: w[cc][n][s] (test vector cc), coarse dof 2*cc + s, coarse site =
+// aggregation block of n.
+// ---------------------------------------------------------------------------
+
+// out[X][2cc+s] = sum_{n in X} conj(w_cc[n][s]) v[n][s]; one workgroup per (X, rhs)
+template <typename R, int NV>
+__global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* __restrict__ w,
+                                                  const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
+                                                  int dofc, int nsc) {
+    __shared__ cplx<R> red[4 * 2 * NV];
+    const int X = blockIdx.x, rhs = blockIdx.y;
+    const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
+    cplx<R> acc[2 * NV];
+#pragma unroll
+    for (int q = 0; q < 2 * NV; q++) acc[q] = czero<R>();
+    for (int m = threadIdx.x; m < g.spb; m += FT) {
+        const int n = agg_site(g, X, m);
+        const cplx<R> v0 = v[off + 2 * n], v1 = v[off + 2 * n + 1];
+#pragma unroll
+        for (int cc = 0; cc < NV; cc++) {
+            if (cc < nv) {
+                acc[2 * cc] += mulc(v0, w[cc * nvec + 2 * n]);
+                acc[2 * cc + 1] += mulc(v1, w[cc * nvec + 2 * n + 1]);
+            }
+        }
+    }
+    block_sum<2 * NV>(acc, red, 2 * nv);
+    if (threadIdx.x < 2 * nv) out[(size_t)rhs * nsc * dofc + (size_t)X * dofc + threadIdx.x] = acc[threadIdx.x];
+}
+
+// out[n][s] (+)= sum_cc w_cc[n][s] vc[X(n)][2cc+s]
+template <typename R>
+__global__ void __launch_bounds__(FT) k_prolong0(Geo g, int nv, const cplx<R>* __restrict__ w,
+                                                 const cplx<R>* __restrict__ vc, cplx<R>* __restrict__ out,
+                                                 int dofc, int nsc, int accumulate) {
+    const int n = blockIdx.x * FT + threadIdx.x;
+    if (n >= g.ns) return;
+    const int rhs = blockIdx.y;
+    const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
+    const cplx<R>* c = vc + (size_t)rhs * nsc * dofc + (size_t)agg_block_of(g, n) * dofc;
+    cplx<R> o0 = czero<R>(), o1 = czero<R>();
+    for (int cc = 0; cc < nv; cc++) {
+        o0 += w[cc * nvec + 2 * n] * c[2 * cc];
+        o1 += w[cc * nvec + 2 * n + 1] * c[2 * cc + 1];
+    }
+    if (accumulate) {
+        out[off + 2 * n] += o0;
+        out[off + 2 * n + 1] += o1;
+    } else {
+        out[off + 2 * n] = o0;
+        out[off + 2 * n + 1] = o1;
+    }
+}
+
+// precision conversion / scaling of whole vectors
+template <typename To, typename From>
+__global__ void k_convert(size_t n, const cplx<From>* __restrict__ in, cplx<To>* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cvt<To>(in[i]);
+}
+template <typename R> __global__ void k_zero(size_t n, cplx<R>* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = czero<R>();
+}
+
+}  // namespace amgs

@@ -1,0 +1,276 @@
+// Outer FGMRES on the fine lattice (FP64, src/fgmres.cpp:5-115 with the
+// FGMRES_AMG operator and preconditioner, include/amg.h:99-145).  An item is
+// one right-hand side; heavy kernels cover the lattice, scalar kernels run one
+// workgroup per right-hand side.  Basis vectors are normalised explicitly.
+#pragma once
+#include "fine.hpp"
+
+namespace amgs {
+
+struct OuterState {
+    int *maxIt, *jbreak, *done, *active, *iters, *nhist;
+    double *normb, *s, *hist;  // s [rhs][M+1]; hist [rhs][hcap]
+    c64 *R, *G, *g, *cn, *sn, *eta;  // R [rhs][M][M], G [rhs][M+1][M+1]
+    c64* qa;     // [rhs][ntile][2M+1]
+    double* qb;  // [rhs][ntile]
+    double* qr;  // [rhs][ntile][2]
+    int M, ntile, hcap;
+};
+
+// r = b - D x with partials |r|^2, |b|^2 (src/fgmres.cpp:10-11, 60-62)
+__global__ void __launch_bounds__(FT) k_outer_resid(Geo g, double mm, const c64* __restrict__ U,
+                                                    const c64* __restrict__ b, const c64* __restrict__ x,
+                                                    c64* __restrict__ r, OuterState st) {
+    __shared__ c64 red[8];
+    const int rhs = blockIdx.y;
+    if (st.done[rhs]) return;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int n = tile * FT + threadIdx.x;
+    const size_t off = (size_t)rhs * g.ns * 2;
+    c64 q[2] = {{0, 0}, {0, 0}};
+    if (n < g.ns) {
+        c64 o0, o1;
+        wilson_site<double, false>(U, x + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
+        const c64 b0 = b[off + 2 * n], b1 = b[off + 2 * n + 1];
+        const c64 r0 = b0 - o0, r1 = b1 - o1;
+        r[off + 2 * n] = r0;
+        r[off + 2 * n + 1] = r1;
+        q[0].re = norm2(r0) + norm2(r1);
+        q[1].re = norm2(b0) + norm2(b1);
+    }
+    block_sum<2>(q, red);
+    if (threadIdx.x == 0) {
+        st.qr[((size_t)rhs * st.ntile + tile) * 2] = q[0].re;
+        st.qr[((size_t)rhs * st.ntile + tile) * 2 + 1] = q[1].re;
+    }
+}
+
+// Cycle start k: error estimate, convergence test of the previous cycle,
+// V_0 scale (src/fgmres.cpp:61-67, 16-18).  One workgroup per rhs.
+__global__ void __launch_bounds__(FT) k_outer_start(OuterState st, int k, double tol, int m) {
+    __shared__ c64 red[8];
+    const int rhs = blockIdx.x;
+    if (st.done[rhs]) {
+        if (threadIdx.x == 0) st.active[rhs] = 0;
+        return;
+    }
+    c64 q[2] = {{0, 0}, {0, 0}};
+    for (int t = threadIdx.x; t < st.ntile; t += FT) {
+        q[0].re += st.qr[((size_t)rhs * st.ntile + t) * 2];
+        q[1].re += st.qr[((size_t)rhs * st.ntile + t) * 2 + 1];
+    }
+    block_sum<2>(q, red);
+    if (threadIdx.x != 0) return;
+    const double err = sqrt(q[0].re);
+    if (k == 0) st.normb[rhs] = sqrt(q[1].re);
+    const double nb = st.normb[rhs];
+    if (nb == 0.0) {  // zero right-hand side: x = 0 is exact
+        st.done[rhs] = 1;
+        st.active[rhs] = 0;
+        st.iters[rhs] = 0;
+        return;
+    }
+    if (k > 0 && err < tol * nb) {
+        st.done[rhs] = 1;
+        st.active[rhs] = 0;
+        st.iters[rhs] = (k - 1) * m + st.maxIt[rhs];
+        return;
+    }
+    if (k == 0) st.maxIt[rhs] = m;
+    st.s[(size_t)rhs * (st.M + 1)] = 1.0 / err;
+    st.g[(size_t)rhs * (st.M + 1)] = c64{err, 0.0};
+    st.jbreak[rhs] = -1;
+    st.active[rhs] = 1;
+}
+
+// V_j = s_j * src (V_0 from r, V_{j+1} from w)
+__global__ void __launch_bounds__(FT) k_outer_scale(int ns, const c64* __restrict__ src, c64* __restrict__ dst,
+                                                    OuterState st, int j) {
+    const int rhs = blockIdx.y;
+    if (!st.active[rhs]) return;
+    const int n = blockIdx.x * FT + threadIdx.x;
+    if (n >= ns) return;
+    const double s = st.s[(size_t)rhs * (st.M + 1) + j];
+    const size_t off = (size_t)rhs * ns * 2;
+    dst[off + 2 * n] = s * src[off + 2 * n];
+    dst[off + 2 * n + 1] = s * src[off + 2 * n + 1];
+}
+
+// w = D Z_j; partials <w, V_i> (i <= j), <V_j, V_k> (k < j)
+__global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __restrict__ U, const c64* __restrict__ Zj,
+                                                const c64* __restrict__ V, c64* __restrict__ w, OuterState st, int j,
+                                                size_t vstride) {
+    __shared__ c64 red[4 * (2 * OUT_MAXM + 1)];
+    const int rhs = blockIdx.y;
+    if (!st.active[rhs]) return;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int n = tile * FT + threadIdx.x;
+    const size_t off = (size_t)rhs * g.ns * 2;
+    const int nq = 2 * j + 1;
+    c64 q[2 * OUT_MAXM + 1];
+    for (int i = 0; i < nq; i++) q[i] = c64{0, 0};
+    if (n < g.ns) {
+        c64 o0, o1;
+        wilson_site<double, false>(U, Zj + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
+        w[off + 2 * n] = o0;
+        w[off + 2 * n + 1] = o1;
+        const c64* vj = V + (size_t)j * vstride + off;
+        const c64 a0 = vj[2 * n], a1 = vj[2 * n + 1];
+        for (int i = 0; i <= j; i++) {
+            const c64* vi = V + (size_t)i * vstride + off;
+            const c64 b0 = vi[2 * n], b1 = vi[2 * n + 1];
+            q[i] = mulc(o0, b0) + mulc(o1, b1);
+            if (i < j) q[j + 1 + i] = mulc(a0, b0) + mulc(a1, b1);
+        }
+    }
+    block_sum<2 * OUT_MAXM + 1>(q, red, nq);
+    if (threadIdx.x < nq) st.qa[((size_t)rhs * st.ntile + tile) * (2 * st.M + 1) + threadIdx.x] = q[threadIdx.x];
+}
+
+// Hessenberg column j by one-pass MGS: H_ij = <w,V_i> - sum_{k<i} H_kj <V_k,V_i>.
+// Stores the unrotated column in R[.][j] temporarily (rotated in k_outer_fin).
+__global__ void __launch_bounds__(FT) k_outer_hcol(OuterState st, int j) {
+    __shared__ c64 d[2 * OUT_MAXM + 1];
+    const int rhs = blockIdx.x;
+    if (!st.active[rhs]) return;
+    const int nq = 2 * j + 1, M = st.M;
+    for (int q = threadIdx.x; q < nq; q += FT) {
+        c64 s = {0, 0};
+        for (int t = 0; t < st.ntile; t++) s += st.qa[((size_t)rhs * st.ntile + t) * (2 * M + 1) + q];
+        d[q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    c64* G = st.G + (size_t)rhs * (M + 1) * (M + 1);
+    c64* Rm = st.R + (size_t)rhs * M * M;
+    c64 col[OUT_MAXM + 1];
+    for (int i = 0; i <= j; i++) {
+        c64 h = d[i];
+        for (int kk = 0; kk < i; kk++) {
+            const c64 Gik = (i == j) ? d[j + 1 + kk] : G[i * (M + 1) + kk];
+            h -= col[kk] * conj(Gik);
+        }
+        col[i] = h;
+        Rm[i * M + j] = h;
+    }
+    for (int kk = 0; kk < j; kk++) G[j * (M + 1) + kk] = d[j + 1 + kk];
+}
+
+// w -= sum_i H_ij V_i; partial |w|^2
+__global__ void __launch_bounds__(FT) k_outer_b(int ns, const c64* __restrict__ V, c64* __restrict__ w, OuterState st,
+                                                int j, size_t vstride) {
+    __shared__ c64 red[4];
+    __shared__ c64 H[OUT_MAXM + 1];
+    const int rhs = blockIdx.y;
+    if (!st.active[rhs]) return;
+    const int M = st.M;
+    if (threadIdx.x <= j) H[threadIdx.x] = st.R[(size_t)rhs * M * M + threadIdx.x * M + j];
+    __syncthreads();
+    const int tile = blockIdx.x, n = tile * FT + threadIdx.x;
+    const size_t off = (size_t)rhs * ns * 2;
+    double nn = 0;
+    if (n < ns) {
+        c64 w0 = w[off + 2 * n], w1 = w[off + 2 * n + 1];
+        for (int i = 0; i <= j; i++) {
+            const c64* vi = V + (size_t)i * vstride + off;
+            w0 -= H[i] * vi[2 * n];
+            w1 -= H[i] * vi[2 * n + 1];
+        }
+        w[off + 2 * n] = w0;
+        w[off + 2 * n + 1] = w1;
+        nn = norm2(w0) + norm2(w1);
+    }
+    c64 q[1] = {{nn, 0.0}};
+    block_sum<1>(q, red);
+    if (threadIdx.x == 0) st.qb[(size_t)rhs * st.ntile + tile] = q[0].re;
+}
+
+// H_{j+1,j} = |w|, Givens rotation, residual estimate, break test (src/fgmres.cpp:35-49)
+__global__ void __launch_bounds__(FT) k_outer_fin(OuterState st, int j, double tol) {
+    __shared__ c64 red[4];
+    const int rhs = blockIdx.x;
+    if (!st.active[rhs]) return;
+    const int M = st.M;
+    c64 q[1] = {{0, 0}};
+    for (int t = threadIdx.x; t < st.ntile; t += FT) q[0].re += st.qb[(size_t)rhs * st.ntile + t];
+    block_sum<1>(q, red);
+    if (threadIdx.x != 0) return;
+    const double hn = sqrt(q[0].re);
+    c64* Rm = st.R + (size_t)rhs * M * M;
+    c64* cn = st.cn + (size_t)rhs * M;
+    c64* sn = st.sn + (size_t)rhs * M;
+    c64* g = st.g + (size_t)rhs * (M + 1);
+    c64 col[OUT_MAXM + 1];
+    for (int i = 0; i <= j; i++) col[i] = Rm[i * M + j];
+    col[j + 1] = c64{hn, 0.0};
+    for (int i = 0; i < j; i++) {
+        c64 t = conj(cn[i]) * col[i] + conj(sn[i]) * col[i + 1];
+        col[i + 1] = -(sn[i] * col[i]) + cn[i] * col[i + 1];
+        col[i] = t;
+    }
+    const double den = sqrt(norm2(col[j]) + norm2(col[j + 1]));
+    const c64 snj = {col[j + 1].re / den, col[j + 1].im / den};
+    const c64 cnj = {col[j].re / den, col[j].im / den};
+    col[j] = conj(cnj) * col[j] + conj(snj) * col[j + 1];
+    for (int i = 0; i <= j; i++) Rm[i * M + j] = col[i];
+    cn[j] = cnj;
+    sn[j] = snj;
+    const c64 gj = g[j];
+    g[j + 1] = -(snj * gj);
+    g[j] = conj(cnj) * gj;
+    st.s[(size_t)rhs * (M + 1) + j + 1] = hn > 0 ? 1.0 / hn : 0.0;
+    const double res = sqrt(norm2(g[j + 1]));
+    if (st.nhist[rhs] < st.hcap) st.hist[(size_t)rhs * st.hcap + st.nhist[rhs]] = res;
+    st.nhist[rhs]++;
+    if (res < tol * st.normb[rhs]) {
+        st.maxIt[rhs] = j + 1;
+        st.jbreak[rhs] = j;
+        st.active[rhs] = 0;
+    }
+}
+
+// back substitution with maxIt (src/fgmres.cpp:52-53, 107-115); re-activates
+// every rhs that is not done for the update/residual kernels
+__global__ void k_outer_backsolve(OuterState st, int nrhs) {
+    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rhs >= nrhs || st.done[rhs]) return;
+    const int M = st.M, mi = st.maxIt[rhs];
+    const c64* Rm = st.R + (size_t)rhs * M * M;
+    const c64* g = st.g + (size_t)rhs * (M + 1);
+    c64* eta = st.eta + (size_t)rhs * M;
+    for (int i = mi - 1; i >= 0; i--) {
+        c64 e = g[i];
+        for (int q = i + 1; q < mi; q++) e -= Rm[i * M + q] * eta[q];
+        eta[i] = cdiv(e, Rm[i * M + i]);
+    }
+    st.active[rhs] = 1;
+}
+
+// x += sum_{j < maxIt} eta_j Z_j (src/fgmres.cpp:55-59)
+__global__ void __launch_bounds__(FT) k_outer_update(int ns, const c64* __restrict__ Z, c64* __restrict__ x,
+                                                     OuterState st, size_t zstride) {
+    const int rhs = blockIdx.y;
+    if (st.done[rhs]) return;
+    const int n = blockIdx.x * FT + threadIdx.x;
+    if (n >= ns) return;
+    const int M = st.M, mi = st.maxIt[rhs];
+    const size_t off = (size_t)rhs * ns * 2;
+    c64 x0 = x[off + 2 * n], x1 = x[off + 2 * n + 1];
+    for (int j = 0; j < mi; j++) {
+        const c64 e = st.eta[(size_t)rhs * M + j];
+        const c64* zj = Z + (size_t)j * zstride + off;
+        x0 = x0 + e * zj[2 * n];
+        x1 = x1 + e * zj[2 * n + 1];
+    }
+    x[off + 2 * n] = x0;
+    x[off + 2 * n + 1] = x1;
+}
+
+// after the last cycle: non-converged count restarts*m (src/fgmres.cpp:84-87)
+__global__ void k_outer_final(OuterState st, int nrhs, int restarts, int m) {
+    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rhs >= nrhs) return;
+    if (!st.done[rhs]) st.iters[rhs] = restarts * m;
+}
+
+}  // namespace amgs

@@ -1,0 +1,156 @@
+/*
+ * amgs.h - C ABI of libamgs, the MI355X-native DD-alpha-AMG solver for the
+ * 2-D Schwinger-model Wilson-Dirac operator.
+ *
+ * This is the drop-in boundary for the reference's solve path
+ * (Fabian2598/AMG_SchwingerModel): FGMRES with a K-cycle multigrid
+ * preconditioner (src/fgmres.cpp:5-88, include/amg.h:99-145, src/amg.cpp:3-178).
+ * The reference has no FFI; its in-process API is the C++ class contract
+ * FGMRES_AMG(dim1, dim2, m, restarts, tol, GaugeConf, m0) + fgmres(b, x0, x)
+ * and its file surface (parameters.dat, x t mu re im gauge/RHS files,
+ * Residuals_FGMRES_*.res).  Each entry point below names what it replaces.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles;
+ *   - a level-l vector is [nsites_l][dof_l], site n = x*nt_l + t,
+ *     dof = 2*colour + spin (src/level.cpp:166); the fine level has dof 2;
+ *   - gauge links U[n][mu], mu = 0 time, mu = 1 space (include/boundary.h:18-21);
+ *   - batched vectors are [nrhs][nsites][dof] (right-hand side outermost);
+ *   - host pointers unless the name ends in _dev (device pointers on the
+ *     context's HIP device, visible to the context's stream);
+ *   - return value: AMGS_OK (0), AMGS_NOT_CONVERGED (1, results still
+ *     written) or a negative error code; amgs_last_error() describes it.
+ *   - one context = one hierarchy, one HIP stream, no global state; calls on
+ *     one context must not overlap.
+ */
+#ifndef AMGS_H
+#define AMGS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMGS_OK 0
+#define AMGS_NOT_CONVERGED 1
+#define AMGS_ERR_ARG (-1)
+#define AMGS_ERR_GEOMETRY (-2)
+#define AMGS_ERR_HIP (-3)
+#define AMGS_ERR_STATE (-4)
+#define AMGS_ERR_IO (-5)
+#define AMGS_ERR_NAN (-6)
+
+#define AMGS_MAX_LEVELS 6
+
+/* Arithmetic of the inner levels.  The outer FGMRES, its Hessenberg
+ * least-squares problem and the true-residual check are FP64 in every mode. */
+#define AMGS_PREC_FP64 0   /* plain FP64 everywhere: the drop-in */
+#define AMGS_PREC_MIXED 1  /* SAP, P/P^dagger and coarse levels in FP32 */
+
+typedef struct {
+    /* lattice and hierarchy: CMakeLists.txt:22-27 (NS, NT, LEVELS) and the
+     * parameters.dat rows "level bx bt ntest sapx sapt" (block COUNTS,
+     * include/params.h:30-46) for levels 0..levels-2 */
+    int nx, nt, levels;
+    int bx[AMGS_MAX_LEVELS], bt[AMGS_MAX_LEVELS], ntest[AMGS_MAX_LEVELS];
+    int sapx[AMGS_MAX_LEVELS], sapt[AMGS_MAX_LEVELS];
+    double m0;                                       /* mass::m0, src/main.cpp:64 */
+    int nu1, nu2;                                    /* src/variables.cpp:63-64   */
+    int sap_m, sap_restarts; double sap_gmres_tol;   /* src/variables.cpp:47-49   */
+    double sap_tol;                                  /* src/variables.cpp:50      */
+    int kc_m, kc_restarts; double kc_tol;            /* src/variables.cpp:70-72   */
+    int crs_m, crs_restarts; double crs_tol;         /* include/params.h:63-65    */
+    int out_m, out_restarts; double out_tol;         /* src/variables.cpp:79-81   */
+    int tv_sap_iters;                                /* src/main.cpp:63           */
+    uint32_t tv_seed;                                /* src/amg.cpp:8 (50)        */
+    int precision;                                   /* AMGS_PREC_*               */
+    int max_rhs;                                     /* batch capacity (>= 1)     */
+} amgs_params;
+
+typedef struct amgs_ctx amgs_ctx;
+
+/* Reference defaults (src/variables.cpp:46-87) with main()'s overrides
+ * (src/main.cpp:61-64,129: K-cycle, 4 SAP sweeps on test vectors, Nit = 0). */
+int amgs_default_params(amgs_params* p);
+
+/* Reads the 6-column parameters.dat (include/params.h:17-68) into p->rows and
+ * sets p->levels = rows + 1.  nx, nt are not touched. */
+int amgs_read_parameters(const char* path, amgs_params* p);
+
+/* Replaces the FGMRES_AMG constructor's allocation part (include/amg.h:101-104,
+ * include/level.h:91-134).  Validates the blocking; allocates device memory. */
+int amgs_create(const amgs_params* p, amgs_ctx** out);
+void amgs_destroy(amgs_ctx* c);
+const char* amgs_last_error(const amgs_ctx* c);
+
+/* GaugeConf::read_conf / setGconf (src/gauge_conf.cpp:18-32): U [nx*nt][2]. */
+int amgs_set_gauge(amgs_ctx* c, const double* U);
+
+/* AlgebraicMG::setUpPhase(Nit = 0) (src/amg.cpp:3-46): test vectors from
+ * mt19937(tv_seed), SAP smoothing, aggregate orthonormalisation, Galerkin
+ * coarse links, on the GPU. */
+int amgs_setup(amgs_ctx* c);
+
+/* Setup from caller-given raw test vectors tv[l] = [ntest_l][nsites_l][dof_l]
+ * (levels 0..levels-2), bypassing the generator; smooth = 0 skips SAP. */
+int amgs_setup_from_tv(amgs_ctx* c, const double* const* tv, int smooth);
+
+/* FGMRES_AMG::fgmres(b, x0 = 0, x) (src/fgmres.cpp:5-88) for nrhs right-hand
+ * sides at once; every right-hand side follows its own iteration exactly as a
+ * single solve would.  iters[nrhs] receives the reference's count
+ * (k*m + maxIt, or restarts*m); relres[nrhs] the FP64 true relative residual;
+ * hist (may be NULL) [nrhs][hist_cap] the |g_{j+1}| history of each solve,
+ * nhist[nrhs] its length (the Residuals_FGMRES_*.res content,
+ * src/fgmres.cpp:65-79).  Returns AMGS_NOT_CONVERGED if any did not converge. */
+int amgs_solve(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, double* relres,
+               double* hist, int hist_cap, int* nhist);
+int amgs_solve_dev(amgs_ctx* c, int nrhs, const void* b_dev, void* x_dev, int* iters, double* relres);
+
+/* ---- component entry points (the reference's building blocks, used by the
+ *      parity tests and by callers that assemble their own solvers) ---- */
+
+/* D_phi / D_dagger_phi (src/dirac_operator.cpp:3-54) at level 0 and
+ * Level::D_operator (src/level.cpp:161-180) at level >= 1 (dagger must be 0). */
+int amgs_apply_D(amgs_ctx* c, int level, int dagger, int nrhs, const double* in, double* out);
+int amgs_apply_D_dev(amgs_ctx* c, int level, int dagger, int nrhs, const void* in_dev, void* out_dev);
+/* Level::P_v and Level::Pt_v (src/level.cpp:182-237), level = the finer one. */
+int amgs_apply_P(amgs_ctx* c, int level, int nrhs, const double* coarse, double* fine);
+int amgs_apply_Pt(amgs_ctx* c, int level, int nrhs, const double* fine, double* coarse);
+/* SAP_C::SAP(v, x, nu) (src/sap.cpp:64-174); x is updated in place. */
+int amgs_sap(amgs_ctx* c, int level, int nrhs, const double* v, double* x, int nu);
+/* One preconditioner application psi = K(eta) from psi = 0 (include/amg.h:134-144). */
+int amgs_kcycle(amgs_ctx* c, int nrhs, const double* eta, double* psi);
+/* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
+ * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as
+ * [nsites_l][dof_l][dof_l] (row = 2c+alpha, col = 2b+beta). */
+int amgs_get_tv(amgs_ctx* c, int level, int i, double* out);
+int amgs_get_links(amgs_ctx* c, int level, int which, int mu, double* out);
+
+/* ---- workload generators (new: the reference only reads files) ---- */
+/* Quenched U(1) plaquette-action Metropolis from a cold start; deterministic
+ * in (nx, nt, beta, sweeps, seed); writes U [nx*nt][2]; *plaq = <Re U_p>. */
+int amgs_gauge_metropolis(int nx, int nt, double beta, int sweeps, uint64_t seed, double* U, double* plaq);
+/* Gaussian complex vector: re, im ~ N(0, 1), deterministic in (n, seed). */
+int amgs_gaussian(int64_t ncomplex, uint64_t seed, double* out);
+
+/* ---- file formats (src/gauge_conf.cpp:25-28, src/variables.cpp:117-155, 101-115) ---- */
+int amgs_read_conf(const char* path, int nx, int nt, double* U);
+int amgs_write_conf(const char* path, int nx, int nt, const double* U);
+int amgs_read_rhs(const char* path, int nx, int nt, double* b);
+int amgs_write_rhs(const char* path, int nx, int nt, const double* b);
+int amgs_write_residuals(const char* path, const double* hist, int n);
+
+/* ---- measurement ---- */
+/* Average device time (ms) of `reps` back-to-back launches of one kernel on
+ * the context's stream, bracketed by HIP events on that stream.
+ * kernel: 0 = fine D (FP64), 1 = fine D (FP32), 2 = SAP block GMRES sweep,
+ * 3 = coarse solve.  Inputs are whatever the context holds. */
+int amgs_time_kernel(amgs_ctx* c, int kernel, int nrhs, int reps, double* avg_ms);
+/* Phase timers of the last solve (ms): [0] total, [1] setup, [2] solve. */
+int amgs_last_timings(amgs_ctx* c, double* ms3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

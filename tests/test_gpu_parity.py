@@ -1,0 +1,139 @@
+"""GPU (libamgs, HIP on gfx950) against the CPU oracle on the same inputs.
+
+Tolerances follow the reference's own parity bar: operator applies 1e-13
+relative in FP64; full solves the same iteration count +-1 and solutions
+within 1e-9 relative.  Set-up results (test vectors, coarse links) carry the
+rounding of many reductions in a different order, so they are compared at
+1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import amg_schwingermodel_amd as amgs
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (nx, nt, rows, m0, beta)
+    "32_3lvl": (32, 32, [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)], -0.10, 2.0),
+    "64_2lvl": (64, 64, [(0, 4, 4, 10, 4, 4)], -0.18, 2.0),
+    "32_4lvl_odd": (32, 32, [(0, 4, 2, 3, 2, 1), (1, 2, 2, 2, 1, 2), (2, 1, 2, 2, 1, 1)], -0.05, 3.0),
+}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def make(case, setup=True, **kw):
+    nx, nt, rows, m0, beta = CASES[case]
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, **kw), U)
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    if setup:
+        s.setup()
+        o.setup(draw_order=1)
+    return s, o, rows
+
+
+def cvec(n, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(n) + 1j * r.standard_normal(n)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fine_operator(case):
+    s, o, _ = make(case, setup=False)
+    v = cvec(s.size(0), 1)
+    assert rel(s.apply_D(0, v), o.dphi(v)) < 1e-13
+    assert rel(s.apply_D(0, v, dagger=True), o.ddag(v)) < 1e-13
+    # batched apply equals single applies
+    V = np.stack([cvec(s.size(0), 2 + i) for i in range(3)])
+    out = s.apply_D(0, V, nrhs=3).reshape(3, -1)
+    for i in range(3):
+        assert rel(out[i], o.dphi(V[i])) < 1e-13
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_setup_hierarchy(case):
+    s, o, rows = make(case)
+    L = len(rows) + 1
+    for l in range(L - 1):
+        for i in range(rows[l][3]):
+            assert rel(s.tv(l, i), o.tv(l, i)) < 1e-9, (l, i)
+    for l in range(1, L):
+        for which in range(3):
+            for mu in range(2 if which else 1):
+                assert rel(s.links(l, which, mu), o.links(l, which, mu)) < 1e-9, (l, which, mu)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_level_operators_from_same_tv(case):
+    """Same raw test vectors on both sides, no smoothing: orthonormalisation,
+    coarse links, P, P^dagger and the coarse stencils at operator tolerance."""
+    s, o, rows = make(case, setup=False)
+    L = len(rows) + 1
+    tvs = [np.concatenate([cvec(o.size(l), 100 * l + i) for i in range(rows[l][3])]) for l in range(L - 1)]
+    s.setup_from_tv(tvs, smooth=False)
+    o.setup_from_tv(tvs, smooth=False)
+    for l in range(L):
+        v = cvec(s.size(l), 7 + l)
+        assert rel(s.apply_D(l, v), o.dop(l, v)) < 1e-12, l
+        if l < L - 1:
+            vc = cvec(s.size(l + 1), 17 + l)
+            assert rel(s.apply_P(l, vc), o.prolong(l, vc)) < 1e-12, l
+            assert rel(s.apply_Pt(l, v), o.restrict(l, v)) < 1e-12, l
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_sap(case):
+    s, o, rows = make(case)
+    L = len(rows) + 1
+    for l in range(L - 1):
+        v = cvec(s.size(l), 31 + l)
+        x0 = cvec(s.size(l), 41 + l)
+        xs = s.sap(l, v, x0, 2)
+        xo, _ = o.sap(l, v, x0, 2)
+        assert rel(xs, xo) < 1e-9, l
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_kcycle(case):
+    s, o, _ = make(case)
+    eta = cvec(s.size(0), 51)
+    assert rel(s.kcycle(eta), o.kcycle0(eta)) < 1e-9
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_full_solve(case):
+    s, o, _ = make(case)
+    b = cvec(s.size(0), 61)
+    x, it, rr, hist = s.solve(b)
+    xo, ito, histo, rro = o.solve(b)
+    assert abs(it - ito) <= 1, (it, ito)
+    assert rr < 1e-10
+    assert rel(x, xo) < 1e-9
+    k = min(len(hist), len(histo), 5)
+    np.testing.assert_allclose(hist[:k], histo[:k], rtol=1e-6)
+
+
+def test_batched_solve_matches_single():
+    s, o, _ = make("32_3lvl", max_rhs=4)
+    B = np.stack([cvec(s.size(0), 70 + i) for i in range(3)])
+    X, its, rrs, _ = s.solve(B)
+    for i in range(3):
+        x1, it1, rr1, _ = s.solve(B[i])
+        assert its[i] == it1
+        assert rel(X[i], x1) < 1e-12
+        assert rrs[i] < 1e-10
+
+
+def test_mixed_precision_converges():
+    s, o, _ = make("64_2lvl", precision=amgs.PREC_MIXED, max_rhs=2)
+    b = cvec(s.size(0), 81)
+    x, it, rr, _ = s.solve(b)
+    xo, ito, _, _ = o.solve(b)
+    assert rr < 1e-10
+    assert rel(x, xo) < 1e-8
+    assert it <= ito + 3
