@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -129,7 +130,7 @@ struct Ctx {
 // ---------------------------------------------------------------------------
 static void build_geometry(Ctx& c) {
     const amgs_params& p = c.P;
-    if (p.levels < 2 || p.levels > MAXL) throw ApiError(AMGS_ERR_GEOMETRY, "levels must be in [2, 6]");
+    if (p.levels < 2 || p.levels > MAXL) throw ApiError(AMGS_ERR_GEOMETRY, "levels must be in [2, 5]");
     if (p.nx < 2 || p.nt < 2) throw ApiError(AMGS_ERR_GEOMETRY, "lattice too small");
     c.L = p.levels;
     for (int l = 0; l < c.L; l++) {
@@ -296,11 +297,26 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     a.mode = mode;
     a.level = level;
     a.nu = nu;
+    // coarsest GMRES vectors in dynamic LDS if they fit beside the static part
+    const size_t ncrs = c->nvec(c->L - 1);
+    const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
+    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
+    a.crs_lds = dyn <= lds_cap ? 1 : 0;
+    const size_t dyn_bytes = a.crs_lds ? dyn : 0;
     a.in = in;
     a.out = out;
     a.in_stride = in_stride;
     a.out_stride = out_stride;
-    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), 0, c->stream, a);
+    if (dyn_bytes > 0) {
+        static bool attr_set[2] = {false, false};
+        bool& done = attr_set[sizeof(R) == 8];
+        if (!done) {
+            HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(160 * 1024 - sizeof(CLds))));
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, a);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -563,6 +579,10 @@ static void outer_solve(Ctx& c, int nrhs) {
     HIP_CHECK(hipMemsetAsync(st.iters, 0, sizeof(int) * nrhs, s));
     const dim3 g2(ntile, nrhs);
     std::vector<int> flags(nrhs);
+    const char* ve = std::getenv("AMGS_VERBOSE");
+    const bool verbose = ve && ve[0] && ve[0] != '0';
+    const auto tstart = std::chrono::steady_clock::now();
+    int step = 0;
     hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
                        (const c64*)x, r, st);
     for (int k = 0; k < c.P.out_restarts; k++) {
@@ -583,6 +603,20 @@ static void outer_solve(Ctx& c, int nrhs) {
                                    st, j + 1);
             HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
             c.sync();
+            step++;
+            if (verbose) {
+                int nh = 0;
+                double h = 0, nb = 0;
+                HIP_CHECK(hipMemcpy(&nh, st.nhist, sizeof(int), hipMemcpyDeviceToHost));
+                if (nh > 0 && nh <= st.hcap) HIP_CHECK(hipMemcpy(&h, st.hist + nh - 1, sizeof(double), hipMemcpyDeviceToHost));
+                HIP_CHECK(hipMemcpy(&nb, st.normb, sizeof(double), hipMemcpyDeviceToHost));
+                const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+                int act = 0;
+                for (int f : flags) act += f != 0;
+                std::fprintf(stderr, "[amgs] cycle %d step %d (total %d): |g|/|b| rhs0 = %.3e, active %d/%d, %.1f ms\n", k, j,
+                             step, nb > 0 ? h / nb : 0.0, act, nrhs, el);
+                std::fflush(stderr);
+            }
             if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
         }
         hipLaunchKernelGGL(k_outer_backsolve, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs);

@@ -18,11 +18,11 @@ namespace amgs {
 
 constexpr int CT = 512;        // threads per coarse workgroup
 constexpr int CW = CT / 64;    // wavefronts per coarse workgroup
-constexpr int MAXL = 6;        // maximum number of levels
+constexpr int MAXL = 5;        // maximum number of levels
 constexpr int KC_MAXM = 8;     // max restart length of K-cycle FGMRES and SAP GMRES
 constexpr int CRS_MAXM = 24;   // max restart length of the coarsest GMRES
 
-enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4, CM_KCYCLE = 5 };
+enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4 };
 
 template <typename R> struct CLevel {
     Geo g;
@@ -52,6 +52,7 @@ template <typename R> struct CArgs {
     int nu1, nu2;
     cplx<R>* ws;
     size_t ws_per_rhs;
+    int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
     int mode, level, nu;
     const cplx<R>* in;
     cplx<R>* out;
@@ -68,6 +69,8 @@ template <int M> struct FgScal {
 struct WaveScal {
     c64 R[KC_MAXM][KC_MAXM];
     c64 g[KC_MAXM + 1], cn[KC_MAXM], sn[KC_MAXM], eta[KC_MAXM];
+    c64 col[KC_MAXM + 1];
+    int brk;
 };
 struct CLds {
     c64 red[CT];
@@ -81,8 +84,9 @@ struct CLds {
 // ---------------------------------------------------------------------------
 template <typename R> struct CK {
     const CArgs<R>& A;
-    cplx<R>* W;  // this rhs's workspace
+    cplx<R>* W;    // this rhs's workspace (global memory)
     CLds& S;
+    cplx<R>* dyn;  // dynamic LDS region
 };
 
 template <typename R> __device__ __forceinline__ c64 c_dot(CLds& S, const cplx<R>* a, const cplx<R>* b, int n) {
@@ -94,7 +98,7 @@ template <typename R> __device__ __forceinline__ c64 c_dot(CLds& S, const cplx<R
 }
 
 // out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))
-template <typename R> __device__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out) {
+template <typename R> __device__ __forceinline__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, n = g.ns * D;
@@ -128,7 +132,7 @@ template <typename R> __device__ void c_dop(const CArgs<R>& A, int l, const cplx
 
 // out = P^dagger_l vf (src/level.cpp:212-237); one thread per coarse entry,
 // summed over the aggregate in the reference order (x, t, colour).
-template <typename R> __device__ void c_restrict(const CArgs<R>& A, int l, const cplx<R>* vf, cplx<R>* out) {
+template <typename R> __device__ __forceinline__ void c_restrict(const CArgs<R>& A, int l, const cplx<R>* vf, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int Dc = A.lv[l + 1].g.dof, nc = A.lv[l + 1].g.ns * Dc, D = g.dof;
@@ -152,7 +156,7 @@ template <typename R> __device__ void c_restrict(const CArgs<R>& A, int l, const
 
 // out (+)= P_l vc (src/level.cpp:182-209)
 template <typename R>
-__device__ void c_prolong(const CArgs<R>& A, int l, const cplx<R>* vc, cplx<R>* out, bool accumulate) {
+__device__ __forceinline__ void c_prolong(const CArgs<R>& A, int l, const cplx<R>* vc, cplx<R>* out, bool accumulate) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, n = g.ns * D, Dc = A.lv[l + 1].g.dof;
@@ -188,7 +192,7 @@ template <typename R> __device__ __forceinline__ c64 w_dot(const cplx<R>* a, con
 
 // block-local stencil, neighbours outside block blk read as zero
 template <typename R>
-__device__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
+__device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63;
@@ -223,23 +227,25 @@ __device__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, c
 }
 
 // Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
-// returns cn_j, sn_j; col[j] becomes the diagonal of R.
+// col, cn, sn live in LDS; one thread executes it.  col[j] becomes the
+// diagonal of R; returns cn_j, sn_j.
 __device__ __forceinline__ void givens_col(c64* col, const c64* cn, const c64* sn, int j, c64& cnj, c64& snj) {
     for (int i = 0; i < j; i++) {
-        c64 t = conj(cn[i]) * col[i] + conj(sn[i]) * col[i + 1];
-        col[i + 1] = -(sn[i] * col[i]) + cn[i] * col[i + 1];
-        col[i] = t;
+        const c64 a = col[i], b = col[i + 1];
+        col[i + 1] = -(sn[i] * a) + cn[i] * b;
+        col[i] = conj(cn[i]) * a + conj(sn[i]) * b;
     }
-    const double den = sqrt(norm2(col[j]) + norm2(col[j + 1]));
-    snj = c64{col[j + 1].re / den, col[j + 1].im / den};
-    cnj = c64{col[j].re / den, col[j].im / den};
-    col[j] = conj(cnj) * col[j] + conj(snj) * col[j + 1];
+    const c64 a = col[j], b = col[j + 1];
+    const double den = sqrt(norm2(a) + norm2(b));
+    snj = c64{b.re / den, b.im / den};
+    cnj = c64{a.re / den, a.im / den};
+    col[j] = conj(cnj) * a + conj(snj) * b;
 }
 
 // GMRES(m, restarts, tol) on D_B for one Schwarz block, x0 = b_B; then
 // x[B] += x_B.  Executed by one wavefront.  True modified Gram-Schmidt.
 template <typename R>
-__device__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob, cplx<R>* x) {
+__device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob, cplx<R>* x) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -275,35 +281,34 @@ __device__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const
         for (int j = 0; j < m; j++) {
             w_dlocal(A, l, blk, V(j), V(j + 1));
             cplx<R>* w = V(j + 1);
-            c64 col[KC_MAXM + 1];
             for (int i = 0; i <= j; i++) {
                 const c64 h = w_dot(w, V(i), nb);
-                col[i] = h;
+                if (lane == 0) S.col[i] = h;
                 const cplx<R> hr = cvt<R>(h);
                 const cplx<R>* vi = V(i);
                 for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
                 wave_sync();
             }
             const double hn = sqrt(w_dot(w, w, nb).re);
-            col[j + 1] = c64{hn, 0.0};
             if (hn > 0) {
                 const R ih = R(1.0 / hn);
                 for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
             }
-            c64 cnj, snj;
-            givens_col(col, S.cn, S.sn, j, cnj, snj);
-            const c64 gj = S.g[j];
-            const c64 gn = -(snj * gj);
-            wave_sync();
             if (lane == 0) {
-                for (int i = 0; i <= j; i++) S.R[i][j] = col[i];
+                S.col[j + 1] = c64{hn, 0.0};
+                c64 cnj, snj;
+                givens_col(S.col, S.cn, S.sn, j, cnj, snj);
+                for (int i = 0; i <= j; i++) S.R[i][j] = S.col[i];
+                const c64 gj = S.g[j];
+                const c64 gn = -(snj * gj);
                 S.cn[j] = cnj;
                 S.sn[j] = snj;
                 S.g[j] = conj(cnj) * gj;
                 S.g[j + 1] = gn;
+                S.brk = sqrt(norm2(gn)) < A.sap_gtol * normb;
             }
             wave_sync();
-            if (sqrt(norm2(gn)) < A.sap_gtol * normb) {
+            if (S.brk) {
                 maxIt = j + 1;
                 break;
             }
@@ -335,7 +340,7 @@ __device__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const
 }
 
 // SAP_C::SAP (src/sap.cpp:64-174): red then black blocks, residual after each colour.
-template <typename R> __device__ void c_sap(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* v, cplx<R>* x, int nu) {
+template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* v, cplx<R>* x, int nu) {
     const CLevel<R>& L = A.lv[l];
     const int n = L.g.ns * L.g.dof;
     cplx<R>* r = K.W + L.sR;
@@ -360,7 +365,7 @@ template <typename R> __device__ void c_sap(const CArgs<R>& A, CK<R>& K, int l, 
 // ---------------------------------------------------------------------------
 // workgroup FGMRES (src/fgmres.cpp:5-88) and the K-cycle recursion
 // ---------------------------------------------------------------------------
-template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi);
+template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi);
 
 // Restarted FGMRES from x0 = b.  crs: the coarsest GMRES (identity
 // preconditioner, GMRES_level_l); otherwise the K-cycle FGMRES whose
@@ -370,17 +375,19 @@ template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>&
 // Scalars (Hessenberg column, Givens, residual estimate) are computed by
 // thread 0 in FP64 and broadcast through LDS.
 template <typename R, int LV, int M>
-__device__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, const cplx<R>* b, cplx<R>* x) {
+__device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, const cplx<R>* b, cplx<R>* x) {
     const CLevel<R>& L = A.lv[LV];
     const int n = L.g.ns * L.g.dof;
     const int m = crs ? A.crs_m : A.kc_m;
     const int restarts = crs ? A.crs_restarts : A.kc_restarts;
     const double tol = crs ? A.crs_tol : A.kc_tol;
-    cplx<R>* Vb = K.W + (crs ? L.crV : L.kcV);
+    // the coarsest GMRES keeps its vectors in LDS when they fit
+    const bool lds = crs && A.crs_lds;
+    cplx<R>* Vb = lds ? K.dyn : K.W + (crs ? L.crV : L.kcV);
     cplx<R>* Zb = crs ? Vb : K.W + L.kcZ;
-    cplx<R>* r = K.W + (crs ? L.crR : L.kcR);
-    cplx<R>* w = K.W + (crs ? L.crW : L.kcW);
-    cplx<R>* Dx = K.W + (crs ? L.crDx : L.kcDx);
+    cplx<R>* r = lds ? K.dyn + (size_t)(m + 1) * n : K.W + (crs ? L.crR : L.kcR);
+    cplx<R>* w = lds ? K.dyn + (size_t)(m + 2) * n : K.W + (crs ? L.crW : L.kcW);
+    cplx<R>* Dx = lds ? K.dyn + (size_t)(m + 3) * n : K.W + (crs ? L.crDx : L.kcDx);
     auto V = [&](int j) { return Vb + (size_t)j * n; };
     auto Z = [&](int j) { return Zb + (size_t)j * n; };
     // setZeros (include/fgmres.h:93-120): Z_j start from zero in the first cycle
@@ -424,16 +431,20 @@ __device__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, co
                 }
                 __syncthreads();
             }
-            if (threadIdx.x == 0) {
+            if (threadIdx.x < 64) {
+                // H_ij = <w,V_i> - sum_{k<i} H_kj conj(G_ik), one wave, k over lanes
+                const int lane = threadIdx.x;
                 for (int i = 0; i <= j; i++) {
-                    c64 h = S.dq[i];
-                    for (int kk = 0; kk < i; kk++) {
+                    c64 t = {0, 0};
+                    for (int kk = lane; kk < i; kk += 64) {
                         const c64 Gik = (i == j) ? S.dq[j + 1 + kk] : S.G[i][kk];
-                        h -= S.col[kk] * conj(Gik);
+                        t += S.col[kk] * conj(Gik);
                     }
-                    S.col[i] = h;
+                    t = wave_sum(t);
+                    if (lane == 0) S.col[i] = S.dq[i] - t;
+                    wave_sync();
                 }
-                for (int kk = 0; kk < j; kk++) S.G[j][kk] = S.dq[j + 1 + kk];
+                for (int kk = lane; kk < j; kk += 64) S.G[j][kk] = S.dq[j + 1 + kk];
             }
             __syncthreads();
             c64 nq1[1] = {{0, 0}};
@@ -451,14 +462,12 @@ __device__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, co
                 for (int e = threadIdx.x; e < n; e += CT) V(j + 1)[e] = ih * w[e];
             }
             if (threadIdx.x == 0) {
-                c64 col[M + 1];
-                for (int i = 0; i <= j; i++) col[i] = S.col[i];
-                col[j + 1] = c64{hn, 0.0};
+                S.col[j + 1] = c64{hn, 0.0};
                 c64 cnj, snj;
-                givens_col(col, S.cn, S.sn, j, cnj, snj);
+                givens_col(S.col, S.cn, S.sn, j, cnj, snj);
+                for (int i = 0; i <= j; i++) S.R[i][j] = S.col[i];
                 const c64 gj = S.g[j];
                 const c64 gn = -(snj * gj);
-                for (int i = 0; i <= j; i++) S.R[i][j] = col[i];
                 S.cn[j] = cnj;
                 S.sn[j] = snj;
                 S.g[j] = conj(cnj) * gj;
@@ -492,7 +501,7 @@ __device__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, co
 }
 
 // k_cycle(l, eta, psi) (src/amg.cpp:135-178)
-template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi) {
+template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi) {
     if (LV == A.L - 1) {
         c_fgmres<R, LV, CRS_MAXM>(A, K, K.S.crs, true, eta, psi);
         return;
@@ -500,52 +509,41 @@ template <typename R, int LV> __device__ void c_kcycle(const CArgs<R>& A, CK<R>&
     if constexpr (LV + 1 < MAXL) {
         const CLevel<R>& L = A.lv[LV];
         const int n = L.g.ns * L.g.dof;
-        if (A.nu1 > 0) c_sap(A, K, LV, eta, psi, A.nu1);
         cplx<R>* Dpsi = K.W + L.kDpsi;
         cplx<R>* r = K.W + L.kR;
         cplx<R>* eta1 = K.W + L.kEta1;
         cplx<R>* psi1 = K.W + L.kPsi1;
-        c_dop(A, LV, psi, Dpsi);
-        for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
-        c_restrict(A, LV, r, eta1);
-        c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
-        c_prolong(A, LV, psi1, psi, true);
-        if (A.nu2 > 0) c_sap(A, K, LV, eta, psi, A.nu2);
-    }
-}
-
-template <typename R> __device__ void c_kcycle_dispatch(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* eta, cplx<R>* psi) {
-    switch (l) {
-        case 1: c_kcycle<R, 1>(A, K, eta, psi); break;
-        case 2: c_kcycle<R, 2>(A, K, eta, psi); break;
-        case 3: c_kcycle<R, 3>(A, K, eta, psi); break;
-        case 4: c_kcycle<R, 4>(A, K, eta, psi); break;
-        default: c_kcycle<R, 5>(A, K, eta, psi); break;
+        for (int ph = 0; ph < 2; ph++) {
+            if (ph == 1) {
+                // coarse-grid correction
+                c_dop(A, LV, psi, Dpsi);
+                for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
+                c_restrict(A, LV, r, eta1);
+                c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
+                c_prolong(A, LV, psi1, psi, true);
+            }
+            const int nu = ph == 0 ? A.nu1 : A.nu2;  // pre- / post-smoothing
+            if (nu > 0) c_sap(A, K, LV, eta, psi, nu);
+        }
     }
 }
 
 template <typename R> __global__ void __launch_bounds__(CT) k_coarse(CArgs<R> A) {
     __shared__ CLds S;
+    extern __shared__ char dyn_lds[];
     const int rhs = blockIdx.x;
-    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S};
+    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds)};
     const cplx<R>* in = A.in + (size_t)rhs * A.in_stride;
     cplx<R>* out = A.out + (size_t)rhs * A.out_stride;
     const int l = A.level;
     switch (A.mode) {
-        case CM_KC_SOLVE:  // FGMRES_k_cycle[l].fgmres(eta, eta, psi) (src/amg.cpp:161)
-            switch (l) {
-                case 1: c_fgmres<R, 1, KC_MAXM>(A, K, S.kc[1], false, in, out); break;
-                case 2: c_fgmres<R, 2, KC_MAXM>(A, K, S.kc[2], false, in, out); break;
-                case 3: c_fgmres<R, 3, KC_MAXM>(A, K, S.kc[3], false, in, out); break;
-                case 4: c_fgmres<R, 4, KC_MAXM>(A, K, S.kc[4], false, in, out); break;
-                default: c_fgmres<R, 5, KC_MAXM>(A, K, S.kc[5], false, in, out); break;
-            }
+        case CM_KC_SOLVE:  // FGMRES_k_cycle[1].fgmres(eta, eta, psi) (src/amg.cpp:161)
+            c_fgmres<R, 1, KC_MAXM>(A, K, S.kc[1], false, in, out);
             break;
         case CM_SAP: c_sap(A, K, l, in, out, A.nu); break;
         case CM_DOP: c_dop(A, l, in, out); break;
         case CM_RESTRICT: c_restrict(A, l, in, out); break;
         case CM_PROLONG: c_prolong(A, l, in, out, false); break;
-        case CM_KCYCLE: c_kcycle_dispatch(A, K, l, in, out); break;
     }
 }
 

@@ -157,6 +157,26 @@ template <typename R> struct SapArgs {
     SapState st;
 };
 
+// Block-wide reduction of per-chunk partial sums: out[q] = sum_c base[c*stride + q]
+// for q < nq, chunks summed in a fixed order (16 threads per quantity, then
+// the 16 partials in order).  All threads must call it.
+template <typename T>
+__device__ __forceinline__ void reduce_chunks(const T* base, size_t stride, int nq, int nchunk, T* tmp, T* out) {
+    const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    if (q < nq) {
+        T s{};
+        for (int c = sub; c < nchunk; c += 16) s += base[(size_t)c * stride + q];
+        tmp[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        T s = tmp[threadIdx.x * 16];
+        for (int i = 1; i < 16; i++) s += tmp[threadIdx.x * 16 + i];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
 struct ItemIdx {
     int item, chunk, rhs, blk, m_lo, m_hi;
 };
@@ -186,15 +206,16 @@ struct ItemLds {
     c64 g[SAP_MAXM + 1];
     c64 eta[SAP_MAXM];
     c64 red[8 * 2 * SAP_MAXM];
+    c64 tmp[FT];           // chunk-reduction scratch
+    c64 dq[2 * SAP_MAXM];  // reduced partial sums
+    double dr[2];
 };
 
 // Finalise Arnoldi step j (src/fgmres.cpp:35-49): H_{j+1,j} from the |w|^2
 // partials, Givens rotation of column j, residual estimate, break test.
 // Executed by thread 0; chunk 0 stores.  Returns through L.
-__device__ void sap_finalize_step(const SapState& st, int item, int nchunk, int M, int j, double tol,
+__device__ void sap_finalize_step(const SapState& st, int item, double hn2, int M, int j, double tol,
                                   bool store, ItemLds& L) {
-    double hn2 = 0;
-    for (int c = 0; c < nchunk; c++) hn2 += st.qb[(size_t)item * nchunk + c];
     const double hn = sqrt(hn2);
     const c64* Hr = st.Hraw + (size_t)item * (M + 1) * M;
     c64 col[SAP_MAXM + 1];
@@ -275,19 +296,44 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
+    if (j == 0) {
+        // |r|^2, |b|^2 partials of the init / residual kernel
+        const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
+        double s = 0;
+        if (q < 2)
+            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qr[((size_t)t.item * a.nchunk + c) * 2 + q];
+        if (q < 2) L.tmp[threadIdx.x].re = s;
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            double h = 0;
+            for (int i = 0; i < 16; i++) h += L.tmp[threadIdx.x * 16 + i].re;
+            L.dq[threadIdx.x].re = h;
+        }
+        __syncthreads();
+    }
+    if (j > 0) {
+        // |w|^2 partials of step j-1
+        const int sub = threadIdx.x;
+        double s = 0;
+        if (sub < 16)
+            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qb[(size_t)t.item * a.nchunk + c];
+        if (sub < 16) L.tmp[sub].re = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double h = 0;
+            for (int i = 0; i < 16; i++) h += L.tmp[i].re;
+            L.dr[0] = h;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         const SapState& st = a.st;
         const bool store = t.chunk == 0;
         int active = !st.done[t.item];
         if (active && j == 0) {
             // cycle start (src/fgmres.cpp:8-17, 61-65)
-            double e2 = 0, b2 = 0;
-            for (int c = 0; c < a.nchunk; c++) {
-                e2 += st.qr[((size_t)t.item * a.nchunk + c) * 2];
-                b2 += st.qr[((size_t)t.item * a.nchunk + c) * 2 + 1];
-            }
-            const double err = sqrt(e2);
-            double nb = k == 0 ? sqrt(b2) : st.normb[t.item];
+            const double err = sqrt(L.dq[0].re);
+            double nb = k == 0 ? sqrt(L.dq[1].re) : st.normb[t.item];
             bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
             if (done) {
                 active = 0;
@@ -304,11 +350,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
         } else if (active) {
             if (st.jbreak[t.item] >= 0) active = 0;
             else {
-                sap_finalize_step(st, t.item, a.nchunk, M, j - 1, a.tol, store, L);
+                sap_finalize_step(st, t.item, L.dr[0], M, j - 1, a.tol, store, L);
                 if (L.broke) active = 0;
             }
         }
-        if (active && j > 0) L.s[j] = L.s[j];  // s_j from the finalisation above
         L.active = active;
     }
     __syncthreads();
@@ -350,16 +395,12 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
+    reduce_chunks(a.st.qa + (size_t)t.item * a.nchunk * 2 * M, (size_t)2 * M, 2 * j + 1, a.nchunk, L.tmp, L.dq);
     if (threadIdx.x == 0) {
         const SapState& st = a.st;
         int active = !st.done[t.item] && st.jbreak[t.item] < 0;
         if (active) {
-            c64 d[2 * SAP_MAXM];
-            for (int q = 0; q < 2 * j + 1; q++) {
-                c64 s = {0, 0};
-                for (int c = 0; c < a.nchunk; c++) s += st.qa[((size_t)t.item * a.nchunk + c) * 2 * M + q];
-                d[q] = s;
-            }
+            const c64* d = L.dq;
             const double* sv = st.s + (size_t)t.item * (M + 1);
             double s[SAP_MAXM + 1];
             for (int i = 0; i <= j; i++) s[i] = sv[i];
@@ -418,6 +459,20 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
+    {
+        const int sub = threadIdx.x;
+        double s = 0;
+        if (sub < 16)
+            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qb[(size_t)t.item * a.nchunk + c];
+        if (sub < 16) L.tmp[sub].re = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double h = 0;
+            for (int i = 0; i < 16; i++) h += L.tmp[i].re;
+            L.dr[0] = h;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         const SapState& st = a.st;
         int active = !st.done[t.item];
@@ -432,7 +487,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
             const double* sv = st.s + (size_t)t.item * (M + 1);
             for (int i = 0; i <= M; i++) L.s[i] = sv[i];
             if (jb < 0) {
-                sap_finalize_step(st, t.item, a.nchunk, M, M - 1, a.tol, t.chunk == 0, L);
+                sap_finalize_step(st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L);
                 if (L.broke) maxIt = M;
             }
             for (int i = maxIt - 1; i >= 0; i--) {

@@ -67,6 +67,7 @@ def lib():
         L.or_kcycle0.argtypes = [vp, P, P]
         L.or_solve.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double)]
+        L.or_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
         L.or_gmres_fine.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         _lib = L
     return _lib
@@ -179,6 +180,11 @@ class Oracle:
         rr = ctypes.c_double(0)
         it = lib().or_solve(self.h, _p(b), _p(x), _p(hist), hist_cap, ctypes.byref(nh), ctypes.byref(rr))
         return x, it, hist[: nh.value].copy(), rr.value
+
+    def stats(self):
+        out = (ctypes.c_long * (6 * self.L))()
+        lib().or_stats(self.h, out)
+        return np.array(out[:]).reshape(self.L, 6)
 
     def gmres_fine(self, b, m=20, restarts=50, tol=1e-10):
         b = _c(b).reshape(-1)

@@ -75,6 +75,7 @@ typedef struct {
     int n, m, restarts;
     double tol;
     cx *V, *Z, *H, *g, *sn, *cn, *eta, *r, *w, *Dx;
+    long calls, iters;  /* statistics: fgmres() calls and summed iteration counts */
 } fg_ws;
 
 typedef struct {
@@ -463,8 +464,17 @@ static void givens(fg_ws* W, int j) {
  * K-cycle uses it as its initial guess, include/amg.h:37-39).  Stopping: the
  * Arnoldi estimate |g_{j+1}| < tol ||b|| ends a cycle early; the true residual
  * is checked after each cycle.  maxIt persists across cycles (src/fgmres.cpp:14). */
+static int fgmres_body(or_ctx* c, fg_ws* W, op_fn op, const void* oarg, op_fn pre, const void* parg,
+                       const cx* b, const cx* x0, cx* x, double* hist, int hist_cap, int* nhist);
 static int fgmres(or_ctx* c, fg_ws* W, op_fn op, const void* oarg, op_fn pre, const void* parg,
                   const cx* b, const cx* x0, cx* x, double* hist, int hist_cap, int* nhist) {
+    int it = fgmres_body(c, W, op, oarg, pre, parg, b, x0, x, hist, hist_cap, nhist);
+    W->calls++;
+    W->iters += it;
+    return it;
+}
+static int fgmres_body(or_ctx* c, fg_ws* W, op_fn op, const void* oarg, op_fn pre, const void* parg,
+                       const cx* b, const cx* x0, cx* x, double* hist, int hist_cap, int* nhist) {
     const int n = W->n, m = W->m;
     memset(W->V, 0, sizeof(cx) * (size_t)(m + 1) * n);
     memset(W->Z, 0, sizeof(cx) * (size_t)m * n);
@@ -774,6 +784,18 @@ int or_solve(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, 
         *true_relres = vnorm(c->tmp1, (int)n) / vnorm((const cx*)b, (int)n);
     }
     return it;
+}
+
+/* per level l: [kc calls, kc iters, coarsest calls, coarsest iters, SAP block calls, SAP block iters] */
+void or_stats(or_ctx* c, long* out) {
+    for (int l = 0; l < c->L; l++) {
+        lvl_t* v = &c->lv[l];
+        long bc = 0, bi = 0;
+        for (int b = 0; v->blk && b < v->ncol; b++) { bc += v->blk[b].calls; bi += v->blk[b].iters; }
+        out[6 * l + 0] = v->kc.calls; out[6 * l + 1] = v->kc.iters;
+        out[6 * l + 2] = v->crs.calls; out[6 * l + 3] = v->crs.iters;
+        out[6 * l + 4] = bc; out[6 * l + 5] = bi;
+    }
 }
 
 int or_gmres_fine(or_ctx* c, const double* b, double* x, int m, int restarts, double tol) {

@@ -87,6 +87,11 @@ void or_kcycle0(or_ctx* c, const double* eta, double* psi);
 int  or_solve(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, int* nhist,
               double* true_relres);
 
+/* Solver statistics since creation, per level l:
+ * out[6l..6l+5] = K-cycle FGMRES calls, iterations; coarsest GMRES calls,
+ * iterations; SAP block GMRES calls, iterations. */
+void or_stats(or_ctx* c, long* out);
+
 /* Plain restarted FGMRES on the fine D with identity preconditioner
  * (FGMRES_fine_level, include/fgmres.h:126-148). */
 int  or_gmres_fine(or_ctx* c, const double* b, double* x, int m, int restarts, double tol);
