@@ -87,8 +87,10 @@ template <typename R> struct Engine {
     int nchunk_max = 0;
     // K-cycle(0) temporaries
     DevBuf eta1, psi1, res0;
-    // coarse workspace
-    DevBuf cws;
+    // coarse workspace; kernel arguments live in device memory (one slot per launch in flight)
+    DevBuf cws, prof, cargs;
+    int carg_slot = 0;
+    static constexpr int CARG_SLOTS = 64;
     CArgs<R> ca{};
 
     void init(Ctx* ctx, int capacity);
@@ -280,6 +282,12 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
         }
     }
     ca.ws_per_rhs = (wo + 63) & ~size_t(63);
+    cargs.alloc(sizeof(CArgs<R>) * CARG_SLOTS);
+    const char* pe = std::getenv("AMGS_PROFILE");
+    if (pe && pe[0] && pe[0] != '0') {
+        prof.alloc(sizeof(unsigned long long) * 32 * cap);
+        HIP_CHECK(hipMemsetAsync(prof.p, 0, prof.bytes, c->stream));
+    }
     cws.alloc(sizeof(cplx<R>) * ca.ws_per_rhs * cap);
     ca.ws = cws.as<cplx<R>>();
 }
@@ -302,6 +310,7 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
     const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
     a.crs_lds = dyn <= lds_cap ? 1 : 0;
+    a.prof = prof.p ? prof.as<unsigned long long>() : nullptr;
     const size_t dyn_bytes = a.crs_lds ? dyn : 0;
     a.in = in;
     a.out = out;
@@ -316,7 +325,10 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
             done = true;
         }
     }
-    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, a);
+    CArgs<R>* dargs = cargs.as<CArgs<R>>() + carg_slot;
+    carg_slot = (carg_slot + 1) % CARG_SLOTS;
+    HIP_CHECK(hipMemcpyAsync(dargs, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, (const CArgs<R>*)dargs);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -628,6 +640,18 @@ static void outer_solve(Ctx& c, int nrhs) {
     }
     hipLaunchKernelGGL(k_outer_final, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs, c.P.out_restarts, M);
     HIP_CHECK(hipGetLastError());
+    auto dump = [&](DevBuf& pb, const char* tag) {
+        if (!pb.p) return;
+        std::vector<unsigned long long> h(32);
+        c.sync();
+        HIP_CHECK(hipMemcpy(h.data(), pb.p, 32 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "[amgs profile %s, rhs 0, ms at 100 MHz]", tag);
+        for (int i = 0; i < 32; i++)
+            if (h[i]) std::fprintf(stderr, " %d:%.3f", i, (i == 7 || i == 11 || i == 15 || i == 19) ? (double)h[i] : h[i] * 1e-5);
+        std::fprintf(stderr, "\n");
+    };
+    dump(c.e64.prof, "fp64");
+    dump(c.e32.prof, "fp32");
 }
 
 // true relative residual |b - D x| / |b| in FP64

@@ -53,6 +53,7 @@ template <typename R> struct CArgs {
     cplx<R>* ws;
     size_t ws_per_rhs;
     int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
+    unsigned long long* prof;  // optional phase timers [rhs][32] (wall clock ticks)
     int mode, level, nu;
     const cplx<R>* in;
     cplx<R>* out;
@@ -72,8 +73,10 @@ struct WaveScal {
     c64 col[KC_MAXM + 1];
     int brk;
 };
+constexpr int VIN_BYTES = 16384;  // LDS staging of stencil inputs
 struct CLds {
     c64 red[CT];
+    alignas(16) char vin[VIN_BYTES];
     FgScal<KC_MAXM> kc[MAXL];
     FgScal<CRS_MAXM> crs;
     WaveScal wv[CW];
@@ -87,6 +90,26 @@ template <typename R> struct CK {
     cplx<R>* W;    // this rhs's workspace (global memory)
     CLds& S;
     cplx<R>* dyn;  // dynamic LDS region
+    unsigned long long* prof;
+};
+
+// phase timers (thread 0, device wall clock), only when a buffer is given
+struct PhaseTimer {
+    unsigned long long* p;
+    unsigned long long t;
+    __device__ __forceinline__ PhaseTimer(unsigned long long* prof) : p(prof), t(0) {
+        if (p && threadIdx.x == 0) t = wall_clock64();
+    }
+    __device__ __forceinline__ void lap(int id) {
+        if (p && threadIdx.x == 0) {
+            const unsigned long long n = wall_clock64();
+            p[id] += n - t;
+            t = n;
+        }
+    }
+    __device__ __forceinline__ void count(int id, unsigned long long v = 1) {
+        if (p && threadIdx.x == 0) p[id] += v;
+    }
 };
 
 template <typename R> __device__ __forceinline__ c64 c_dot(CLds& S, const cplx<R>* a, const cplx<R>* b, int n) {
@@ -97,35 +120,79 @@ template <typename R> __device__ __forceinline__ c64 c_dot(CLds& S, const cplx<R
     return q[0];
 }
 
-// out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))
-template <typename R> __device__ __forceinline__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out) {
+// out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu)).
+// One thread per output row; the five link blocks are walked column by column
+// with independent accumulators (many loads in flight).  The input vector is
+// staged in LDS when it is not there already (all rows of a site read it).
+// Global-memory view of a pointer: loads through it are global (not flat)
+// instructions, so they do not serialise against LDS traffic.
+template <typename T> using gptr = const __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> as_global(const T* p) { return (gptr<T>)p; }
+template <typename R> __device__ __forceinline__ cplx<R> ld(gptr<cplx<R>> p) {
+    using V2 = R __attribute__((ext_vector_type(2)));
+    const V2 v = *(const __attribute__((address_space(1))) V2*)p;
+    return {v.x, v.y};
+}
+template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { return *p; }
+
+template <typename R, typename VP>
+__device__ __forceinline__ void stencil_row(gptr<cplx<R>> links, int ksz, int msz, int D, int x, int r, VP v0, VP vr0,
+                                            VP vr1, VP vl0, VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
+    gptr<cplx<R>> L0 = links + x * msz + r;
+    gptr<cplx<R>> L1 = L0 + ksz;
+    gptr<cplx<R>> L2 = L0 + 2 * ksz;
+    gptr<cplx<R>> L3 = L0 + 3 * ksz;
+    gptr<cplx<R>> L4 = L0 + 4 * ksz;
+    cplx<R> a0 = czero<R>(), a1 = czero<R>(), a2 = czero<R>(), a3 = czero<R>(), a4 = czero<R>();
+#pragma unroll 4
+    for (int c = 0; c < D; c++) {
+        const int o = c * D;
+        a0 += ld<R>(L0 + o) * ld<R>(v0 + c);
+        a1 += ld<R>(L1 + o) * ld<R>(vr0 + c);
+        a2 += ld<R>(L2 + o) * ld<R>(vr1 + c);
+        a3 += ld<R>(L3 + o) * ld<R>(vl0 + c);
+        a4 += ld<R>(L4 + o) * ld<R>(vl1 + c);
+    }
+    acc -= a0;
+    acc -= f1 * a1 + f3 * a3;
+    acc -= f2 * a2 + f4 * a4;
+}
+
+template <typename R, typename VP>
+__device__ __forceinline__ void dop_rows(const CArgs<R>& A, int l, VP src, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, n = g.ns * D;
-    const size_t msz = (size_t)D * D, ksz = (size_t)g.ns * msz;
-    __syncthreads();
+    const int msz = D * D, ksz = g.ns * msz;
+    gptr<cplx<R>> links = as_global(L.links);
     for (int e = threadIdx.x; e < n; e += CT) {
         const int x = e / D, r = e - x * D;
-        cplx<R> acc = A.mm * in[e];
-        const cplx<R>* a0 = L.links + x * msz + r;
-        const cplx<R>* v0 = in + (size_t)x * D;
-        for (int c = 0; c < D; c++) acc -= a0[(size_t)c * D] * v0[c];
-        for (int mu = 0; mu < 2; mu++) {
-            const int xr = nb_right(x, mu, g.nx, g.nt), xl = nb_left(x, mu, g.nx, g.nt);
-            const cplx<R>* bm = L.links + (1 + mu) * ksz + x * msz + r;
-            const cplx<R>* cm = L.links + (3 + mu) * ksz + x * msz + r;
-            const cplx<R>* vr = in + (size_t)xr * D;
-            const cplx<R>* vl = in + (size_t)xl * D;
-            cplx<R> sr = czero<R>(), sl = czero<R>();
-            for (int c = 0; c < D; c++) {
-                sr += bm[(size_t)c * D] * vr[c];
-                sl += cm[(size_t)c * D] * vl[c];
-            }
-            if (wraps_right(x, mu, g.nt)) sr = -sr;
-            if (wraps_left(x, mu, g.nt)) sl = -sl;
-            acc -= sr + sl;
-        }
+        const int xr0 = nb_right(x, 0, g.nx, g.nt), xr1 = nb_right(x, 1, g.nx, g.nt);
+        const int xl0 = nb_left(x, 0, g.nx, g.nt), xl1 = nb_left(x, 1, g.nx, g.nt);
+        const R f1 = wraps_right(x, 0, g.nt) ? R(-1) : R(1);
+        const R f3 = wraps_left(x, 0, g.nt) ? R(-1) : R(1);
+        cplx<R> acc = A.mm * ld<R>(src + e);
+        stencil_row<R>(links, ksz, msz, D, x, r, src + x * D, src + xr0 * D, src + xr1 * D, src + xl0 * D, src + xl1 * D,
+                       f1, R(1), f3, R(1), acc);
         out[e] = acc;
+    }
+}
+
+// out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu)).
+// One thread per output row; the five link blocks are walked column by column
+// with independent accumulators.  The input is first copied to the LDS
+// staging buffer when it fits (every row of a site reads it).
+template <typename R>
+__device__ __forceinline__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out, cplx<R>* stage = nullptr,
+                                      int stage_cap = 0) {
+    const int n = A.lv[l].g.ns * A.lv[l].g.dof;
+    __syncthreads();
+    if (stage && n <= stage_cap) {
+        for (int e = threadIdx.x; e < n; e += CT) stage[e] = in[e];
+        __syncthreads();
+        dop_rows<R>(A, l, (const cplx<R>*)stage, out);
+    } else {
+        dop_rows<R>(A, l, as_global(in), out);
     }
     __syncthreads();
 }
@@ -191,36 +258,30 @@ template <typename R> __device__ __forceinline__ c64 w_dot(const cplx<R>* a, con
 }
 
 // block-local stencil, neighbours outside block blk read as zero
+// (multiplier 0; the read is redirected to the site itself to stay in bounds)
 template <typename R>
 __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63;
-    const size_t msz = (size_t)D * D, ksz = (size_t)g.ns * msz;
+    const int msz = D * D, ksz = g.ns * msz;
     for (int e = lane; e < nb; e += 64) {
         const int m = e / D, r = e - m * D;
         const int n = sap_site(g, blk, m);
-        cplx<R> acc = A.mm * in[e];
-        const cplx<R>* a0 = L.links + n * msz + r;
-        const cplx<R>* v0 = in + (size_t)m * D;
-        for (int c = 0; c < D; c++) acc -= a0[(size_t)c * D] * v0[c];
-        for (int mu = 0; mu < 2; mu++) {
-            const int xr = nb_right(n, mu, g.nx, g.nt), xl = nb_left(n, mu, g.nx, g.nt);
-            if (sap_block_of(g, xr) == blk) {
-                const cplx<R>* bm = L.links + (1 + mu) * ksz + n * msz + r;
-                const cplx<R>* vr = in + (size_t)sap_local_of(g, xr) * D;
-                cplx<R> sr = czero<R>();
-                for (int c = 0; c < D; c++) sr += bm[(size_t)c * D] * vr[c];
-                acc -= wraps_right(n, mu, g.nt) ? -sr : sr;
-            }
-            if (sap_block_of(g, xl) == blk) {
-                const cplx<R>* cm = L.links + (3 + mu) * ksz + n * msz + r;
-                const cplx<R>* vl = in + (size_t)sap_local_of(g, xl) * D;
-                cplx<R> sl = czero<R>();
-                for (int c = 0; c < D; c++) sl += cm[(size_t)c * D] * vl[c];
-                acc -= wraps_left(n, mu, g.nt) ? -sl : sl;
-            }
+        int nbr[4] = {nb_right(n, 0, g.nx, g.nt), nb_right(n, 1, g.nx, g.nt), nb_left(n, 0, g.nx, g.nt),
+                      nb_left(n, 1, g.nx, g.nt)};
+        R f[4];
+        const cplx<R>* vp[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const bool inb = sap_block_of(g, nbr[q]) == blk;
+            const bool wrap = (q == 0 && wraps_right(n, 0, g.nt)) || (q == 2 && wraps_left(n, 0, g.nt));
+            f[q] = inb ? (wrap ? R(-1) : R(1)) : R(0);
+            vp[q] = in + (size_t)(inb ? sap_local_of(g, nbr[q]) : m) * D;
         }
+        cplx<R> acc = A.mm * in[e];
+        stencil_row<R>(as_global(L.links), ksz, msz, D, n, r, in + m * D, vp[0], vp[1], vp[2], vp[3], f[0], f[1], f[2],
+                       f[3], acc);
         out[e] = acc;
     }
     wave_sync();
@@ -346,16 +407,19 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
     cplx<R>* r = K.W + L.sR;
     cplx<R>* Dx = K.W + L.sDx;
     const double vn = sqrt(c_dot(K.S, v, v, n).re);
-    c_dop(A, l, x, Dx);
+    c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
     for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             const int* list = colour == 0 ? L.red : L.black;
             __syncthreads();
+            PhaseTimer T(K.prof);
             for (int bi = threadIdx.x >> 6; bi < L.g.ncol; bi += CW) w_block_gmres(A, K, l, list[bi], r, x);
             __syncthreads();
-            c_dop(A, l, x, Dx);
+            T.lap(20);
+            c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
             for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+            T.lap(21);
         }
         const double err = sqrt(c_dot(K.S, r, r, n).re);
         if (err < A.sap_tol * vn) return;
@@ -395,7 +459,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
     if (!crs)
         for (int e = threadIdx.x; e < m * n; e += CT) Zb[e] = czero<R>();
     for (int e = threadIdx.x; e < n; e += CT) x[e] = b[e];
-    c_dop(A, LV, x, Dx);
+    c_dop(A, LV, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
     for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
     const double nb = sqrt(c_dot(K.S, b, b, n).re);
     double err = sqrt(c_dot(K.S, r, r, n).re);
@@ -410,7 +474,11 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
             if (!crs) {
                 if constexpr (LV < MAXL) c_kcycle<R, LV>(A, K, V(j), zj);
             }
-            c_dop(A, LV, zj, w);
+            const int pb = crs ? 0 : 8 + 4 * (LV - 1 > 2 ? 2 : LV - 1);  // timer slots
+            PhaseTimer T(K.prof);
+            T.count(crs ? 7 : pb + 3);
+            c_dop(A, LV, zj, w, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            T.lap(pb + 0);
             // one pass: <w, V_i> (i <= j) and <V_j, V_k> (k < j); nq dots,
             // CT/nq threads per dot, partial sums combined in fixed order
             const int nq = 2 * j + 1, per = CT / nq;
@@ -431,6 +499,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
                 }
                 __syncthreads();
             }
+            T.lap(pb + 1);
             if (threadIdx.x < 64) {
                 // H_ij = <w,V_i> - sum_{k<i} H_kj conj(G_ik), one wave, k over lanes
                 const int lane = threadIdx.x;
@@ -447,6 +516,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
                 for (int kk = lane; kk < j; kk += 64) S.G[j][kk] = S.dq[j + 1 + kk];
             }
             __syncthreads();
+            T.lap(pb + 2);
             c64 nq1[1] = {{0, 0}};
             for (int e = threadIdx.x; e < n; e += CT) {
                 c64 we = cvt<double>(w[e]);
@@ -457,6 +527,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
             }
             block_sum<1>(nq1, K.S.red);
             const double hn = sqrt(nq1[0].re);
+            T.lap(pb + 3);
             if (hn > 0) {
                 const R ih = R(1.0 / hn);
                 for (int e = threadIdx.x; e < n; e += CT) V(j + 1)[e] = ih * w[e];
@@ -475,6 +546,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
                 S.brk = sqrt(norm2(gn)) < tol * nb;
             }
             __syncthreads();
+            T.lap(crs ? 4 : pb + 2);
             if (S.brk) {
                 maxIt = j + 1;
                 break;
@@ -493,7 +565,7 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
             for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(Z(j)[e]);
             x[e] = cvt<R>(s);
         }
-        c_dop(A, LV, x, Dx);
+        c_dop(A, LV, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
         for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
         err = sqrt(c_dot(K.S, r, r, n).re);
         if (err < tol * nb) return;
@@ -516,7 +588,7 @@ template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CAr
         for (int ph = 0; ph < 2; ph++) {
             if (ph == 1) {
                 // coarse-grid correction
-                c_dop(A, LV, psi, Dpsi);
+                c_dop(A, LV, psi, Dpsi, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
                 for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
                 c_restrict(A, LV, r, eta1);
                 c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
@@ -528,11 +600,14 @@ template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CAr
     }
 }
 
-template <typename R> __global__ void __launch_bounds__(CT) k_coarse(CArgs<R> A) {
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R>* __restrict__ Ap) {
+    const CArgs<R>& A = *Ap;
     __shared__ CLds S;
     extern __shared__ char dyn_lds[];
     const int rhs = blockIdx.x;
-    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds)};
+    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
+            A.prof ? A.prof + (size_t)rhs * 32 : nullptr};
+    PhaseTimer Ttot(K.prof);
     const cplx<R>* in = A.in + (size_t)rhs * A.in_stride;
     cplx<R>* out = A.out + (size_t)rhs * A.out_stride;
     const int l = A.level;
@@ -545,6 +620,7 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(CArgs<R> A)
         case CM_RESTRICT: c_restrict(A, l, in, out); break;
         case CM_PROLONG: c_prolong(A, l, in, out, false); break;
     }
+    Ttot.lap(31);
 }
 
 }  // namespace amgs

@@ -136,7 +136,7 @@ struct SapState {
     int *maxIt, *jbreak, *done;
     double *normb, *s;          // s: [item][M+1]
     c64 *Hraw, *Hrot;           // [item][(M+1)*M]  (row i, column j at i*M+j)
-    c64 *G;                     // [item][(M+1)*(M+1)]
+    c64 *G;                     // [item][(M+1)*(M+1)]  G[i][k] = <V_i, V_k>, k < i
     c64 *gpre, *gpost, *cn, *sn;
     c64 *qa;                    // [item][nchunk][2M]   <w,Vt_i>, <Vt_j,Vt_k>
     double *qb;                 // [item][nchunk]       |w|^2
@@ -156,26 +156,6 @@ template <typename R> struct SapArgs {
     const int* skip;       // per-rhs: SAP already converged
     SapState st;
 };
-
-// Block-wide reduction of per-chunk partial sums: out[q] = sum_c base[c*stride + q]
-// for q < nq, chunks summed in a fixed order (16 threads per quantity, then
-// the 16 partials in order).  All threads must call it.
-template <typename T>
-__device__ __forceinline__ void reduce_chunks(const T* base, size_t stride, int nq, int nchunk, T* tmp, T* out) {
-    const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
-    if (q < nq) {
-        T s{};
-        for (int c = sub; c < nchunk; c += 16) s += base[(size_t)c * stride + q];
-        tmp[threadIdx.x] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < nq) {
-        T s = tmp[threadIdx.x * 16];
-        for (int i = 1; i < 16; i++) s += tmp[threadIdx.x * 16 + i];
-        out[threadIdx.x] = s;
-    }
-    __syncthreads();
-}
 
 struct ItemIdx {
     int item, chunk, rhs, blk, m_lo, m_hi;
@@ -197,47 +177,95 @@ template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>&
     return a.Vt + ((size_t)j * a.nrhs + rhs) * a.g.ns * 2;
 }
 
-// Per-item scalar state held in LDS during a kernel.
+// Per-item scalar state held in LDS during a kernel (matrices with row
+// stride M, as in global memory).
 struct ItemLds {
-    int active, maxIt, broke;
+    int active, maxIt, broke, done, jbreak;
+    double normb;
     double s[SAP_MAXM + 1];
-    c64 H[SAP_MAXM + 1];   // working column
-    c64 R[SAP_MAXM][SAP_MAXM];  // rotated (upper-triangular) Hessenberg
-    c64 g[SAP_MAXM + 1];
+    c64 Hraw[(SAP_MAXM + 1) * SAP_MAXM];
+    c64 R[(SAP_MAXM + 1) * SAP_MAXM];
+    c64 G[(SAP_MAXM + 1) * (SAP_MAXM + 1)];
+    c64 gpre[SAP_MAXM + 1], gpost[SAP_MAXM + 1];
+    c64 cn[SAP_MAXM], sn[SAP_MAXM];
+    c64 Hcol[SAP_MAXM + 1];
     c64 eta[SAP_MAXM];
-    c64 red[8 * 2 * SAP_MAXM];
-    c64 tmp[FT];           // chunk-reduction scratch
-    c64 dq[2 * SAP_MAXM];  // reduced partial sums
+    c64 dq[2 * SAP_MAXM];
     double dr[2];
+    c64 tmp[FT];  // reduction scratch
 };
 
-// Finalise Arnoldi step j (src/fgmres.cpp:35-49): H_{j+1,j} from the |w|^2
-// partials, Givens rotation of column j, residual estimate, break test.
-// Executed by thread 0; chunk 0 stores.  Returns through L.
-__device__ void sap_finalize_step(const SapState& st, int item, double hn2, int M, int j, double tol,
-                                  bool store, ItemLds& L) {
+// All threads: copy the item's state into LDS (one parallel pass of loads).
+__device__ __forceinline__ void load_item(ItemLds& L, const SapState& st, int item, int M) {
+    const int t = threadIdx.x;
+    const int nH = (M + 1) * M, nG = (M + 1) * (M + 1);
+    for (int i = t; i < nH; i += FT) {
+        L.Hraw[i] = st.Hraw[(size_t)item * nH + i];
+        L.R[i] = st.Hrot[(size_t)item * nH + i];
+    }
+    for (int i = t; i < nG; i += FT) L.G[i] = st.G[(size_t)item * nG + i];
+    if (t < M + 1) {
+        L.s[t] = st.s[(size_t)item * (M + 1) + t];
+        L.gpre[t] = st.gpre[(size_t)item * (M + 1) + t];
+        L.gpost[t] = st.gpost[(size_t)item * (M + 1) + t];
+    }
+    if (t >= 64 && t < 64 + M) {
+        L.cn[t - 64] = st.cn[(size_t)item * M + t - 64];
+        L.sn[t - 64] = st.sn[(size_t)item * M + t - 64];
+    }
+    if (t == 128) {
+        L.done = st.done[item];
+        L.jbreak = st.jbreak[item];
+        L.maxIt = st.maxIt[item];
+        L.normb = st.normb[item];
+    }
+}
+
+// Sum of per-chunk partials base[c*stride + q] over chunks for q < nq, in a
+// fixed order (16 threads per quantity, then the 16 partials in order).
+template <typename T>
+__device__ __forceinline__ void reduce_chunks(const T* base, int stride, int nq, int nchunk, c64* tmp, T* out) {
+    const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    T sacc{};
+    if (q < nq)
+        for (int c = sub; c < nchunk; c += 16) sacc += base[(size_t)c * stride + q];
+    reinterpret_cast<T*>(tmp)[threadIdx.x] = sacc;
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        T r = reinterpret_cast<T*>(tmp)[threadIdx.x * 16];
+        for (int i = 1; i < 16; i++) r += reinterpret_cast<T*>(tmp)[threadIdx.x * 16 + i];
+        out[threadIdx.x] = r;
+    }
+    __syncthreads();
+}
+
+// Finalise Arnoldi step j (src/fgmres.cpp:35-49): H_{j+1,j} = |w|, Givens
+// rotation of column j, residual estimate, break test.  Thread 0 on LDS
+// state; chunk 0 also stores the results.
+__device__ void sap_finalize_step(const SapState& st, int item, double hn2, int M, int j, double tol, bool store,
+                                  ItemLds& L) {
     const double hn = sqrt(hn2);
-    const c64* Hr = st.Hraw + (size_t)item * (M + 1) * M;
-    c64 col[SAP_MAXM + 1];
-    for (int i = 0; i <= j; i++) col[i] = Hr[i * M + j];
+    c64* col = L.Hcol;
+    for (int i = 0; i <= j; i++) col[i] = L.Hraw[i * M + j];
     col[j + 1] = c64{hn, 0.0};
-    const c64* cn = st.cn + (size_t)item * M;
-    const c64* sn = st.sn + (size_t)item * M;
     for (int i = 0; i < j; i++) {
-        c64 t = conj(cn[i]) * col[i] + conj(sn[i]) * col[i + 1];
-        col[i + 1] = -(sn[i] * col[i]) + cn[i] * col[i + 1];
-        col[i] = t;
+        const c64 a = col[i], b = col[i + 1];
+        col[i + 1] = -(L.sn[i] * a) + L.cn[i] * b;
+        col[i] = conj(L.cn[i]) * a + conj(L.sn[i]) * b;
     }
     const double den = sqrt(norm2(col[j]) + norm2(col[j + 1]));
     const c64 snj = {col[j + 1].re / den, col[j + 1].im / den};
     const c64 cnj = {col[j].re / den, col[j].im / den};
     col[j] = conj(cnj) * col[j] + conj(snj) * col[j + 1];
-    const c64 gj = st.gpre[(size_t)item * (M + 1) + j];
+    const c64 gj = L.gpre[j];
     const c64 gn = -(snj * gj);
     const c64 gp = conj(cnj) * gj;
-    const bool broke = sqrt(norm2(gn)) < tol * st.normb[item];
-    for (int i = 0; i <= j; i++) L.R[i][j] = col[i];
-    L.g[j] = gp;
+    const bool broke = sqrt(norm2(gn)) < tol * L.normb;
+    for (int i = 0; i <= j; i++) L.R[i * M + j] = col[i];
+    L.cn[j] = cnj;
+    L.sn[j] = snj;
+    L.gpost[j] = gp;
+    L.gpre[j + 1] = gn;
     L.s[j + 1] = hn > 0 ? 1.0 / hn : 0.0;
     L.broke = broke;
     if (store) {
@@ -253,6 +281,13 @@ __device__ void sap_finalize_step(const SapState& st, int item, double hn2, int 
             st.maxIt[item] = j + 1;
         }
     }
+}
+
+// Wave-reduce v for quantity q and park it in tmp[wave][q] (no private arrays
+// with run-time indices).
+__device__ __forceinline__ void park(c64 v, int q, c64* tmp, int nqmax) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) tmp[(threadIdx.x >> 6) * nqmax + q] = v;
 }
 
 // init: x_B = b_B (x0 = b, src/sap.cpp:61), Vt_0 = b_B - D_B b_B; partials |r|^2, |b|^2
@@ -296,45 +331,21 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
+    load_item(L, a.st, t.item, M);
     if (j == 0) {
-        // |r|^2, |b|^2 partials of the init / residual kernel
-        const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
-        double s = 0;
-        if (q < 2)
-            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qr[((size_t)t.item * a.nchunk + c) * 2 + q];
-        if (q < 2) L.tmp[threadIdx.x].re = s;
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            double h = 0;
-            for (int i = 0; i < 16; i++) h += L.tmp[threadIdx.x * 16 + i].re;
-            L.dq[threadIdx.x].re = h;
-        }
-        __syncthreads();
-    }
-    if (j > 0) {
-        // |w|^2 partials of step j-1
-        const int sub = threadIdx.x;
-        double s = 0;
-        if (sub < 16)
-            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qb[(size_t)t.item * a.nchunk + c];
-        if (sub < 16) L.tmp[sub].re = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double h = 0;
-            for (int i = 0; i < 16; i++) h += L.tmp[i].re;
-            L.dr[0] = h;
-        }
-        __syncthreads();
+        reduce_chunks<double>(a.st.qr + (size_t)t.item * a.nchunk * 2, 2, 2, a.nchunk, L.tmp, L.dr);
+    } else {
+        reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
     }
     if (threadIdx.x == 0) {
         const SapState& st = a.st;
         const bool store = t.chunk == 0;
-        int active = !st.done[t.item];
+        int active = !L.done;
         if (active && j == 0) {
             // cycle start (src/fgmres.cpp:8-17, 61-65)
-            const double err = sqrt(L.dq[0].re);
-            double nb = k == 0 ? sqrt(L.dq[1].re) : st.normb[t.item];
-            bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
+            const double err = sqrt(L.dr[0]);
+            const double nb = k == 0 ? sqrt(L.dr[1]) : L.normb;
+            const bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
             if (done) {
                 active = 0;
                 if (store) st.done[t.item] = 1;
@@ -348,7 +359,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
                 }
             }
         } else if (active) {
-            if (st.jbreak[t.item] >= 0) active = 0;
+            if (L.jbreak >= 0) active = 0;
             else {
                 sap_finalize_step(st, t.item, L.dr[0], M, j - 1, a.tol, store, L);
                 if (L.broke) active = 0;
@@ -360,70 +371,70 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
     if (!L.active) return;
     const double sj = L.s[j];
     const int m = t.m_lo + threadIdx.x;
-    const size_t off = (size_t)t.rhs * a.g.ns * 2;
     const cplx<R>* vj = slot(a, j, t.rhs);
     cplx<R>* w = slot(a, j + 1, t.rhs);
-    c64 q[2 * SAP_MAXM];
-    for (int i = 0; i < 2 * SAP_MAXM; i++) q[i] = c64{0, 0};
-    if (m < t.m_hi) {
-        const int n = sap_site(a.g, t.blk, m);
+    const int nq = 2 * j + 1;
+    c64 w0 = {0, 0}, w1 = {0, 0}, a0 = {0, 0}, a1 = {0, 0};
+    int n = 0;
+    const bool live = m < t.m_hi;
+    if (live) {
+        n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
         wilson_site<R, false>(a.U, vj, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
         o0 = R(sj) * o0;
         o1 = R(sj) * o1;
         w[2 * n] = o0;
         w[2 * n + 1] = o1;
-        const c64 w0 = cvt<double>(o0), w1 = cvt<double>(o1);
-        const c64 vj0 = cvt<double>(vj[2 * n]), vj1 = cvt<double>(vj[2 * n + 1]);
-        for (int i = 0; i <= j; i++) {
-            const cplx<R>* vi = slot(a, i, t.rhs);
-            const c64 a0 = cvt<double>(vi[2 * n]), a1 = cvt<double>(vi[2 * n + 1]);
-            q[i] = mulc(w0, a0) + mulc(w1, a1);
-            if (i < j) q[j + 1 + i] = mulc(vj0, a0) + mulc(vj1, a1);
-        }
+        w0 = cvt<double>(o0);
+        w1 = cvt<double>(o1);
+        a0 = cvt<double>(vj[2 * n]);
+        a1 = cvt<double>(vj[2 * n + 1]);
     }
-    (void)off;
-    block_sum<2 * SAP_MAXM>(q, L.red, 2 * j + 1);
-    if (threadIdx.x < 2 * j + 1)
-        a.st.qa[((size_t)t.item * a.nchunk + t.chunk) * 2 * M + threadIdx.x] = q[threadIdx.x];
+    for (int i = 0; i <= j; i++) {
+        c64 pw = {0, 0}, pv = {0, 0};
+        if (live) {
+            const cplx<R>* vi = slot(a, i, t.rhs);
+            const c64 b0 = cvt<double>(vi[2 * n]), b1 = cvt<double>(vi[2 * n + 1]);
+            pw = mulc(w0, b0) + mulc(w1, b1);
+            pv = mulc(a0, b0) + mulc(a1, b1);
+        }
+        park(pw, i, L.tmp, 2 * SAP_MAXM);
+        if (i < j) park(pv, j + 1 + i, L.tmp, 2 * SAP_MAXM);
+    }
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        c64 r = L.tmp[threadIdx.x];
+        for (int wv = 1; wv < FT / 64; wv++) r += L.tmp[wv * 2 * SAP_MAXM + threadIdx.x];
+        a.st.qa[((size_t)t.item * a.nchunk + t.chunk) * 2 * M + threadIdx.x] = r;
+    }
 }
 
-// Arnoldi step j, part B: prologue builds Hessenberg column j (MGS form),
+// Arnoldi step j, part B: prologue builds Hessenberg column j (one-pass MGS),
 // body w -= sum_i H_ij V_i (in place in Vt_{j+1}); partial |w|^2.
 template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
-    reduce_chunks(a.st.qa + (size_t)t.item * a.nchunk * 2 * M, (size_t)2 * M, 2 * j + 1, a.nchunk, L.tmp, L.dq);
+    load_item(L, a.st, t.item, M);
+    reduce_chunks<c64>(a.st.qa + (size_t)t.item * a.nchunk * 2 * M, 2 * M, 2 * j + 1, a.nchunk, L.tmp, L.dq);
     if (threadIdx.x == 0) {
         const SapState& st = a.st;
-        int active = !st.done[t.item] && st.jbreak[t.item] < 0;
+        const int active = !L.done && L.jbreak < 0;
         if (active) {
             const c64* d = L.dq;
-            const double* sv = st.s + (size_t)t.item * (M + 1);
-            double s[SAP_MAXM + 1];
-            for (int i = 0; i <= j; i++) s[i] = sv[i];
-            // Gram row j: G_jk = s_j s_k <Vt_j, Vt_k>,  k < j
-            c64* Gm = st.G + (size_t)t.item * (M + 1) * (M + 1);
-            c64 gj[SAP_MAXM];
-            for (int kk = 0; kk < j; kk++) gj[kk] = (s[j] * s[kk]) * d[j + 1 + kk];
-            c64 H[SAP_MAXM + 1];
+            // Gram row j: G_jk = s_j s_k <Vt_j, Vt_k>, k < j
+            for (int kk = 0; kk < j; kk++) L.G[j * (M + 1) + kk] = (L.s[j] * L.s[kk]) * d[j + 1 + kk];
             for (int i = 0; i <= j; i++) {
-                c64 h = s[i] * d[i];
-                for (int kk = 0; kk < i; kk++) {
-                    // G_ki = <V_k, V_i> = conj(G_ik); rows < j come from memory
-                    c64 Gik = (i == j) ? gj[kk] : Gm[i * (M + 1) + kk];
-                    h -= H[kk] * conj(Gik);
-                }
-                H[i] = h;
-                L.H[i] = h;
-                L.s[i] = s[i];
+                c64 h = L.s[i] * d[i];
+                for (int kk = 0; kk < i; kk++) h -= L.Hcol[kk] * conj(L.G[i * (M + 1) + kk]);
+                L.Hcol[i] = h;
             }
             if (t.chunk == 0) {
                 c64* Hr = st.Hraw + (size_t)t.item * (M + 1) * M;
-                for (int i = 0; i <= j; i++) Hr[i * M + j] = H[i];
-                for (int kk = 0; kk < j; kk++) Gm[j * (M + 1) + kk] = gj[kk];
+                for (int i = 0; i <= j; i++) Hr[i * M + j] = L.Hcol[i];
+                c64* Gm = st.G + (size_t)t.item * (M + 1) * (M + 1);
+                for (int kk = 0; kk < j; kk++) Gm[j * (M + 1) + kk] = L.G[j * (M + 1) + kk];
             }
         }
         L.active = active;
@@ -438,7 +449,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
         c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
         for (int i = 0; i <= j; i++) {
             const cplx<R>* vi = slot(a, i, t.rhs);
-            const c64 c = L.s[i] * L.H[i];
+            const c64 c = L.s[i] * L.Hcol[i];
             w0 -= c * cvt<double>(vi[2 * n]);
             w1 -= c * cvt<double>(vi[2 * n + 1]);
         }
@@ -448,7 +459,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
         nn = double(norm2(r0)) + double(norm2(r1));
     }
     c64 q[1] = {{nn, 0.0}};
-    block_sum<1>(q, L.red);
+    block_sum<1>(q, L.tmp);
     if (threadIdx.x == 0) a.st.qb[(size_t)t.item * a.nchunk + t.chunk] = q[0].re;
 }
 
@@ -459,41 +470,20 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
-    {
-        const int sub = threadIdx.x;
-        double s = 0;
-        if (sub < 16)
-            for (int c = sub; c < a.nchunk; c += 16) s += a.st.qb[(size_t)t.item * a.nchunk + c];
-        if (sub < 16) L.tmp[sub].re = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double h = 0;
-            for (int i = 0; i < 16; i++) h += L.tmp[i].re;
-            L.dr[0] = h;
-        }
-        __syncthreads();
-    }
+    load_item(L, a.st, t.item, M);
+    reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
     if (threadIdx.x == 0) {
-        const SapState& st = a.st;
-        int active = !st.done[t.item];
+        const int active = !L.done;
         if (active) {
-            const int jb = st.jbreak[t.item];
-            int maxIt = st.maxIt[t.item];
-            const c64* Ro = st.Hrot + (size_t)t.item * (M + 1) * M;
-            const int ncols = jb >= 0 ? jb + 1 : M;
-            for (int jj = 0; jj < ncols; jj++)
-                for (int i = 0; i <= jj; i++) L.R[i][jj] = Ro[i * M + jj];
-            for (int jj = 0; jj < ncols; jj++) L.g[jj] = st.gpost[(size_t)t.item * (M + 1) + jj];
-            const double* sv = st.s + (size_t)t.item * (M + 1);
-            for (int i = 0; i <= M; i++) L.s[i] = sv[i];
-            if (jb < 0) {
-                sap_finalize_step(st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L);
+            int maxIt = L.maxIt;
+            if (L.jbreak < 0) {
+                sap_finalize_step(a.st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L);
                 if (L.broke) maxIt = M;
             }
             for (int i = maxIt - 1; i >= 0; i--) {
-                c64 e = L.g[i];
-                for (int q = i + 1; q < maxIt; q++) e -= L.R[i][q] * L.eta[q];
-                L.eta[i] = cdiv(e, L.R[i][i]);
+                c64 e = L.gpost[i];
+                for (int q = i + 1; q < maxIt; q++) e -= L.R[i * M + q] * L.eta[q];
+                L.eta[i] = cdiv(e, L.R[i * M + i]);
             }
             L.maxIt = maxIt;
         }
@@ -608,8 +598,19 @@ __global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* 
             }
         }
     }
-    block_sum<2 * NV>(acc, red, 2 * nv);
-    if (threadIdx.x < 2 * nv) out[(size_t)rhs * nsc * dofc + (size_t)X * dofc + threadIdx.x] = acc[threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < 2 * NV; q++) {
+        if (q < 2 * nv) {
+            const cplx<R> v = wave_sum(acc[q]);
+            if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * 2 * NV + q] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * nv) {
+        cplx<R> r = red[threadIdx.x];
+        for (int wv = 1; wv < FT / 64; wv++) r += red[wv * 2 * NV + threadIdx.x];
+        out[(size_t)rhs * nsc * dofc + (size_t)X * dofc + threadIdx.x] = r;
+    }
 }
 
 // out[n][s] (+)= sum_cc w_cc[n][s] vc[X(n)][2cc+s]

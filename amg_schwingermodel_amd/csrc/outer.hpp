@@ -100,50 +100,73 @@ __global__ void __launch_bounds__(FT) k_outer_scale(int ns, const c64* __restric
 __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __restrict__ U, const c64* __restrict__ Zj,
                                                 const c64* __restrict__ V, c64* __restrict__ w, OuterState st, int j,
                                                 size_t vstride) {
-    __shared__ c64 red[4 * (2 * OUT_MAXM + 1)];
+    constexpr int NQ = 2 * OUT_MAXM + 1;
+    __shared__ c64 red[(FT / 64) * NQ];
     const int rhs = blockIdx.y;
     if (!st.active[rhs]) return;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int n = tile * FT + threadIdx.x;
     const size_t off = (size_t)rhs * g.ns * 2;
     const int nq = 2 * j + 1;
-    c64 q[2 * OUT_MAXM + 1];
-    for (int i = 0; i < nq; i++) q[i] = c64{0, 0};
-    if (n < g.ns) {
-        c64 o0, o1;
+    const bool live = n < g.ns;
+    c64 o0 = {0, 0}, o1 = {0, 0}, a0 = {0, 0}, a1 = {0, 0};
+    if (live) {
         wilson_site<double, false>(U, Zj + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
         w[off + 2 * n] = o0;
         w[off + 2 * n + 1] = o1;
         const c64* vj = V + (size_t)j * vstride + off;
-        const c64 a0 = vj[2 * n], a1 = vj[2 * n + 1];
-        for (int i = 0; i <= j; i++) {
+        a0 = vj[2 * n];
+        a1 = vj[2 * n + 1];
+    }
+    for (int i = 0; i <= j; i++) {
+        c64 pw = {0, 0}, pv = {0, 0};
+        if (live) {
             const c64* vi = V + (size_t)i * vstride + off;
             const c64 b0 = vi[2 * n], b1 = vi[2 * n + 1];
-            q[i] = mulc(o0, b0) + mulc(o1, b1);
-            if (i < j) q[j + 1 + i] = mulc(a0, b0) + mulc(a1, b1);
+            pw = mulc(o0, b0) + mulc(o1, b1);
+            pv = mulc(a0, b0) + mulc(a1, b1);
         }
+        park(pw, i, red, NQ);
+        if (i < j) park(pv, j + 1 + i, red, NQ);
     }
-    block_sum<2 * OUT_MAXM + 1>(q, red, nq);
-    if (threadIdx.x < nq) st.qa[((size_t)rhs * st.ntile + tile) * (2 * st.M + 1) + threadIdx.x] = q[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        c64 r = red[threadIdx.x];
+        for (int wv = 1; wv < FT / 64; wv++) r += red[wv * NQ + threadIdx.x];
+        st.qa[((size_t)rhs * st.ntile + tile) * (2 * st.M + 1) + threadIdx.x] = r;
+    }
+}
+
+// Sum over tiles of quantity q of base[t*stride + q], all threads of the
+// workgroup in a fixed order; the total lands in out.
+__device__ __forceinline__ void tile_sum(const c64* base, int stride, int ntile, int q, c64* red, c64* out) {
+    c64 v = {0, 0};
+    for (int t = threadIdx.x; t < ntile; t += FT) v += base[(size_t)t * stride + q];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c64 r = red[0];
+        for (int wv = 1; wv < FT / 64; wv++) r += red[wv];
+        *out = r;
+    }
+    __syncthreads();
 }
 
 // Hessenberg column j by one-pass MGS: H_ij = <w,V_i> - sum_{k<i} H_kj <V_k,V_i>.
 // Stores the unrotated column in R[.][j] temporarily (rotated in k_outer_fin).
 __global__ void __launch_bounds__(FT) k_outer_hcol(OuterState st, int j) {
     __shared__ c64 d[2 * OUT_MAXM + 1];
+    __shared__ c64 col[OUT_MAXM + 1];
+    __shared__ c64 red[FT / 64];
     const int rhs = blockIdx.x;
     if (!st.active[rhs]) return;
     const int nq = 2 * j + 1, M = st.M;
-    for (int q = threadIdx.x; q < nq; q += FT) {
-        c64 s = {0, 0};
-        for (int t = 0; t < st.ntile; t++) s += st.qa[((size_t)rhs * st.ntile + t) * (2 * M + 1) + q];
-        d[q] = s;
-    }
-    __syncthreads();
+    for (int q = 0; q < nq; q++)
+        tile_sum(st.qa + (size_t)rhs * st.ntile * (2 * M + 1), 2 * M + 1, st.ntile, q, red, &d[q]);
     if (threadIdx.x != 0) return;
     c64* G = st.G + (size_t)rhs * (M + 1) * (M + 1);
     c64* Rm = st.R + (size_t)rhs * M * M;
-    c64 col[OUT_MAXM + 1];
     for (int i = 0; i <= j; i++) {
         c64 h = d[i];
         for (int kk = 0; kk < i; kk++) {
@@ -200,7 +223,7 @@ __global__ void __launch_bounds__(FT) k_outer_fin(OuterState st, int j, double t
     c64* cn = st.cn + (size_t)rhs * M;
     c64* sn = st.sn + (size_t)rhs * M;
     c64* g = st.g + (size_t)rhs * (M + 1);
-    c64 col[OUT_MAXM + 1];
+    __shared__ c64 col[OUT_MAXM + 1];
     for (int i = 0; i <= j; i++) col[i] = Rm[i * M + j];
     col[j + 1] = c64{hn, 0.0};
     for (int i = 0; i < j; i++) {
