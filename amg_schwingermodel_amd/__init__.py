@@ -89,6 +89,8 @@ SIGNATURES = {
     "amgs_write_residuals": ([ctypes.c_char_p, _P, ctypes.c_int], ctypes.c_int),
     "amgs_time_kernel": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "amgs_last_timings": ([_V, _P], ctypes.c_int),
+    "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
 }
 
 
@@ -263,6 +265,20 @@ class Solver:
         ms = ctypes.c_double(0)
         self._chk(lib().amgs_time_kernel(self.h, kernel, nrhs, reps, ctypes.byref(ms)), "amgs_time_kernel")
         return ms.value
+
+    def kernel_timing(self, kclass, enable=True):
+        """Record every launch of a kernel class (0: SAP Arnoldi step A,
+        1: step B) in later solves with HIP events on the solver's stream."""
+        self._chk(lib().amgs_kernel_timing(self.h, kclass, int(enable)), "amgs_kernel_timing")
+
+    def kernel_stats(self, kclass):
+        """(launches, total device ms, algorithmic bytes) recorded so far."""
+        n = ctypes.c_longlong(0)
+        ms = ctypes.c_double(0)
+        b = ctypes.c_double(0)
+        self._chk(lib().amgs_kernel_stats(self.h, kclass, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
+                  "amgs_kernel_stats")
+        return n.value, ms.value, b.value
 
     def last_timings(self):
         a = np.zeros(3)

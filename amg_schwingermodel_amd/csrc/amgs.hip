@@ -72,6 +72,68 @@ static inline unsigned cdiv_u(size_t a, size_t b) { return (unsigned)((a + b - 1
 struct Ctx;
 
 // ---------------------------------------------------------------------------
+// Per-launch timing of one kernel class inside solves: a HIP event pair on the
+// library stream around every launch, and a device counter of the items that
+// did work in that launch (for the algorithmic byte count).
+// ---------------------------------------------------------------------------
+struct KTimer {
+    static constexpr int CAP = 8192;  // launches recorded between two resolves
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> fixed, per_item;  // bytes = fixed + per_item * active items
+    DevBuf actlog;
+    long long launches = 0;
+    double ms = 0, bytes = 0;
+
+    ~KTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void reset() {
+        launches = 0;
+        ms = bytes = 0;
+        fixed.clear();
+        per_item.clear();
+    }
+    // returns the counter for this launch, or nullptr when not recording
+    int* begin(hipStream_t s, double fx, double pi) {
+        if (!on || (int)fixed.size() >= CAP) return nullptr;
+        if (!actlog.p) {
+            actlog.alloc(sizeof(int) * CAP);
+            HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+        }
+        const size_t i = fixed.size();
+        while (ev.size() < 2 * i + 2) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        HIP_CHECK(hipEventRecord(ev[2 * i], s));
+        fixed.push_back(fx);
+        per_item.push_back(pi);
+        return actlog.as<int>() + i;
+    }
+    void end(hipStream_t s) { HIP_CHECK(hipEventRecord(ev[2 * (fixed.size() - 1) + 1], s)); }
+    // fold the pending records into the totals (synchronises the stream)
+    void resolve(hipStream_t s) {
+        if (fixed.empty()) return;
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<int> act(fixed.size());
+        HIP_CHECK(hipMemcpy(act.data(), actlog.p, sizeof(int) * act.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < fixed.size(); i++) {
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            ms += t;
+            launches++;
+            if (act[i] > 0) bytes += fixed[i] + per_item[i] * act[i];
+        }
+        HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+        fixed.clear();
+        per_item.clear();
+    }
+};
+enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_CLASSES = 2 };
+
+// ---------------------------------------------------------------------------
 // per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments
 // ---------------------------------------------------------------------------
 template <typename R> struct Engine {
@@ -122,6 +184,7 @@ struct Ctx {
     DevBuf io_a, io_b;
     double last_ms[3] = {0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    KTimer kt[KT_CLASSES];
 
     size_t nvec(int l) const { return (size_t)geo[l].ns * geo[l].dof; }
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
@@ -346,6 +409,30 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
     HIP_CHECK(hipGetLastError());
 }
 
+// Chunks per SAP item (block x rhs): enough workgroups to fill the chip
+// (about AMGS_SAP_WGS, default 2048), each chunk a multiple of FT sites.
+static int sap_chunks(const Geo& g, int nrhs) {
+    static const int target = [] {
+        const char* e = std::getenv("AMGS_SAP_WGS");
+        const int v = e ? std::atoi(e) : 0;
+        return v > 0 ? v : 2048;
+    }();
+    const int items = g.ncol * nrhs, maxc = (int)cdiv_u(g.sspb, FT);
+    return std::max(1, std::min(maxc, (int)cdiv_u(target, items)));
+}
+
+template <typename R>
+static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act) {
+    switch (j) {
+#define AMGS_SAP_A(J) \
+    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J>), grid, dim3(FT), 0, s, a, k, act); break;
+        AMGS_SAP_A(0) AMGS_SAP_A(1) AMGS_SAP_A(2) AMGS_SAP_A(3)
+        AMGS_SAP_A(4) AMGS_SAP_A(5) AMGS_SAP_A(6) AMGS_SAP_A(7)
+#undef AMGS_SAP_A
+        default: throw ApiError(AMGS_ERR_ARG, "SAP restart length exceeds 8");
+    }
+}
+
 // SAP_C::SAP at level 0 (src/sap.cpp:64-174) for nrhs right-hand sides.
 template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu) {
     if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
@@ -358,13 +445,14 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                        (double*)nullptr, skip.as<int>());
-    const int nchunk = (int)cdiv_u(g.sspb, FT);
+    const int nchunk = sap_chunks(g, nrhs), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
     a.mm = mm;
     a.U = U;
     a.ncol = g.ncol;
-    a.nchunk = nchunk;
+    a.nchunk = (int)cdiv_u(g.sspb, cs);
+    a.cs = cs;
     a.nrhs = nrhs;
     a.m = c->P.sap_m;
     a.tol = c->P.sap_gmres_tol;
@@ -373,25 +461,33 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     a.xl = xl.as<cplx<R>>();
     a.skip = skip.as<int>();
     a.st = st;
-    const dim3 grid(g.ncol * nchunk, nrhs);
+    const dim3 grid(g.ncol * a.nchunk, nrhs);
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
             hipLaunchKernelGGL(k_sap_init<R>, grid, dim3(FT), 0, s, a);
             for (int k = 0; k < c->P.sap_restarts; k++) {
                 for (int j = 0; j < a.m; j++) {
-                    hipLaunchKernelGGL(k_sap_arnoldi_a<R>, grid, dim3(FT), 0, s, a, j, k);
-                    hipLaunchKernelGGL(k_sap_arnoldi_b<R>, grid, dim3(FT), 0, s, a, j);
+                    // algorithmic bytes: links of the colour's sites once; per active
+                    // item Vt_j read, w written, Vt_i (i < j) read for the dots
+                    const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
+                    int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * 2 * S * (2 + j));
+                    sap_arnoldi_a(grid, s, a, j, k, act);
+                    if (act) c->kt[KT_SAP_A].end(s);
+                    // w and Vt_i (i <= j) read, w written
+                    act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * 2 * S * (3 + j));
+                    hipLaunchKernelGGL(k_sap_arnoldi_b<R>, grid, dim3(FT), 0, s, a, j, act);
+                    if (act) c->kt[KT_SAP_B].end(s);
                 }
-                hipLaunchKernelGGL(k_sap_update<R>, grid, dim3(FT), 0, s, a);
-                if (k + 1 < c->P.sap_restarts) hipLaunchKernelGGL(k_sap_resid<R>, grid, dim3(FT), 0, s, a);
+                const bool last = k + 1 == c->P.sap_restarts;
+                hipLaunchKernelGGL(k_sap_update<R>, grid, dim3(FT), 0, s, a, last ? x : (cplx<R>*)nullptr);
+                if (!last) hipLaunchKernelGGL(k_sap_resid<R>, grid, dim3(FT), 0, s, a);
             }
-            hipLaunchKernelGGL(k_sap_accum<R>, grid, dim3(FT), 0, s, a, x);
             hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                                colour == 1 ? part_r.as<double>() : (double*)nullptr, skip.as<int>());
         }
-        hipLaunchKernelGGL(k_sap_check, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, (const double*)part_r.as<double>(),
-                           (const double*)part_v.as<double>(), (int)ntile, nrhs, c->P.sap_tol, skip.as<int>());
+        hipLaunchKernelGGL(k_sap_check, dim3(nrhs), dim3(FT), 0, s, (const double*)part_r.as<double>(),
+                           (const double*)part_v.as<double>(), (int)ntile, c->P.sap_tol, skip.as<int>());
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -615,6 +711,7 @@ static void outer_solve(Ctx& c, int nrhs) {
                                    st, j + 1);
             HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
             c.sync();
+            for (KTimer& kt : c.kt) kt.resolve(s);
             step++;
             if (verbose) {
                 int nh = 0;
@@ -1061,6 +1158,29 @@ extern "C" int amgs_time_kernel(amgs_ctx* ctx, int kernel, int nrhs, int reps, d
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     *avg_ms = ms / reps;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_kernel_timing(amgs_ctx* ctx, int kclass, int enable) {
+    if (!ctx || kclass < 0 || kclass >= KT_CLASSES) return AMGS_ERR_ARG;
+    API_BEGIN
+    KTimer& k = ctx->c.kt[kclass];
+    k.resolve(ctx->c.stream);
+    k.reset();
+    k.on = enable != 0;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_kernel_stats(amgs_ctx* ctx, int kclass, long long* launches, double* total_ms, double* bytes) {
+    if (!ctx || kclass < 0 || kclass >= KT_CLASSES || !launches || !total_ms || !bytes) return AMGS_ERR_ARG;
+    API_BEGIN
+    KTimer& k = ctx->c.kt[kclass];
+    k.resolve(ctx->c.stream);
+    *launches = k.launches;
+    *total_ms = k.ms;
+    *bytes = k.bytes;
     return AMGS_OK;
     API_END
 }

@@ -148,7 +148,7 @@ template <typename R> struct SapArgs {
     R mm;
     const cplx<R>* U;
     const int* blocks;  // block ids of this colour [ncol]
-    int ncol, nchunk, nrhs, m;
+    int ncol, nchunk, cs, nrhs, m;  // cs: sites per chunk (a multiple of FT)
     double tol;
     const cplx<R>* rglob;  // b of each item = global residual restricted to the block
     cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]
@@ -168,8 +168,8 @@ template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R
     t.rhs = blockIdx.y;
     t.item = t.rhs * a.ncol + i;
     t.blk = a.blocks[i];
-    t.m_lo = t.chunk * FT;
-    t.m_hi = min(a.g.sspb, t.m_lo + FT);
+    t.m_lo = t.chunk * a.cs;
+    t.m_hi = min(a.g.sspb, t.m_lo + a.cs);
     return t;
 }
 
@@ -295,10 +295,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
     __shared__ c64 red[8];
     const ItemIdx t = item_of(a);
     if (a.skip && a.skip[t.rhs]) return;
-    const int m = t.m_lo + threadIdx.x;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    cplx<R>* v0 = slot(a, 0, t.rhs);
     c64 q[2] = {{0, 0}, {0, 0}};
-    if (m < t.m_hi) {
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
         wilson_site<R, false>(a.U, a.rglob + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
@@ -306,11 +306,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
         a.xl[off + 2 * n] = b0;
         a.xl[off + 2 * n + 1] = b1;
         const cplx<R> r0 = b0 - o0, r1 = b1 - o1;
-        cplx<R>* v0 = slot(a, 0, t.rhs);
         v0[2 * n] = r0;
         v0[2 * n + 1] = r1;
-        q[0].re = double(norm2(r0)) + double(norm2(r1));
-        q[1].re = double(norm2(b0)) + double(norm2(b1));
+        q[0].re += double(norm2(r0)) + double(norm2(r1));
+        q[1].re += double(norm2(b0)) + double(norm2(b1));
     }
     block_sum<2>(q, red);
     if (threadIdx.x == 0) {
@@ -324,15 +323,16 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
     }
 }
 
-// Arnoldi step j, part A: prologue finalises step j-1 (or starts cycle k),
-// body w = s_j D_B Vt_j -> Vt_{j+1}; partials <w, Vt_i> (i <= j), <Vt_j, Vt_k> (k < j).
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int j, int k) {
+// Arnoldi step J, part A: prologue finalises step J-1 (or starts cycle k),
+// body w = s_J D_B Vt_J -> Vt_{J+1}; partials <w, Vt_i> (i <= J), <Vt_J, Vt_i> (i < J).
+template <typename R, int J>
+__global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
     load_item(L, a.st, t.item, M);
-    if (j == 0) {
+    if (J == 0) {
         reduce_chunks<double>(a.st.qr + (size_t)t.item * a.nchunk * 2, 2, 2, a.nchunk, L.tmp, L.dr);
     } else {
         reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
@@ -341,7 +341,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
         const SapState& st = a.st;
         const bool store = t.chunk == 0;
         int active = !L.done;
-        if (active && j == 0) {
+        if (active && J == 0) {
             // cycle start (src/fgmres.cpp:8-17, 61-65)
             const double err = sqrt(L.dr[0]);
             const double nb = k == 0 ? sqrt(L.dr[1]) : L.normb;
@@ -361,48 +361,46 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
         } else if (active) {
             if (L.jbreak >= 0) active = 0;
             else {
-                sap_finalize_step(st, t.item, L.dr[0], M, j - 1, a.tol, store, L);
+                sap_finalize_step(st, t.item, L.dr[0], M, J - 1, a.tol, store, L);
                 if (L.broke) active = 0;
             }
         }
         L.active = active;
+        if (act && active && t.chunk == 0) atomicAdd(act, 1);
     }
     __syncthreads();
     if (!L.active) return;
-    const double sj = L.s[j];
-    const int m = t.m_lo + threadIdx.x;
-    const cplx<R>* vj = slot(a, j, t.rhs);
-    cplx<R>* w = slot(a, j + 1, t.rhs);
-    const int nq = 2 * j + 1;
-    c64 w0 = {0, 0}, w1 = {0, 0}, a0 = {0, 0}, a1 = {0, 0};
-    int n = 0;
-    const bool live = m < t.m_hi;
-    if (live) {
-        n = sap_site(a.g, t.blk, m);
+    const R sj = R(L.s[J]);
+    const cplx<R>* vj = slot(a, J, t.rhs);
+    cplx<R>* w = slot(a, J + 1, t.rhs);
+    c64 pw[J + 1], pv[J + 1];
+#pragma unroll
+    for (int i = 0; i <= J; i++) pw[i] = pv[i] = c64{0, 0};
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
+        const int n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
         wilson_site<R, false>(a.U, vj, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
-        o0 = R(sj) * o0;
-        o1 = R(sj) * o1;
+        o0 = sj * o0;
+        o1 = sj * o1;
         w[2 * n] = o0;
         w[2 * n + 1] = o1;
-        w0 = cvt<double>(o0);
-        w1 = cvt<double>(o1);
-        a0 = cvt<double>(vj[2 * n]);
-        a1 = cvt<double>(vj[2 * n + 1]);
-    }
-    for (int i = 0; i <= j; i++) {
-        c64 pw = {0, 0}, pv = {0, 0};
-        if (live) {
+        const c64 w0 = cvt<double>(o0), w1 = cvt<double>(o1);
+        const c64 a0 = cvt<double>(vj[2 * n]), a1 = cvt<double>(vj[2 * n + 1]);
+#pragma unroll
+        for (int i = 0; i <= J; i++) {
             const cplx<R>* vi = slot(a, i, t.rhs);
             const c64 b0 = cvt<double>(vi[2 * n]), b1 = cvt<double>(vi[2 * n + 1]);
-            pw = mulc(w0, b0) + mulc(w1, b1);
-            pv = mulc(a0, b0) + mulc(a1, b1);
+            pw[i] += mulc(w0, b0) + mulc(w1, b1);
+            if (i < J) pv[i] += mulc(a0, b0) + mulc(a1, b1);
         }
-        park(pw, i, L.tmp, 2 * SAP_MAXM);
-        if (i < j) park(pv, j + 1 + i, L.tmp, 2 * SAP_MAXM);
+    }
+#pragma unroll
+    for (int i = 0; i <= J; i++) {
+        park(pw[i], i, L.tmp, 2 * SAP_MAXM);
+        if (i < J) park(pv[i], J + 1 + i, L.tmp, 2 * SAP_MAXM);
     }
     __syncthreads();
-    if (threadIdx.x < nq) {
+    if (threadIdx.x < 2 * J + 1) {
         c64 r = L.tmp[threadIdx.x];
         for (int wv = 1; wv < FT / 64; wv++) r += L.tmp[wv * 2 * SAP_MAXM + threadIdx.x];
         a.st.qa[((size_t)t.item * a.nchunk + t.chunk) * 2 * M + threadIdx.x] = r;
@@ -411,7 +409,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapA
 
 // Arnoldi step j, part B: prologue builds Hessenberg column j (one-pass MGS),
 // body w -= sum_i H_ij V_i (in place in Vt_{j+1}); partial |w|^2.
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j) {
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
@@ -430,6 +428,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
                 for (int kk = 0; kk < i; kk++) h -= L.Hcol[kk] * conj(L.G[i * (M + 1) + kk]);
                 L.Hcol[i] = h;
             }
+            for (int i = 0; i <= j; i++) L.eta[i] = L.s[i] * L.Hcol[i];  // coefficients on the Vt_i
             if (t.chunk == 0) {
                 c64* Hr = st.Hraw + (size_t)t.item * (M + 1) * M;
                 for (int i = 0; i <= j; i++) Hr[i * M + j] = L.Hcol[i];
@@ -438,25 +437,25 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
             }
         }
         L.active = active;
+        if (act && active && t.chunk == 0) atomicAdd(act, 1);
     }
     __syncthreads();
     if (!L.active) return;
-    const int m = t.m_lo + threadIdx.x;
+    cplx<R>* w = slot(a, j + 1, t.rhs);
     double nn = 0;
-    if (m < t.m_hi) {
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
-        cplx<R>* w = slot(a, j + 1, t.rhs);
         c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
         for (int i = 0; i <= j; i++) {
             const cplx<R>* vi = slot(a, i, t.rhs);
-            const c64 c = L.s[i] * L.Hcol[i];
+            const c64 c = L.eta[i];
             w0 -= c * cvt<double>(vi[2 * n]);
             w1 -= c * cvt<double>(vi[2 * n + 1]);
         }
         const cplx<R> r0 = cvt<R>(w0), r1 = cvt<R>(w1);
         w[2 * n] = r0;
         w[2 * n + 1] = r1;
-        nn = double(norm2(r0)) + double(norm2(r1));
+        nn += double(norm2(r0)) + double(norm2(r1));
     }
     c64 q[1] = {{nn, 0.0}};
     block_sum<1>(q, L.tmp);
@@ -464,8 +463,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
 }
 
 // End of cycle: finalise the last step if the cycle ran to m, back-substitute
-// (src/fgmres.cpp:52-59, 107-115) and update x_B += sum_j eta_j V_j.
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a) {
+// (src/fgmres.cpp:52-59, 107-115) and update x_B += sum_j eta_j V_j.  With
+// xacc set (last cycle) the block solution is added into the global x
+// instead (src/sap.cpp:117-125), also for items that converged earlier.
+template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a, cplx<R>* xacc) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
@@ -474,8 +475,9 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
     reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
     if (threadIdx.x == 0) {
         const int active = !L.done;
+        int maxIt = 0;
         if (active) {
-            int maxIt = L.maxIt;
+            maxIt = L.maxIt;
             if (L.jbreak < 0) {
                 sap_finalize_step(a.st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L);
                 if (L.broke) maxIt = M;
@@ -485,25 +487,31 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
                 for (int q = i + 1; q < maxIt; q++) e -= L.R[i * M + q] * L.eta[q];
                 L.eta[i] = cdiv(e, L.R[i * M + i]);
             }
-            L.maxIt = maxIt;
+            for (int i = 0; i < maxIt; i++) L.eta[i] = L.s[i] * L.eta[i];  // coefficients on the Vt_i
         }
+        L.maxIt = maxIt;
         L.active = active;
     }
     __syncthreads();
-    if (!L.active) return;
-    const int m = t.m_lo + threadIdx.x;
-    if (m < t.m_hi) {
+    if (!L.active && !xacc) return;
+    const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    const int mi = L.maxIt;
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
-        const size_t off = (size_t)t.rhs * a.g.ns * 2;
         c64 x0 = cvt<double>(a.xl[off + 2 * n]), x1 = cvt<double>(a.xl[off + 2 * n + 1]);
-        for (int jj = 0; jj < L.maxIt; jj++) {
+        for (int jj = 0; jj < mi; jj++) {
             const cplx<R>* vj = slot(a, jj, t.rhs);
-            const c64 c = L.s[jj] * L.eta[jj];
+            const c64 c = L.eta[jj];
             x0 += c * cvt<double>(vj[2 * n]);
             x1 += c * cvt<double>(vj[2 * n + 1]);
         }
-        a.xl[off + 2 * n] = cvt<R>(x0);
-        a.xl[off + 2 * n + 1] = cvt<R>(x1);
+        if (xacc) {
+            xacc[off + 2 * n] += cvt<R>(x0);
+            xacc[off + 2 * n + 1] += cvt<R>(x1);
+        } else {
+            a.xl[off + 2 * n] = cvt<R>(x0);
+            a.xl[off + 2 * n + 1] = cvt<R>(x1);
+        }
     }
 }
 
@@ -516,45 +524,36 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<
     if (threadIdx.x == 0) act = !a.st.done[t.item];
     __syncthreads();
     if (!act) return;
-    const int m = t.m_lo + threadIdx.x;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    cplx<R>* v0 = slot(a, 0, t.rhs);
     c64 q[1] = {{0, 0}};
-    if (m < t.m_hi) {
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
         wilson_site<R, false>(a.U, a.xl + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
         const cplx<R> r0 = a.rglob[off + 2 * n] - o0, r1 = a.rglob[off + 2 * n + 1] - o1;
-        cplx<R>* v0 = slot(a, 0, t.rhs);
         v0[2 * n] = r0;
         v0[2 * n + 1] = r1;
-        q[0].re = double(norm2(r0)) + double(norm2(r1));
+        q[0].re += double(norm2(r0)) + double(norm2(r1));
     }
     block_sum<1>(q, red);
     if (threadIdx.x == 0) a.st.qr[((size_t)t.item * a.nchunk + t.chunk) * 2] = q[0].re;
 }
 
-// x += x_B on the sites of this colour's blocks (src/sap.cpp:117-125)
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_accum(SapArgs<R> a, cplx<R>* x) {
-    const ItemIdx t = item_of(a);
-    if (a.skip && a.skip[t.rhs]) return;
-    const int m = t.m_lo + threadIdx.x;
-    if (m >= t.m_hi) return;
-    const int n = sap_site(a.g, t.blk, m);
-    const size_t off = (size_t)t.rhs * a.g.ns * 2;
-    x[off + 2 * n] += a.xl[off + 2 * n];
-    x[off + 2 * n + 1] += a.xl[off + 2 * n + 1];
-}
-
-// per-rhs SAP convergence |r| < tol |v| after a sweep (src/sap.cpp:159-168)
-__global__ void k_sap_check(const double* part_r, const double* part_v, int ntile, int nrhs, double tol, int* skip) {
-    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rhs >= nrhs || skip[rhs]) return;
-    double e = 0, v = 0;
-    for (int i = 0; i < ntile; i++) {
-        e += part_r[(size_t)rhs * ntile + i];
-        v += part_v[(size_t)rhs * ntile + i];
+// per-rhs SAP convergence |r| < tol |v| after a sweep (src/sap.cpp:159-168);
+// one workgroup per rhs
+__global__ void __launch_bounds__(FT) k_sap_check(const double* part_r, const double* part_v, int ntile, double tol,
+                                                  int* skip) {
+    __shared__ c64 red[8];
+    const int rhs = blockIdx.x;
+    if (skip[rhs]) return;
+    c64 q[2] = {{0, 0}, {0, 0}};
+    for (int i = threadIdx.x; i < ntile; i += FT) {
+        q[0].re += part_r[(size_t)rhs * ntile + i];
+        q[1].re += part_v[(size_t)rhs * ntile + i];
     }
-    if (sqrt(e) < tol * sqrt(v)) skip[rhs] = 1;
+    block_sum<2>(q, red);
+    if (threadIdx.x == 0 && sqrt(q[0].re) < tol * sqrt(q[1].re)) skip[rhs] = 1;
 }
 
 // |v|^2 per tile (for SAP's ||v||)

@@ -149,6 +149,16 @@ int amgs_write_residuals(const char* path, const double* hist, int n);
 int amgs_time_kernel(amgs_ctx* c, int kernel, int nrhs, int reps, double* avg_ms);
 /* Phase timers of the last solve (ms): [0] total, [1] setup, [2] solve. */
 int amgs_last_timings(amgs_ctx* c, double* ms3);
+/* Per-launch timing of one kernel class during later solves: a HIP event
+ * pair on the context's stream around every launch of the class, plus the
+ * algorithmic bytes of the items that did work in that launch.
+ * kclass: 0 = SAP Arnoldi step A (block stencil + dot products),
+ *         1 = SAP Arnoldi step B (orthogonalisation + norm).
+ * enable != 0 starts recording (and clears the totals), 0 stops it.
+ * amgs_kernel_stats returns the totals: launches, summed device time (ms)
+ * and summed algorithmic bytes. */
+int amgs_kernel_timing(amgs_ctx* c, int kclass, int enable);
+int amgs_kernel_stats(amgs_ctx* c, int kclass, long long* launches, double* total_ms, double* bytes);
 
 #ifdef __cplusplus
 }
