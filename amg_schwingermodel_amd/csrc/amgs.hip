@@ -373,8 +373,14 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
     const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
     a.crs_lds = dyn <= lds_cap ? 1 : 0;
+    size_t dyn_bytes = a.crs_lds ? dyn : 0;
+    // SAP block vectors of a level share that region (never live at the same time)
+    for (int l = 0; l < MAXL; l++) {
+        const size_t need = l >= 1 && l < c->L - 1 ? CW * a.lv[l].blkStride * sizeof(cplx<R>) : 0;
+        a.sap_lds[l] = need > 0 && need <= lds_cap ? 1 : 0;
+        if (a.sap_lds[l]) dyn_bytes = std::max(dyn_bytes, need);
+    }
     a.prof = prof.p ? prof.as<unsigned long long>() : nullptr;
-    const size_t dyn_bytes = a.crs_lds ? dyn : 0;
     a.in = in;
     a.out = out;
     a.in_stride = in_stride;

@@ -53,6 +53,7 @@ template <typename R> struct CArgs {
     cplx<R>* ws;
     size_t ws_per_rhs;
     int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
+    int sap_lds[MAXL];  // SAP block vectors of level l in dynamic LDS (CW * blkStride entries)
     unsigned long long* prof;  // optional phase timers [rhs][32] (wall clock ticks)
     int mode, level, nu;
     const cplx<R>* in;
@@ -135,14 +136,12 @@ template <typename R> __device__ __forceinline__ cplx<R> ld(gptr<cplx<R>> p) {
 }
 template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { return *p; }
 
+// Row r of site x: the five link blocks (column stride D) against the on-site
+// and the four neighbour vectors, five independent accumulators.
 template <typename R, typename VP>
-__device__ __forceinline__ void stencil_row(gptr<cplx<R>> links, int ksz, int msz, int D, int x, int r, VP v0, VP vr0,
-                                            VP vr1, VP vl0, VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
-    gptr<cplx<R>> L0 = links + x * msz + r;
-    gptr<cplx<R>> L1 = L0 + ksz;
-    gptr<cplx<R>> L2 = L0 + 2 * ksz;
-    gptr<cplx<R>> L3 = L0 + 3 * ksz;
-    gptr<cplx<R>> L4 = L0 + 4 * ksz;
+__device__ __forceinline__ void stencil_row_p(gptr<cplx<R>> L0, gptr<cplx<R>> L1, gptr<cplx<R>> L2, gptr<cplx<R>> L3,
+                                              gptr<cplx<R>> L4, int D, VP v0, VP vr0, VP vr1, VP vl0, VP vl1, R f1, R f2,
+                                              R f3, R f4, cplx<R>& acc) {
     cplx<R> a0 = czero<R>(), a1 = czero<R>(), a2 = czero<R>(), a3 = czero<R>(), a4 = czero<R>();
 #pragma unroll 4
     for (int c = 0; c < D; c++) {
@@ -156,6 +155,14 @@ __device__ __forceinline__ void stencil_row(gptr<cplx<R>> links, int ksz, int ms
     acc -= a0;
     acc -= f1 * a1 + f3 * a3;
     acc -= f2 * a2 + f4 * a4;
+}
+
+template <typename R, typename VP>
+__device__ __forceinline__ void stencil_row(gptr<cplx<R>> links, int ksz, int msz, int D, int x, int r, VP v0, VP vr0,
+                                            VP vr1, VP vl0, VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
+    gptr<cplx<R>> L0 = links + x * msz + r;
+    stencil_row_p<R>(L0, L0 + ksz, L0 + 2 * ksz, L0 + 3 * ksz, L0 + 4 * ksz, D, v0, vr0, vr1, vl0, vl1, f1, f2, f3, f4,
+                     acc);
 }
 
 template <typename R, typename VP>
@@ -257,14 +264,16 @@ template <typename R> __device__ __forceinline__ c64 w_dot(const cplx<R>* a, con
     return wave_sum(s);
 }
 
-// block-local stencil, neighbours outside block blk read as zero
-// (multiplier 0; the read is redirected to the site itself to stay in bounds)
+// block-local stencil, neighbours outside block blk read as zero: multiplier
+// 0, and both the vector and the link reads are redirected to the site's own
+// (already loaded) entries, so no link block outside the block is fetched
 template <typename R>
 __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63;
     const int msz = D * D, ksz = g.ns * msz;
+    gptr<cplx<R>> links = as_global(L.links);
     for (int e = lane; e < nb; e += 64) {
         const int m = e / D, r = e - m * D;
         const int n = sap_site(g, blk, m);
@@ -272,16 +281,19 @@ __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, cons
                       nb_left(n, 1, g.nx, g.nt)};
         R f[4];
         const cplx<R>* vp[4];
+        gptr<cplx<R>> lk[5];
+        lk[0] = links + n * msz + r;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const bool inb = sap_block_of(g, nbr[q]) == blk;
             const bool wrap = (q == 0 && wraps_right(n, 0, g.nt)) || (q == 2 && wraps_left(n, 0, g.nt));
             f[q] = inb ? (wrap ? R(-1) : R(1)) : R(0);
             vp[q] = in + (size_t)(inb ? sap_local_of(g, nbr[q]) : m) * D;
+            lk[q + 1] = inb ? lk[0] + (q + 1) * ksz : lk[0];
         }
         cplx<R> acc = A.mm * in[e];
-        stencil_row<R>(as_global(L.links), ksz, msz, D, n, r, in + m * D, vp[0], vp[1], vp[2], vp[3], f[0], f[1], f[2],
-                       f[3], acc);
+        stencil_row_p<R>(lk[0], lk[1], lk[2], lk[3], lk[4], D, in + m * D, vp[0], vp[1], vp[2], vp[3], f[0], f[1], f[2],
+                         f[3], acc);
         out[e] = acc;
     }
     wave_sync();
@@ -312,7 +324,9 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = A.sap_m;
     WaveScal& S = K.S.wv[wv];
-    cplx<R>* base = K.W + L.blk + (size_t)wv * L.blkStride;
+    // block Krylov vectors in the dynamic LDS region when they fit (it is
+    // otherwise used only by the coarsest GMRES, never live during SAP)
+    cplx<R>* base = (A.sap_lds[l] ? K.dyn : K.W + L.blk) + (size_t)wv * L.blkStride;
     auto V = [&](int j) { return base + (size_t)j * nb; };
     cplx<R>* xb = base + (size_t)(m + 1) * nb;
     cplx<R>* bb = base + (size_t)(m + 2) * nb;
