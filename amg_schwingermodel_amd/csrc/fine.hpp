@@ -373,9 +373,11 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
     const R sj = R(L.s[J]);
     const cplx<R>* vj = slot(a, J, t.rhs);
     cplx<R>* w = slot(a, J + 1, t.rhs);
-    c64 pw[J + 1], pv[J + 1];
+    // per-thread partial sums in the working precision, combined in FP64
+    cplx<R> pw[J + 1], pv[J + 1];
 #pragma unroll
-    for (int i = 0; i <= J; i++) pw[i] = pv[i] = c64{0, 0};
+    for (int i = 0; i <= J; i++) pw[i] = pv[i] = czero<R>();
+#pragma unroll 2
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
@@ -384,20 +386,19 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
         o1 = sj * o1;
         w[2 * n] = o0;
         w[2 * n + 1] = o1;
-        const c64 w0 = cvt<double>(o0), w1 = cvt<double>(o1);
-        const c64 a0 = cvt<double>(vj[2 * n]), a1 = cvt<double>(vj[2 * n + 1]);
+        const cplx<R> a0 = vj[2 * n], a1 = vj[2 * n + 1];
 #pragma unroll
         for (int i = 0; i <= J; i++) {
             const cplx<R>* vi = slot(a, i, t.rhs);
-            const c64 b0 = cvt<double>(vi[2 * n]), b1 = cvt<double>(vi[2 * n + 1]);
-            pw[i] += mulc(w0, b0) + mulc(w1, b1);
+            const cplx<R> b0 = vi[2 * n], b1 = vi[2 * n + 1];
+            pw[i] += mulc(o0, b0) + mulc(o1, b1);
             if (i < J) pv[i] += mulc(a0, b0) + mulc(a1, b1);
         }
     }
 #pragma unroll
     for (int i = 0; i <= J; i++) {
-        park(pw[i], i, L.tmp, 2 * SAP_MAXM);
-        if (i < J) park(pv[i], J + 1 + i, L.tmp, 2 * SAP_MAXM);
+        park(cvt<double>(pw[i]), i, L.tmp, 2 * SAP_MAXM);
+        if (i < J) park(cvt<double>(pv[i]), J + 1 + i, L.tmp, 2 * SAP_MAXM);
     }
     __syncthreads();
     if (threadIdx.x < 2 * J + 1) {
