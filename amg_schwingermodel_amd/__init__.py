@@ -46,6 +46,8 @@ class Params(ctypes.Structure):
         ("tv_seed", ctypes.c_uint32),
         ("precision", ctypes.c_int),
         ("max_rhs", ctypes.c_int),
+        ("cycle", ctypes.c_int),
+        ("nit", ctypes.c_int),
     ]
 
 

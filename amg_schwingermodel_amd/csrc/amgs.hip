@@ -314,6 +314,7 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     ca.sap_tol = c->P.sap_tol;
     ca.nu1 = c->P.nu1;
     ca.nu2 = c->P.nu2;
+    ca.vcycle = c->P.cycle == 0;
     size_t wo = 0;
     for (int l = 1; l < c->L; l++) {
         CLevel<R>& lv = ca.lv[l];
@@ -499,7 +500,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
 }
 
 // One application psi = K(eta) at level 0 from psi = 0 (include/amg.h:134-144,
-// src/amg.cpp:135-178).
+// src/amg.cpp:135-178); with P.cycle == 0 the V-cycle (src/amg.cpp:90-132).
 template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi) {
     const Geo& g = c->geo[0];
     const size_t n0 = c->nvec(0), n1 = c->nvec(1);
@@ -514,7 +515,8 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
         rr = res0.as<cplx<R>>();
     }
     restrict0(nrhs, tv[0], rr, eta1.as<cplx<R>>());
-    coarse(CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
+    // K-cycle: FGMRES_k_cycle[1] (src/amg.cpp:161); V-cycle: v_cycle(1) from zero (src/amg.cpp:110)
+    coarse(c->P.cycle == 0 ? CM_CYCLE : CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
     hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
                        (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, 1);
     if (c->P.nu2 > 0) sap0(nrhs, eta, psi, c->P.nu2);
@@ -551,18 +553,59 @@ static void smooth_level(Ctx& c, int l) {
     }
 }
 
+// orthonormalise level l's test vectors and build level l+1's links
+// (src/level.cpp:242-301, 373-417)
+static void ortho_links(Ctx& c, int l) {
+    const Geo& g = c.geo[l];
+    hipLaunchKernelGGL(k_orthonormalize, dim3(g.nagg), dim3(ST), 0, c.stream, g, c.tv64[l].as<c64>());
+    const Geo& gc = c.geo[l + 1];
+    LinkSrc src{g, l == 0, c.U64.as<c64>(), l == 0 ? nullptr : c.links64[l].as<c64>()};
+    hipLaunchKernelGGL(k_galerkin, dim3(gc.ns, 4), dim3(ST), 0, c.stream, src, (const c64*)c.tv64[l].as<c64>(), gc.ns,
+                       gc, c.links64[l + 1].as<c64>());
+    HIP_CHECK(hipGetLastError());
+}
+
+// Bootstrap (src/amg.cpp:52-84) in FP64: per level, every test vector v
+// becomes v + cycle_l(v - D_l v) from a zero guess with the current
+// hierarchy (all test vectors of the level as one batch), then the level's
+// interpolator and the next level's links are rebuilt.
+static void bootstrap(Ctx& c) {
+    Engine<double>& e = c.e64;
+    DevBuf rhs, tvn;
+    for (int it = 0; it < c.P.nit; it++)
+        for (int l = 0; l < c.L - 1; l++) {
+            const Geo& g = c.geo[l];
+            const int nv = g.ntest;
+            const size_t n = c.nvec(l), tot = (size_t)nv * n;
+            rhs.alloc(sizeof(c64) * tot);
+            tvn.alloc(sizeof(c64) * tot);
+            c64* w = c.tv64[l].as<c64>();
+            const unsigned nb = cdiv_u(tot, 256);
+            if (l == 0) {
+                hipLaunchKernelGGL(k_resid<double>, dim3(cdiv_u(g.ns, FT), nv), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                                   (const c64*)c.U64.as<c64>(), (const c64*)w, (const c64*)w, rhs.as<c64>(),
+                                   (double*)nullptr, (const int*)nullptr);
+                e.kcycle0(nv, rhs.as<c64>(), tvn.as<c64>());
+            } else {
+                e.coarse(CM_DOP, l, nv, w, n, rhs.as<c64>(), n);
+                hipLaunchKernelGGL(k_lin<double>, dim3(nb), dim3(256), 0, c.stream, tot, 1.0, (const c64*)w, -1.0,
+                                   (const c64*)rhs.as<c64>(), rhs.as<c64>());
+                e.coarse(CM_CYCLE, l, nv, rhs.as<c64>(), n, tvn.as<c64>(), n);
+            }
+            hipLaunchKernelGGL(k_lin<double>, dim3(nb), dim3(256), 0, c.stream, tot, 1.0, (const c64*)tvn.as<c64>(),
+                               1.0, (const c64*)w, w);
+            ortho_links(c, l);
+        }
+    c.sync();
+}
+
 static void setup_common(Ctx& c, bool smooth) {
     bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
     for (int l = 0; l < c.L - 1; l++) {
         if (smooth) smooth_level(c, l);
-        const Geo& g = c.geo[l];
-        hipLaunchKernelGGL(k_orthonormalize, dim3(g.nagg), dim3(ST), 0, c.stream, g, c.tv64[l].as<c64>());
-        const Geo& gc = c.geo[l + 1];
-        LinkSrc src{g, l == 0, c.U64.as<c64>(), l == 0 ? nullptr : c.links64[l].as<c64>()};
-        hipLaunchKernelGGL(k_galerkin, dim3(gc.ns, 4), dim3(ST), 0, c.stream, src, (const c64*)c.tv64[l].as<c64>(),
-                           gc.ns, gc, c.links64[l + 1].as<c64>());
-        HIP_CHECK(hipGetLastError());
+        ortho_links(c, l);
     }
+    if (c.P.nit > 0) bootstrap(c);
     if (c.P.precision == AMGS_PREC_MIXED) {
         convert64to32(c, c.U64, c.U32, c.nvec(0));
         for (int l = 0; l < c.L; l++) {

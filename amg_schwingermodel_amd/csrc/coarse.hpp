@@ -22,7 +22,7 @@ constexpr int MAXL = 5;        // maximum number of levels
 constexpr int KC_MAXM = 8;     // max restart length of K-cycle FGMRES and SAP GMRES
 constexpr int CRS_MAXM = 24;   // max restart length of the coarsest GMRES
 
-enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4 };
+enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4, CM_CYCLE = 5 };
 
 template <typename R> struct CLevel {
     Geo g;
@@ -50,6 +50,7 @@ template <typename R> struct CArgs {
     int sap_m, sap_restarts;
     double sap_gtol, sap_tol;
     int nu1, nu2;
+    int vcycle;  // 1: V-cycle (src/amg.cpp:90-132), 0: K-cycle (src/amg.cpp:135-178)
     cplx<R>* ws;
     size_t ws_per_rhs;
     int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
@@ -586,7 +587,9 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
     }
 }
 
-// k_cycle(l, eta, psi) (src/amg.cpp:135-178)
+// k_cycle(l, eta, psi) (src/amg.cpp:135-178), or v_cycle(l, eta, psi)
+// (src/amg.cpp:90-132) when A.vcycle: the coarse correction is then a
+// recursive cycle from a zero guess instead of the level's K-cycle FGMRES.
 template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi) {
     if (LV == A.L - 1) {
         c_fgmres<R, LV, CRS_MAXM>(A, K, K.S.crs, true, eta, psi);
@@ -605,7 +608,14 @@ template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CAr
                 c_dop(A, LV, psi, Dpsi, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
                 for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
                 c_restrict(A, LV, r, eta1);
-                c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
+                if (A.vcycle) {
+                    const int nc = A.lv[LV + 1].g.ns * A.lv[LV + 1].g.dof;
+                    for (int e = threadIdx.x; e < nc; e += CT) psi1[e] = czero<R>();
+                    __syncthreads();
+                    c_kcycle<R, LV + 1>(A, K, eta1, psi1);
+                } else {
+                    c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
+                }
                 c_prolong(A, LV, psi1, psi, true);
             }
             const int nu = ph == 0 ? A.nu1 : A.nu2;  // pre- / post-smoothing
@@ -633,6 +643,18 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
         case CM_DOP: c_dop(A, l, in, out); break;
         case CM_RESTRICT: c_restrict(A, l, in, out); break;
         case CM_PROLONG: c_prolong(A, l, in, out, false); break;
+        case CM_CYCLE: {  // the cycle at level l from psi = 0
+            const int n = A.lv[l].g.ns * A.lv[l].g.dof;
+            for (int e = threadIdx.x; e < n; e += CT) out[e] = czero<R>();
+            __syncthreads();
+            switch (l) {
+                case 1: c_kcycle<R, 1>(A, K, in, out); break;
+                case 2: c_kcycle<R, 2>(A, K, in, out); break;
+                case 3: c_kcycle<R, 3>(A, K, in, out); break;
+                case 4: c_kcycle<R, 4>(A, K, in, out); break;
+            }
+            break;
+        }
     }
     Ttot.lap(31);
 }

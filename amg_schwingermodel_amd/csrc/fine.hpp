@@ -643,6 +643,12 @@ __global__ void k_convert(size_t n, const cplx<From>* __restrict__ in, cplx<To>*
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = cvt<To>(in[i]);
 }
+// out = a x + b y (real a, b; out may alias x or y)
+template <typename R>
+__global__ void k_lin(size_t n, double a, const cplx<R>* x, double b, const cplx<R>* y, cplx<R>* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = R(a) * x[i] + R(b) * y[i];
+}
 template <typename R> __global__ void k_zero(size_t n, cplx<R>* out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = czero<R>();

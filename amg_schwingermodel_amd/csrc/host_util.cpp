@@ -150,6 +150,8 @@ extern "C" int amgs_default_params(amgs_params* p) {
     p->tv_seed = 50;
     p->precision = AMGS_PREC_FP64;
     p->max_rhs = 1;
+    p->cycle = 1;
+    p->nit = 0;
     return AMGS_OK;
 }
 

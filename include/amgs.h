@@ -66,6 +66,8 @@ typedef struct {
     uint32_t tv_seed;                                /* src/amg.cpp:8 (50)        */
     int precision;                                   /* AMGS_PREC_*               */
     int max_rhs;                                     /* batch capacity (>= 1)     */
+    int cycle;                                       /* 1 K-cycle, 0 V-cycle (src/main.cpp:58) */
+    int nit;                                         /* bootstrap iterations (src/amg.cpp:52-84) */
 } amgs_params;
 
 typedef struct amgs_ctx amgs_ctx;

@@ -30,6 +30,8 @@ class Params(ctypes.Structure):
         ("tv_sap_iters", ctypes.c_int),
         ("seed", ctypes.c_uint),
         ("threads", ctypes.c_int),
+        ("cycle", ctypes.c_int),
+        ("nit", ctypes.c_int),
     ]
 
 
@@ -65,6 +67,7 @@ def lib():
         L.or_get_tv.argtypes = [vp, ctypes.c_int, ctypes.c_int, P]
         L.or_get_links.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
         L.or_kcycle0.argtypes = [vp, P, P]
+        L.or_vcycle.argtypes = [vp, ctypes.c_int, P, P]
         L.or_solve.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double)]
         L.or_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
@@ -171,6 +174,13 @@ class Oracle:
 
     def kcycle0(self, eta):
         return self._un(lib().or_kcycle0, eta, self.size(0))
+
+    def vcycle(self, l, eta, psi=None):
+        """V-cycle at level l from psi (zero when omitted)."""
+        eta = _c(eta).reshape(-1)
+        out = np.zeros(self.size(l), np.complex128) if psi is None else _c(psi).reshape(-1).copy()
+        lib().or_vcycle(self.h, l, _p(eta), _p(out))
+        return out
 
     def solve(self, b, hist_cap=4096):
         b = _c(b).reshape(-1)

@@ -135,6 +135,8 @@ void or_default_params(or_params* p) {
     p->out_m = 20; p->out_restarts = 50; p->out_tol = 1e-10;
     p->tv_sap_iters = 4;
     p->seed = 50;
+    p->cycle = 1;                      /* src/main.cpp:58 */
+    p->nit = 0;                        /* src/main.cpp:129 */
 }
 
 /* Neighbour of site n on an nx x nt periodic lattice (include/boundary.h:36-40). */
@@ -611,11 +613,38 @@ static void kcycle(or_ctx* c, int l, const cx* eta, cx* psi) {
     if (c->p.nu2 > 0) sap(c, l, eta, psi, c->p.nu2);
 }
 
-/* Outer preconditioner: zero, then K-cycle at level 0 (include/amg.h:134-144). */
+/* V-cycle (src/amg.cpp:90-132): smoothing, coarse correction by a recursive
+ * V-cycle from a zero guess, smoothing; GMRES from x0 = eta at the coarsest. */
+static void vcycle(or_ctx* c, int l, const cx* eta, cx* psi) {
+    lvl_t* v = &c->lv[l];
+    oparg_t oa = {l, 0};
+    if (l == c->L - 1) {
+        fgmres(c, &v->crs, op_dop, &oa, NULL, NULL, eta, eta, psi, NULL, 0, NULL);
+        return;
+    }
+    const size_t n = (size_t)v->ns * v->dof;
+    const size_t nc = (size_t)c->lv[l + 1].ns * c->lv[l + 1].dof;
+    if (c->p.nu1 > 0) sap(c, l, eta, psi, c->p.nu1);
+    dop(c, l, psi, v->t_Dpsi);
+    for (size_t i = 0; i < n; i++) v->t_r[i] = eta[i] - v->t_Dpsi[i];
+    restrict_(c, l, v->t_r, v->t_eta1);
+    memset(v->t_psi1, 0, sizeof(cx) * nc);
+    vcycle(c, l + 1, v->t_eta1, v->t_psi1);
+    prolong(c, l, v->t_psi1, v->t_P);
+    for (size_t i = 0; i < n; i++) psi[i] += v->t_P[i];
+    if (c->p.nu2 > 0) sap(c, l, eta, psi, c->p.nu2);
+}
+
+static void cycle(or_ctx* c, int l, const cx* eta, cx* psi) {
+    if (c->p.cycle == 0) vcycle(c, l, eta, psi);
+    else kcycle(c, l, eta, psi);
+}
+
+/* Outer preconditioner: zero, then the cycle at level 0 (include/amg.h:134-144). */
 static void pre_outer(or_ctx* c, const void* a, const cx* in, cx* out) {
     (void)a;
     memset(out, 0, sizeof(cx) * (size_t)c->lv[0].ns * 2);
-    kcycle(c, 0, in, out);
+    cycle(c, 0, in, out);
 }
 
 /* ------------------------------------------------------------------ */
@@ -716,6 +745,32 @@ static void smooth_ortho_links(or_ctx* c, int smooth) {
     }
 }
 
+/* Bootstrap (src/amg.cpp:52-84): per level, each test vector v becomes
+ * v + cycle_l(v - D_l v) from a zero guess, using the current hierarchy; then
+ * the level's interpolator and the next level's links are rebuilt. */
+static void bootstrap(or_ctx* c) {
+    for (int it = 0; it < c->p.nit; it++)
+        for (int l = 0; l < c->L - 1; l++) {
+            lvl_t* v = &c->lv[l];
+            const size_t n = (size_t)v->ns * v->dof;
+            cx* tvn = xcalloc((size_t)v->ntest * n, sizeof(cx));
+            cx* rhs = xcalloc(n, sizeof(cx));
+            for (int i = 0; i < v->ntest; i++) {
+                const cx* w = v->w + i * n;
+                cx* t = tvn + i * n;
+                dop(c, l, w, rhs);
+                for (size_t q = 0; q < n; q++) rhs[q] = w[q] - rhs[q];
+                cycle(c, l, rhs, t);
+                for (size_t q = 0; q < n; q++) t[q] += w[q];
+            }
+            memcpy(v->w, tvn, sizeof(cx) * (size_t)v->ntest * n);
+            free(tvn);
+            free(rhs);
+            orthonormalize(c, l);
+            coarse_links(c, l);
+        }
+}
+
 void or_setup(or_ctx* c, int draw_order) {
     mt_t rng;
     mt_seed(&rng, c->p.seed);
@@ -734,6 +789,7 @@ void or_setup(or_ctx* c, int draw_order) {
         }
     }
     smooth_ortho_links(c, 1);
+    bootstrap(c);
 }
 
 void or_setup_from_tv(or_ctx* c, const double* const* tv, int smooth) {
@@ -742,6 +798,7 @@ void or_setup_from_tv(or_ctx* c, const double* const* tv, int smooth) {
         memcpy(v->w, tv[l], sizeof(cx) * (size_t)v->ntest * v->ns * v->dof);
     }
     smooth_ortho_links(c, smooth);
+    bootstrap(c);
 }
 
 void or_get_tv(or_ctx* c, int l, int i, double* out) {
@@ -766,6 +823,8 @@ void or_dop(or_ctx* c, int l, const double* in, double* out) { dop(c, l, (const 
 void or_prolong(or_ctx* c, int l, const double* vc, double* out) { prolong(c, l, (const cx*)vc, (cx*)out); }
 void or_restrict(or_ctx* c, int l, const double* vf, double* out) { restrict_(c, l, (const cx*)vf, (cx*)out); }
 int or_sap(or_ctx* c, int l, const double* v, double* x, int nu) { return sap(c, l, (const cx*)v, (cx*)x, nu); }
+
+void or_vcycle(or_ctx* c, int l, const double* eta, double* psi) { vcycle(c, l, (const cx*)eta, (cx*)psi); }
 
 void or_kcycle0(or_ctx* c, const double* eta, double* psi) {
     memset(psi, 0, sizeof(cx) * (size_t)c->lv[0].ns * 2);

@@ -40,6 +40,8 @@ typedef struct {
     int tv_sap_iters;                               /* src/main.cpp:63           */
     unsigned seed;                                  /* src/amg.cpp:8             */
     int threads;                                    /* OpenMP threads (0: default) */
+    int cycle;                                      /* 1 K-cycle, 0 V-cycle (src/main.cpp:58) */
+    int nit;                                        /* bootstrap iterations (src/amg.cpp:52-84) */
 } or_params;
 
 typedef struct or_ctx or_ctx;
@@ -80,6 +82,8 @@ void or_get_links(or_ctx* c, int l, int which, int mu, double* out);
 
 /* One K-cycle application psi = K(eta) at level 0 from psi = 0 (include/amg.h:134-144). */
 void or_kcycle0(or_ctx* c, const double* eta, double* psi);
+/* V-cycle (src/amg.cpp:90-132) at level l from psi as given. */
+void or_vcycle(or_ctx* c, int l, const double* eta, double* psi);
 
 /* Outer FGMRES with the K-cycle (src/fgmres.cpp:5-88, include/amg.h:99-145).
  * Returns the reference's iteration count (k*m + maxIt, or restarts*m).

@@ -65,8 +65,12 @@ static void common_init(const char* params, double m0, int tv_iters) {
     readParameters(params);
     Coordinates();
     boundary();
-    AMGV::cycle = 1;
-    AMGV::Nit = 0;
+    // cycle type and bootstrap iterations: K-cycle and Nit = 0 as main() runs
+    // them (src/main.cpp:58,129) unless AMGS_REF_CYCLE / AMGS_REF_NIT say otherwise
+    const char* cy = getenv("AMGS_REF_CYCLE");
+    const char* nit = getenv("AMGS_REF_NIT");
+    AMGV::cycle = cy ? atoi(cy) : 1;
+    AMGV::Nit = nit ? atoi(nit) : 0;
     AMGV::SAP_test_vectors_iterations = tv_iters;
     mass::m0 = m0;
     SAPV::sap_blocks_per_proc = 1 << 20;
@@ -97,7 +101,7 @@ static int mode_setup(int argc, char** argv) {
     }
 
     AlgebraicMG amg(G, m0, AMGV::nu1, AMGV::nu2);
-    amg.setUpPhase(0);
+    amg.setUpPhase(AMGV::Nit);
 
     for (int l = 0; l < L - 1; l++) {
         Level* lv = amg.levels[l];
@@ -136,6 +140,9 @@ static int mode_setup(int argc, char** argv) {
         spinor psi(LV::Ntot, c_vector(2, 0));
         amg.k_cycle(0, eta, psi);
         write_spinor(od + "/kcycle.bin", psi);
+        spinor psv(LV::Ntot, c_vector(2, 0));
+        amg.v_cycle(0, eta, psv);
+        write_spinor(od + "/vcycle.bin", psv);
     }
     return 0;
 }

@@ -37,6 +37,13 @@ CASES = {
     "setup_b": dict(mode="setup", rows=ROWS_B, m0=-0.10, tv=2, field=(21, 0.4), vec=22),
     "solve_a": dict(mode="solve", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, restarts=50),
     "solve_c": dict(mode="solve", rows=ROWS_A, m0=-0.18, tv=4, field=(11, 0.5), vec=12, restarts=1),
+    # bootstrap set-up (Nit > 0) with the K-cycle and with the V-cycle, and
+    # V-cycle / bootstrapped K-cycle solves
+    "setup_boot": dict(mode="setup", rows=ROWS_A, m0=-0.18, tv=4, field=(11, 0.5), vec=12, cycle=1, nit=1),
+    "setup_vboot": dict(mode="setup", rows=ROWS_A, m0=-0.18, tv=4, field=(11, 0.5), vec=12, cycle=0, nit=1),
+    "solve_v": dict(mode="solve", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, restarts=50, cycle=0),
+    "solve_boot": dict(mode="solve", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, restarts=50, cycle=1,
+                       nit=2),
 }
 
 NSAMP = 64
@@ -76,7 +83,8 @@ def run_case(name, cfg):
         else:
             cmd = [HARNESS, "solve", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin",
                    "vec/b.bin", "out", str(cfg["restarts"])]
-        subprocess.run(cmd, cwd=td, check=True, stdout=subprocess.DEVNULL)
+        env = dict(os.environ, AMGS_REF_CYCLE=str(cfg.get("cycle", 1)), AMGS_REF_NIT=str(cfg.get("nit", 0)))
+        subprocess.run(cmd, cwd=td, check=True, stdout=subprocess.DEVNULL, env=env)
         for fn in sorted(os.listdir(od)):
             path = os.path.join(od, fn)
             if fn.endswith(".bin"):

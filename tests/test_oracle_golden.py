@@ -57,10 +57,12 @@ def links_ref_layout(o, l):
     return G1.reshape(-1), G2.reshape(-1), G3.reshape(-1)
 
 
-@pytest.mark.parametrize("case", ["setup_a", "setup_b"])
+@pytest.mark.parametrize("case", ["setup_a", "setup_b", "setup_boot", "setup_vboot"])
 def test_setup_components(case):
+    """setup_boot / setup_vboot: bootstrap set-up (Nit = 1, src/amg.cpp:52-84)
+    with the K-cycle and with the V-cycle."""
     z, meta = load(case)
-    o, v, rows = build(meta)
+    o, v, rows = build(meta, cycle=meta.get("cycle", 1), nit=meta.get("nit", 0))
     L = len(rows) + 1
     check(z, "Dphi", o.dphi(v["psi"]))
     check(z, "Ddagphi", o.ddag(v["psi"]))
@@ -81,6 +83,8 @@ def test_setup_components(case):
             x, _ = o.sap(l, v[f"v{l}"], v[f"x{l}"], 2)
             check(z, f"SAP_l{l}", x)
     check(z, "kcycle", o.kcycle0(v["v0"]))
+    if "vcycle__sha" in z.files:
+        check(z, "vcycle", o.vcycle(0, v["v0"]))
 
 
 def test_draw_order_matters():
@@ -91,10 +95,11 @@ def test_draw_order_matters():
     assert inputs.sha(o.tv(0, 0)) != str(z["tv_l0_0__sha"])
 
 
-@pytest.mark.parametrize("case", ["solve_a", "solve_c"])
+@pytest.mark.parametrize("case", ["solve_a", "solve_c", "solve_v", "solve_boot"])
 def test_full_solve(case):
+    """solve_v: V-cycle preconditioner; solve_boot: K-cycle after Nit = 2."""
     z, meta = load(case)
-    o, v, rows = build(meta, out_restarts=meta["restarts"])
+    o, v, rows = build(meta, out_restarts=meta["restarts"], cycle=meta.get("cycle", 1), nit=meta.get("nit", 0))
     o.setup(draw_order=1)
     x, it, hist, rr = o.solve(v["b"])
     assert it == int(z["iterations"])
