@@ -121,7 +121,9 @@ int amgs_apply_P(amgs_ctx* c, int level, int nrhs, const double* coarse, double*
 int amgs_apply_Pt(amgs_ctx* c, int level, int nrhs, const double* fine, double* coarse);
 /* SAP_C::SAP(v, x, nu) (src/sap.cpp:64-174); x is updated in place. */
 int amgs_sap(amgs_ctx* c, int level, int nrhs, const double* v, double* x, int nu);
-/* One preconditioner application psi = K(eta) from psi = 0 (include/amg.h:134-144). */
+/* One preconditioner application psi = K(eta) from psi = 0 (include/amg.h:134-144):
+ * the K-cycle (src/amg.cpp:135-178), or the V-cycle (src/amg.cpp:90-132) when
+ * params.cycle == 0. */
 int amgs_kcycle(amgs_ctx* c, int nrhs, const double* eta, double* psi);
 /* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
  * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as

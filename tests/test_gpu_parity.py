@@ -26,11 +26,13 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def make(case, setup=True, **kw):
+def make(case, setup=True, common=None, **kw):
+    """GPU solver (parameters kw + common) and oracle (common) on one field."""
     nx, nt, rows, m0, beta = CASES[case]
+    common = common or {}
     U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
-    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, **kw), U)
-    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, **common, **kw), U)
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8, **common), U)
     if setup:
         s.setup()
         o.setup(draw_order=1)
@@ -137,3 +139,39 @@ def test_mixed_precision_converges():
     assert rr < 1e-10
     assert rel(x, xo) < 1e-8
     assert it <= ito + 3
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_vcycle(case):
+    """V-cycle preconditioner (src/amg.cpp:90-132): one application at level 0
+    and the full solve."""
+    s, o, _ = make(case, common=dict(cycle=0))
+    eta = cvec(s.size(0), 91)
+    assert rel(s.kcycle(eta), o.vcycle(0, eta)) < 1e-9
+    b = cvec(s.size(0), 92)
+    x, it, rr, _ = s.solve(b)
+    xo, ito, _, _ = o.solve(b)
+    assert abs(it - ito) <= 1, (it, ito)
+    assert rr < 1e-10
+    assert rel(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("cycle", [1, 0])
+@pytest.mark.parametrize("case", list(CASES))
+def test_bootstrap(case, cycle):
+    """Bootstrap set-up with Nit = 2 (src/amg.cpp:52-84): test vectors, coarse
+    links and the solve."""
+    s, o, rows = make(case, common=dict(cycle=cycle, nit=2))
+    L = len(rows) + 1
+    for l in range(L - 1):
+        for i in range(rows[l][3]):
+            assert rel(s.tv(l, i), o.tv(l, i)) < 1e-9, (l, i)
+    for l in range(1, L):
+        assert rel(s.links(l, 0), o.links(l, 0)) < 1e-9, l
+        assert rel(s.links(l, 1, 1), o.links(l, 1, 1)) < 1e-9, l
+    b = cvec(s.size(0), 93)
+    x, it, rr, _ = s.solve(b)
+    xo, ito, _, _ = o.solve(b)
+    assert abs(it - ito) <= 1, (it, ito)
+    assert rr < 1e-10
+    assert rel(x, xo) < 1e-9
