@@ -56,6 +56,11 @@ def build(arch="gfx950"):
     subprocess.run(["make", "-s", "-C", _HERE, f"ARCH={arch}", "-B"], check=True)
 
 
+def cli_path():
+    """Path of the amgs_solve command-line driver (built with the library)."""
+    return os.path.join(_HERE, "amgs_solve")
+
+
 _lib = None
 
 # exported symbols and their argument types (include/amgs.h)
@@ -91,6 +96,7 @@ SIGNATURES = {
     "amgs_write_residuals": ([ctypes.c_char_p, _P, ctypes.c_int], ctypes.c_int),
     "amgs_time_kernel": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "amgs_last_timings": ([_V, _P], ctypes.c_int),
+    "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
 }
@@ -251,6 +257,19 @@ class Solver:
 
     def kcycle(self, eta, nrhs=1):
         return self._apply(lib().amgs_kcycle, self.size(0), self.size(0), eta, nrhs, nrhs=nrhs)
+
+    def multigrid(self, b, x0=None, max_cycles=100, tol=1e-10):
+        """Stand-alone multigrid (src/amg.cpp:181-231); returns x, cycles, relres."""
+        b = _c(b)
+        single = b.ndim == 1
+        b = b.reshape(-1, self.size(0))
+        nrhs = b.shape[0]
+        x = np.zeros_like(b) if x0 is None else _c(x0).reshape(nrhs, -1).copy()
+        cyc = np.zeros(nrhs, np.int32)
+        rr = np.zeros(nrhs)
+        rc = lib().amgs_multigrid(self.h, nrhs, _p(b), _p(x), max_cycles, tol, cyc.ctypes.data_as(_I), _p(rr))
+        self._chk(rc, "amgs_multigrid", ok=(AMGS_OK, AMGS_NOT_CONVERGED))
+        return (x[0], int(cyc[0]), float(rr[0])) if single else (x, cyc, rr)
 
     def tv(self, level, i):
         out = np.zeros(self.size(level), np.complex128)

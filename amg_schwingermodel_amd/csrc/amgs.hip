@@ -157,7 +157,7 @@ template <typename R> struct Engine {
 
     void init(Ctx* ctx, int capacity);
     void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
-    void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi);
+    void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi, bool from_zero = true);
     void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
                 int nu = 0);
     void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
@@ -501,15 +501,15 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
 
 // One application psi = K(eta) at level 0 from psi = 0 (include/amg.h:134-144,
 // src/amg.cpp:135-178); with P.cycle == 0 the V-cycle (src/amg.cpp:90-132).
-template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi) {
+template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi, bool from_zero) {
     const Geo& g = c->geo[0];
     const size_t n0 = c->nvec(0), n1 = c->nvec(1);
     hipStream_t s = c->stream;
     const unsigned ntile = cdiv_u(g.ns, FT);
-    hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(nrhs * n0, 256)), dim3(256), 0, s, nrhs * n0, psi);
+    if (from_zero) hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(nrhs * n0, 256)), dim3(256), 0, s, nrhs * n0, psi);
     const cplx<R>* rr = eta;  // r = eta - D psi = eta for psi = 0
-    if (c->P.nu1 > 0) {
-        sap0(nrhs, eta, psi, c->P.nu1);
+    if (c->P.nu1 > 0 || !from_zero) {
+        if (c->P.nu1 > 0) sap0(nrhs, eta, psi, c->P.nu1);
         hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, R(c->P.m0 + 2.0), U, eta, psi,
                            res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr);
         rr = res0.as<cplx<R>>();
@@ -1141,6 +1141,70 @@ extern "C" int amgs_kcycle(amgs_ctx* ctx, int nrhs, const double* eta, double* p
     need(nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad nrhs");
     const size_t n = c.nvec(0) * nrhs;
     return host_wrap(ctx, n, eta, n, psi, [&](c64* a, c64* b) { c.e64.kcycle0(nrhs, a, b); });
+    API_END
+}
+
+// AlgebraicMG::applyMultilevel (src/amg.cpp:181-231): x <- cycle(b, x) until
+// |b - D x| < tol |b| or max_cycles, in FP64; x is the initial guess on entry.
+// Each right-hand side keeps the iterate of the cycle at which it converged.
+extern "C" int amgs_multigrid(amgs_ctx* ctx, int nrhs, const double* b, double* x, int max_cycles, double tol,
+                              int* cycles, double* relres) {
+    if (!ctx || !b || !x || !cycles || !relres || max_cycles < 1) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(nrhs >= 1 && nrhs <= c.e64.cap && nrhs <= c.ocap, AMGS_ERR_ARG, "bad nrhs");
+    const Geo& g = c.geo[0];
+    const size_t n = c.nvec(0), tot = n * nrhs;
+    const unsigned ntile = cdiv_u(g.ns, FT);
+    c64* db = c.ob.as<c64>();
+    c64* dx = c.ox.as<c64>();
+    c64* dr = c.or_.as<c64>();
+    c64* keep = c.ow.as<c64>();
+    HIP_CHECK(hipMemcpyAsync(db, b, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(dx, x, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    std::vector<double> pr((size_t)nrhs * ntile), pv((size_t)nrhs * ntile);
+    DevBuf part;
+    part.alloc(sizeof(double) * 2 * nrhs * ntile);
+    double* dpr = part.as<double>();
+    double* dpv = dpr + (size_t)nrhs * ntile;
+    hipLaunchKernelGGL(k_norm_part<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g.ns, (const c64*)db, dpv);
+    HIP_CHECK(hipMemcpyAsync(pv.data(), dpv, sizeof(double) * nrhs * ntile, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::vector<double> nb(nrhs, 0.0);
+    for (int r = 0; r < nrhs; r++)
+        for (unsigned t = 0; t < ntile; t++) nb[r] += pv[(size_t)r * ntile + t];
+    for (int r = 0; r < nrhs; r++) {
+        nb[r] = std::sqrt(nb[r]);
+        cycles[r] = 0;
+        relres[r] = 1.0;
+    }
+    int left = nrhs;
+    for (int it = 0; it < max_cycles && left > 0; it++) {
+        c.e64.kcycle0(nrhs, db, dx, false);
+        hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
+                           (const c64*)c.U64.as<c64>(), (const c64*)db, (const c64*)dx, dr, dpr, (const int*)nullptr);
+        HIP_CHECK(hipMemcpyAsync(pr.data(), dpr, sizeof(double) * nrhs * ntile, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (int r = 0; r < nrhs; r++) {
+            if (cycles[r] > 0) continue;
+            double e = 0;
+            for (unsigned t = 0; t < ntile; t++) e += pr[(size_t)r * ntile + t];
+            e = std::sqrt(e);
+            relres[r] = nb[r] > 0 ? e / nb[r] : 0.0;
+            if (e < tol * nb[r] || it + 1 == max_cycles) {
+                cycles[r] = e < tol * nb[r] ? it + 1 : -max_cycles;
+                HIP_CHECK(hipMemcpyAsync(keep + (size_t)r * n, dx + (size_t)r * n, sizeof(c64) * n,
+                                         hipMemcpyDeviceToDevice, c.stream));
+                left--;
+            }
+        }
+    }
+    HIP_CHECK(hipMemcpyAsync(x, keep, sizeof(c64) * tot, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int r = 0; r < nrhs; r++)
+        if (cycles[r] < 0) return AMGS_NOT_CONVERGED;
+    return AMGS_OK;
     API_END
 }
 

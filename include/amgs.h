@@ -125,6 +125,13 @@ int amgs_sap(amgs_ctx* c, int level, int nrhs, const double* v, double* x, int n
  * the K-cycle (src/amg.cpp:135-178), or the V-cycle (src/amg.cpp:90-132) when
  * params.cycle == 0. */
 int amgs_kcycle(amgs_ctx* c, int nrhs, const double* eta, double* psi);
+/* AlgebraicMG::applyMultilevel(it, rhs, out, tol) (src/amg.cpp:181-231): the
+ * stand-alone multigrid x <- cycle(b, x) (K or V per params.cycle, FP64) until
+ * |b - D x| < tol |b| or max_cycles.  x holds the initial guess on entry.
+ * cycles[r] = cycles used, or -max_cycles if right-hand side r did not
+ * converge (then AMGS_NOT_CONVERGED); relres[r] = its relative residual. */
+int amgs_multigrid(amgs_ctx* c, int nrhs, const double* b, double* x, int max_cycles, double tol, int* cycles,
+                   double* relres);
 /* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
  * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as
  * [nsites_l][dof_l][dof_l] (row = 2c+alpha, col = 2b+beta). */

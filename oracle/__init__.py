@@ -68,6 +68,7 @@ def lib():
         L.or_get_links.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
         L.or_kcycle0.argtypes = [vp, P, P]
         L.or_vcycle.argtypes = [vp, ctypes.c_int, P, P]
+        L.or_multigrid.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.or_solve.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double)]
         L.or_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
@@ -174,6 +175,14 @@ class Oracle:
 
     def kcycle0(self, eta):
         return self._un(lib().or_kcycle0, eta, self.size(0))
+
+    def multigrid(self, b, x0=None, it=100, tol=1e-10):
+        """Stand-alone multigrid; returns x, cycles (-it if not converged), relres."""
+        b = _c(b).reshape(-1)
+        x = np.zeros_like(b) if x0 is None else _c(x0).reshape(-1).copy()
+        rr = ctypes.c_double(0)
+        cyc = lib().or_multigrid(self.h, _p(b), _p(x), it, tol, ctypes.byref(rr))
+        return x, cyc, rr.value
 
     def vcycle(self, l, eta, psi=None):
         """V-cycle at level l from psi (zero when omitted)."""

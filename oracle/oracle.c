@@ -831,6 +831,32 @@ void or_kcycle0(or_ctx* c, const double* eta, double* psi) {
     kcycle(c, 0, (const cx*)eta, (cx*)psi);
 }
 
+/* AlgebraicMG::applyMultilevel (src/amg.cpp:181-231): x <- cycle(0, b, x),
+ * residual with the level-0 stencil, stop when |r| < tol |b|.  Returns the
+ * cycles used, or -it when it did not converge. */
+int or_multigrid(or_ctx* c, const double* bd, double* xd, int it, double tol, double* relres) {
+    const cx* b = (const cx*)bd;
+    cx* x = (cx*)xd;
+    const size_t n = (size_t)c->lv[0].ns * 2;
+    cx* Dx = xcalloc(n, sizeof(cx));
+    const double nb = vnorm(b, (int)n);
+    double err = 0;
+    int ret = -it;
+    for (int i = 0; i < it; i++) {
+        cycle(c, 0, b, x);
+        dop(c, 0, x, Dx);
+        for (size_t q = 0; q < n; q++) Dx[q] = b[q] - Dx[q];
+        err = vnorm(Dx, (int)n);
+        if (err < tol * nb) {
+            ret = i + 1;
+            break;
+        }
+    }
+    if (relres) *relres = nb > 0 ? err / nb : 0.0;
+    free(Dx);
+    return ret;
+}
+
 int or_solve(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, int* nhist,
              double* true_relres) {
     size_t n = (size_t)c->lv[0].ns * 2;

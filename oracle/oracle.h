@@ -82,6 +82,9 @@ void or_get_links(or_ctx* c, int l, int which, int mu, double* out);
 
 /* One K-cycle application psi = K(eta) at level 0 from psi = 0 (include/amg.h:134-144). */
 void or_kcycle0(or_ctx* c, const double* eta, double* psi);
+/* Stand-alone multigrid (src/amg.cpp:181-231) from the x given; returns the
+ * cycles used or -it. */
+int or_multigrid(or_ctx* c, const double* b, double* x, int it, double tol, double* relres);
 /* V-cycle (src/amg.cpp:90-132) at level l from psi as given. */
 void or_vcycle(or_ctx* c, int l, const double* eta, double* psi);
 

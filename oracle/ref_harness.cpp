@@ -169,11 +169,30 @@ static int mode_solve(int argc, char** argv) {
     return 0;
 }
 
+// applyMultilevel(it, b, x, tol) from x0 (src/amg.cpp:181-231); writes x.bin
+static int mode_multigrid(int argc, char** argv) {
+    if (argc < 11) return 1;
+    double m0 = atof(argv[3]);
+    common_init(argv[2], m0, atoi(argv[4]));
+    GaugeConf G = load_conf(argv[5]);
+    spinor b = to_spinor(read_bin(argv[6], (size_t)LV::Ntot * 2), LV::Ntot, 2);
+    spinor x = to_spinor(read_bin(argv[7], (size_t)LV::Ntot * 2), LV::Ntot, 2);
+    std::string od = argv[8];
+    const int it = atoi(argv[9]);
+    const double tol = atof(argv[10]);
+    AlgebraicMG amg(G, m0, AMGV::nu1, AMGV::nu2);
+    amg.setUpPhase(AMGV::Nit);
+    amg.applyMultilevel(it, b, x, tol, false);
+    write_spinor(od + "/x.bin", x);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     MPI_Init(&argc, &argv);
     int rc = 1;
     if (argc > 1 && std::string(argv[1]) == "setup") rc = mode_setup(argc, argv);
     else if (argc > 1 && std::string(argv[1]) == "solve") rc = mode_solve(argc, argv);
+    else if (argc > 1 && std::string(argv[1]) == "multigrid") rc = mode_multigrid(argc, argv);
     if (rc == 1) fprintf(stderr, "usage: ref_harness setup|solve ...\n");
     MPI_Finalize();
     return rc;

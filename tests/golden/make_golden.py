@@ -44,6 +44,9 @@ CASES = {
     "solve_v": dict(mode="solve", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, restarts=50, cycle=0),
     "solve_boot": dict(mode="solve", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, restarts=50, cycle=1,
                        nit=2),
+    # stand-alone multigrid from a non-zero guess, a fixed number of cycles
+    "mg_k": dict(mode="multigrid", rows=ROWS_A, m0=-0.05, tv=4, field=(11, 0.5), vec=12, cycles=3, cycle=1),
+    "mg_v": dict(mode="multigrid", rows=ROWS_A, m0=-0.05, tv=4, field=(11, 0.5), vec=12, cycles=3, cycle=0),
 }
 
 NSAMP = 64
@@ -80,6 +83,9 @@ def run_case(name, cfg):
             f.write("\n".join(" ".join(str(x) for x in r) for r in rows) + "\n")
         if cfg["mode"] == "setup":
             cmd = [HARNESS, "setup", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin", "vec", "out"]
+        elif cfg["mode"] == "multigrid":
+            cmd = [HARNESS, "multigrid", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin", "vec/b.bin",
+                   "vec/x0.bin", "out", str(cfg["cycles"]), "1e-14"]
         else:
             cmd = [HARNESS, "solve", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin",
                    "vec/b.bin", "out", str(cfg["restarts"])]

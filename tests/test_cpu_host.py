@@ -4,6 +4,7 @@ forms and invariants that the golden vectors leave open (other lattice sizes,
 free field, symmetries, Galerkin identity)."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -218,3 +219,18 @@ def test_solve_matches_dense(hier):
     D = dense(o.dphi, o.size(0))
     assert rr < 1e-10 and it > 0
     np.testing.assert_allclose(x, np.linalg.solve(D, b), atol=1e-8)
+
+
+# ------------------------------------------------------------------- CLI ----
+def test_cli_argument_errors(tmp_path):
+    """The driver rejects incomplete command lines and a missing
+    parameters.dat before touching the GPU."""
+    exe = amgs.cli_path()
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage: amgs_solve" in r.stderr
+    r = subprocess.run([exe, "--nx", "8", "--nt", "8", "--cycle", "W"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--cycle K or V" in r.stderr
+    r = subprocess.run([exe, "--nx", "8", "--nt", "8", "--params", str(tmp_path / "none.dat")], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "cannot read" in r.stderr

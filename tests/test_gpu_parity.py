@@ -6,6 +6,9 @@ within 1e-9 relative.  Set-up results (test vectors, coarse links) carry the
 rounding of many reductions in a different order, so they are compared at
 1e-9 relative.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -175,3 +178,52 @@ def test_bootstrap(case, cycle):
     assert abs(it - ito) <= 1, (it, ito)
     assert rr < 1e-10
     assert rel(x, xo) < 1e-9
+
+
+def test_cli_matches_library(tmp_path):
+    """amgs_solve reads parameters.dat, the conf and rhs files, prints the
+    reference's convergence lines and writes the residual file and x."""
+    nx, nt, rows, m0, beta = CASES["32_3lvl"]
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
+    b = cvec(nx * nt * 2, 95)
+    L = amgs.lib()
+    assert L.amgs_write_conf(str(tmp_path / "U.ctxt").encode(), nx, nt, amgs._p(U)) == 0
+    assert L.amgs_write_rhs(str(tmp_path / "b.rhs").encode(), nx, nt, amgs._p(b)) == 0
+    (tmp_path / "parameters.dat").write_text("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
+    r = subprocess.run([amgs.cli_path(), "--nx", str(nx), "--nt", str(nt), "--params", "parameters.dat", "--conf",
+                        "U.ctxt", "--rhs", "b.rhs", "--m0", repr(m0), "--x-out", "x.rhs"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    its = [int(l.split()[1]) for l in lines if l.startswith("With ") and l.endswith(" iterations")]
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0), U)
+    s.setup()
+    x, it, rr, hist = s.solve(b)
+    assert its == [it]
+    assert any(l.startswith("FGMRES converged in") for l in lines)
+    xf = np.zeros_like(b)
+    assert L.amgs_read_rhs(str(tmp_path / "x.rhs").encode(), nx, nt, amgs._p(xf)) == 0
+    assert rel(xf, x) < 1e-12
+    res = np.loadtxt(tmp_path / f"Residuals_FGMRES_{nx}x{nt}_Levels3_Kcycle.res", ndmin=2)
+    np.testing.assert_allclose(res[:, 1], hist, rtol=1e-12)
+
+
+@pytest.mark.parametrize("cycle", [1, 0])
+def test_standalone_multigrid(cycle):
+    """Stand-alone multigrid (src/amg.cpp:181-231) from a non-zero guess:
+    fixed cycle count against the oracle, then convergence to 1e-10."""
+    s, o, _ = make("32_3lvl", common=dict(cycle=cycle), max_rhs=2)
+    b = cvec(s.size(0), 96)
+    x0 = cvec(s.size(0), 97)
+    x, cyc, rr = s.multigrid(b, x0, max_cycles=3, tol=1e-14)
+    xo, cyco, rro = o.multigrid(b, x0, it=3, tol=1e-14)
+    assert cyc == cyco == -3
+    assert rel(x, xo) < 1e-9
+    assert abs(rr - rro) <= 1e-6 * rro
+    B = np.stack([b, cvec(s.size(0), 98)])
+    X, cycs, rrs = s.multigrid(B, max_cycles=200, tol=1e-10)
+    for i in range(2):
+        xo, cyco, _ = o.multigrid(B[i], it=200, tol=1e-10)
+        assert cyco > 0 and abs(int(cycs[i]) - cyco) <= 1, (cycs[i], cyco)
+        assert rrs[i] < 1e-10
+        assert rel(X[i], xo) < 1e-8

@@ -106,3 +106,15 @@ def test_full_solve(case):
     if "residuals" in z.files:  # the reference writes them only on convergence (src/fgmres.cpp:65-79)
         np.testing.assert_array_equal(hist, z["residuals"])
     check(z, "x", x)
+
+
+@pytest.mark.parametrize("case", ["mg_k", "mg_v"])
+def test_standalone_multigrid(case):
+    """AlgebraicMG::applyMultilevel (src/amg.cpp:181-231): three K- or
+    V-cycles from a non-zero initial guess."""
+    z, meta = load(case)
+    o, v, rows = build(meta, cycle=meta["cycle"])
+    o.setup(draw_order=1)
+    x, cyc, _ = o.multigrid(v["b"], v["x0"], it=meta["cycles"], tol=1e-14)
+    assert cyc == -meta["cycles"]
+    check(z, "x", x)
