@@ -96,6 +96,11 @@ SIGNATURES = {
     "amgs_write_residuals": ([ctypes.c_char_p, _P, ctypes.c_int], ctypes.c_int),
     "amgs_time_kernel": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "amgs_last_timings": ([_V, _P], ctypes.c_int),
+    "amgs_bicgstab": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P, _P, ctypes.c_int, _I],
+                      ctypes.c_int),
+    "amgs_cg_normal": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
+    "amgs_fgmres_sap": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
+    "amgs_gmres": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
     "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
@@ -257,6 +262,40 @@ class Solver:
 
     def kcycle(self, eta, nrhs=1):
         return self._apply(lib().amgs_kcycle, self.size(0), self.size(0), eta, nrhs, nrhs=nrhs)
+
+    def _krylov(self, fn, b, *args, hist_cap=0, extra_hist=True):
+        b = _c(b)
+        single = b.ndim == 1
+        b = b.reshape(-1, self.size(0))
+        nrhs = b.shape[0]
+        x = np.zeros_like(b)
+        it = np.zeros(nrhs, np.int32)
+        rr = np.zeros(nrhs)
+        hist = np.zeros((nrhs, max(hist_cap, 1)))
+        nh = np.zeros(nrhs, np.int32)
+        tail = (_p(hist), hist_cap, nh.ctypes.data_as(_I)) if extra_hist else ()
+        rc = fn(self.h, nrhs, _p(b), _p(x), *args, it.ctypes.data_as(_I), _p(rr), *tail)
+        self._chk(rc, fn.__name__, ok=(AMGS_OK, AMGS_NOT_CONVERGED))
+        hists = [hist[r, : nh[r]].copy() for r in range(nrhs)]
+        if single:
+            return x[0], int(it[0]), float(rr[0]), hists[0]
+        return x, it, rr, hists
+
+    def bicgstab(self, b, max_iter=10000, tol=1e-10, hist_cap=10000):
+        """Bi-CGstab on D from x0 = 0 (src/bi_cgstab.cpp:5-87): x, iterations, relres, |s| history."""
+        return self._krylov(lib().amgs_bicgstab, b, max_iter, tol, hist_cap=hist_cap)
+
+    def cg_normal(self, phi, max_iter=100000, tol=1e-10):
+        """CG on D D^dagger from x0 = phi (src/conjugate_gradient.cpp:5-65): x, iterations, relres."""
+        return self._krylov(lib().amgs_cg_normal, phi, max_iter, tol, extra_hist=False)[:3]
+
+    def fgmres_sap(self, b, hist_cap=1000):
+        """FGMRES with one SAP sweep as preconditioner (include/sap.h:194-225)."""
+        return self._krylov(lib().amgs_fgmres_sap, b, hist_cap=hist_cap)
+
+    def gmres(self, b, hist_cap=1000):
+        """GMRES without preconditioning (include/fgmres.h:126-148)."""
+        return self._krylov(lib().amgs_gmres, b, hist_cap=hist_cap)
 
     def multigrid(self, b, x0=None, max_cycles=100, tol=1e-10):
         """Stand-alone multigrid (src/amg.cpp:181-231); returns x, cycles, relres."""

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +28,7 @@
 #include "common.hpp"
 #include "fine.hpp"
 #include "host_util.hpp"
+#include "krylov.hpp"
 #include "outer.hpp"
 #include "setup.hpp"
 
@@ -182,6 +184,9 @@ struct Ctx {
     int ocap = 0;
     // io staging
     DevBuf io_a, io_b;
+    // comparison solvers: Krylov vectors, partial sums
+    DevBuf kbuf, kpart, ksum;
+    int outer_prec = 0;  // outer FGMRES preconditioner: 0 multigrid cycle, 1 one SAP sweep, 2 none
     double last_ms[3] = {0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     KTimer kt[KT_CLASSES];
@@ -704,6 +709,16 @@ static void outer_alloc(Ctx& c, int cap) {
 // preconditioner Z_j = K(V_j) for all rhs of the batch
 static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
+    if (c.outer_prec == 1) {
+        // FGMRES_SAP::preconditioner (include/sap.h:212-224): zero, one SAP sweep
+        hipLaunchKernelGGL(k_zero<double>, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, zj);
+        c.e64.sap0(nrhs, vj, zj, 1);
+        return;
+    }
+    if (c.outer_prec == 2) {  // FGMRES_fine_level::preconditioner: identity (include/fgmres.h:144-147)
+        HIP_CHECK(hipMemcpyAsync(zj, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        return;
+    }
     if (c.P.precision == AMGS_PREC_MIXED) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                            c.oeta32.as<c32>());
@@ -965,7 +980,8 @@ extern "C" int amgs_setup_from_tv(amgs_ctx* ctx, const double* const* tv, int sm
 }
 
 static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist, int hist_cap, int* nhist) {
-    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(c.setup_done || c.outer_prec != 0, AMGS_ERR_STATE, "amgs_setup first");
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
     auto t0 = std::chrono::steady_clock::now();
     outer_solve(c, nrhs);
@@ -988,10 +1004,9 @@ static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist
     std::vector<double> rr(nrhs);
     true_relres(c, nrhs, rr.data());
     if (relres) std::copy(rr.begin(), rr.end(), relres);
+    // convergence is judged on the true residual (an iteration count of
+    // restarts*m is also what a solve converging in the last step returns)
     bool all = true;
-    for (int r = 0; r < nrhs; r++) all = all && it[r] < c.P.out_restarts * c.P.out_m;
-    // an exact solve of iteration count restarts*m cannot be told apart; use the residual
-    all = true;
     for (int r = 0; r < nrhs; r++) all = all && rr[r] < c.P.out_tol * 1.0000001;
     return all ? AMGS_OK : AMGS_NOT_CONVERGED;
 }
@@ -1022,6 +1037,284 @@ extern "C" int amgs_solve_dev(amgs_ctx* ctx, int nrhs, const void* b_dev, void* 
     HIP_CHECK(hipMemcpyAsync(x_dev, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToDevice, c.stream));
     c.sync();
     return rc;
+    API_END
+}
+
+// ---- comparison solvers on the fine lattice (FP64) ----
+using zc = std::complex<double>;
+
+struct KryRun {
+    Ctx& c;
+    int nrhs;
+    unsigned ntile;
+    KCoef k{};
+    std::vector<c64> sums;
+    KryRun(Ctx& cc, int n) : c(cc), nrhs(n), ntile(cdiv_u(cc.geo[0].ns, FT)), sums(2 * n) {
+        c.kpart.alloc(sizeof(c64) * 2 * ntile * nrhs);
+        c.ksum.alloc(sizeof(c64) * 2 * nrhs);
+        k.act = nrhs >= 32 ? 0xffffffffu : ((1u << nrhs) - 1u);
+    }
+    c64* part() { return c.kpart.as<c64>(); }
+    void vec(int op, c64* v0, c64* v1 = nullptr, c64* v2 = nullptr, c64* v3 = nullptr, c64* v4 = nullptr,
+             c64* v5 = nullptr, bool dots = false) {
+        hipLaunchKernelGGL(k_vec, dim3(ntile, nrhs), dim3(FT), 0, c.stream, c.geo[0].ns, op, k, v0, v1, v2, v3, v4, v5,
+                           dots ? part() : (c64*)nullptr);
+    }
+    template <bool DAG> void stencil(const c64* in, c64* out, const c64* y, int mode) {
+        hipLaunchKernelGGL(k_stencil_dots<DAG>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, c.geo[0], c.P.m0 + 2.0,
+                           (const c64*)c.U64.as<c64>(), in, out, y, part(), mode, k.act);
+    }
+    // per-rhs sums of the last partials (host synchronises)
+    void fold() {
+        hipLaunchKernelGGL(k_sum_parts, dim3(nrhs), dim3(FT), 0, c.stream, (const c64*)part(), (int)ntile,
+                           c.ksum.as<c64>());
+        HIP_CHECK(hipMemcpyAsync(sums.data(), c.ksum.p, sizeof(c64) * 2 * nrhs, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    }
+    zc q(int r, int i) const { return zc(sums[2 * r + i].re, sums[2 * r + i].im); }
+    bool on(int r) const { return (k.act >> r) & 1u; }
+};
+
+static c64 to_c(zc z) { return c64{z.real(), z.imag()}; }
+
+// Bi-CGstab (src/bi_cgstab.cpp:5-87) on D_phi from x0 = 0, b in c.ob, x in
+// c.ox; iters[r] = k+1 at convergence or max_iter; hist gets |s| per iteration.
+static void bicgstab_impl(Ctx& c, int nrhs, int max_iter, double tol, int* iters, double* hist, int hcap, int* nhist) {
+    const size_t tot = c.nvec(0) * nrhs;
+    c.kbuf.alloc(sizeof(c64) * tot * 6);
+    c64* base = c.kbuf.as<c64>();
+    c64 *r = base, *rt = base + tot, *d = base + 2 * tot, *s = base + 3 * tot, *t = base + 4 * tot, *Ad = base + 5 * tot;
+    c64* x = c.ox.as<c64>();
+    c64* b = c.ob.as<c64>();
+    KryRun K(c, nrhs);
+    HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * tot, c.stream));
+    HIP_CHECK(hipMemcpyAsync(r, b, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));  // r = b - D 0
+    HIP_CHECK(hipMemcpyAsync(rt, b, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    K.vec(KOP_DOT, b, b, nullptr, nullptr, nullptr, nullptr, true);
+    K.fold();
+    std::vector<double> nb(nrhs);
+    std::vector<zc> alpha(nrhs), omega(nrhs), rho(nrhs), rho_prev(nrhs);
+    for (int q = 0; q < nrhs; q++) {
+        nb[q] = std::sqrt(K.q(q, 0).real());
+        iters[q] = max_iter;
+        if (nhist) nhist[q] = 0;
+    }
+    K.vec(KOP_DOT, r, rt, nullptr, nullptr, nullptr, nullptr, true);  // rho_0 = <r, rt>
+    K.fold();
+    for (int q = 0; q < nrhs; q++) rho[q] = K.q(q, 0);
+    for (int k = 0; k < max_iter && K.k.act; k++) {
+        if (k == 0) {
+            HIP_CHECK(hipMemcpyAsync(d, r, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        } else {
+            for (int q = 0; q < nrhs; q++)
+                if (K.on(q)) {
+                    K.k.b[q] = to_c(alpha[q] * rho[q] / (omega[q] * rho_prev[q]));
+                    K.k.c[q] = to_c(omega[q]);
+                }
+            K.vec(KOP_DIR, d, r, Ad);
+        }
+        K.stencil<false>(d, Ad, rt, 1);  // Ad = D d, <Ad, rt>
+        K.fold();
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) {
+                alpha[q] = rho[q] / K.q(q, 0);
+                K.k.a[q] = to_c(alpha[q]);
+            }
+        K.vec(KOP_S, s, r, Ad, nullptr, nullptr, nullptr, true);  // s = r - alpha Ad, |s|^2
+        K.fold();
+        unsigned conv = 0;
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) {
+                const double err = std::sqrt(K.q(q, 0).real());
+                if (hist && nhist && nhist[q] < hcap) hist[(size_t)q * hcap + nhist[q]++] = err;
+                if (err < tol * nb[q]) {
+                    conv |= 1u << q;
+                    iters[q] = k + 1;
+                }
+            }
+        if (conv) {  // x = x + alpha d for the right-hand sides that converged now
+            const unsigned keep = K.k.act;
+            K.k.act = conv;
+            K.vec(KOP_XD, x, d);
+            K.k.act = keep & ~conv;
+        }
+        if (!K.k.act) break;
+        K.stencil<false>(s, t, s, 2);  // t = D s, <s, t>, <t, t>
+        K.fold();
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) {
+                omega[q] = K.q(q, 0) / K.q(q, 1);
+                K.k.c[q] = to_c(omega[q]);
+            }
+        K.vec(KOP_RX, r, s, t, x, d, rt, true);  // r = s - omega t; x += alpha d + omega s; <r, rt>
+        K.fold();
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) {
+                rho_prev[q] = rho[q];
+                rho[q] = K.q(q, 0);
+            }
+    }
+    c.sync();
+}
+
+// CG on D D^dagger (src/conjugate_gradient.cpp:5-65) from x0 = phi (c.ob);
+// iters[r] = loop passes at convergence (the reference prints one less), or
+// -max_iter.
+static void cg_impl(Ctx& c, int nrhs, int max_iter, double tol, int* iters) {
+    const size_t tot = c.nvec(0) * nrhs;
+    c.kbuf.alloc(sizeof(c64) * tot * 4);
+    c64* base = c.kbuf.as<c64>();
+    c64 *r = base, *d = base + tot, *Ad = base + 2 * tot, *tmp = base + 3 * tot;
+    c64* x = c.ox.as<c64>();
+    c64* phi = c.ob.as<c64>();
+    KryRun K(c, nrhs);
+    HIP_CHECK(hipMemcpyAsync(x, phi, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    K.stencil<true>(x, tmp, nullptr, 0);
+    K.stencil<false>(tmp, Ad, nullptr, 0);
+    for (int q = 0; q < nrhs; q++) K.k.a[q] = c64{1.0, 0.0};
+    HIP_CHECK(hipMemcpyAsync(r, phi, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    // r = phi - Ad (as x += 0*d; r -= 1*Ad) and |r|^2
+    K.vec(KOP_CGXR, tmp, tmp, r, Ad, nullptr, nullptr, true);
+    K.fold();
+    HIP_CHECK(hipMemcpyAsync(d, r, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    std::vector<zc> rn(nrhs);
+    for (int q = 0; q < nrhs; q++) rn[q] = K.q(q, 0);
+    K.vec(KOP_DOT, phi, phi, nullptr, nullptr, nullptr, nullptr, true);
+    K.fold();
+    std::vector<double> np(nrhs);
+    for (int q = 0; q < nrhs; q++) {
+        np[q] = std::sqrt(K.q(q, 0).real());
+        iters[q] = -max_iter;
+    }
+    for (int k = 0; k < max_iter && K.k.act; k++) {
+        K.stencil<true>(d, tmp, nullptr, 0);
+        K.stencil<false>(tmp, Ad, d, 2);  // Ad = D D^dagger d, <d, Ad>
+        K.fold();
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) K.k.a[q] = to_c(rn[q] / K.q(q, 0));
+        K.vec(KOP_CGXR, x, d, r, Ad, nullptr, nullptr, true);  // x += a d; r -= a Ad; |r|^2
+        K.fold();
+        unsigned conv = 0;
+        for (int q = 0; q < nrhs; q++)
+            if (K.on(q)) {
+                const double e2 = K.q(q, 0).real();
+                if (std::sqrt(e2) < tol * np[q]) {
+                    conv |= 1u << q;
+                    iters[q] = k + 1;
+                } else {
+                    K.k.b[q] = to_c(zc(e2, 0.0) / rn[q]);
+                    rn[q] = zc(e2, 0.0);
+                }
+            }
+        K.k.act &= ~conv;
+        if (!K.k.act) break;
+        K.vec(KOP_CGD, d, r);  // d = beta d + r
+    }
+    c.sync();
+}
+
+static void krylov_prologue(Ctx& c, int nrhs) {
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    need(nrhs >= 1 && nrhs <= std::min(c.P.max_rhs, KMAXB), AMGS_ERR_ARG, "nrhs must be in [1, min(max_rhs, 32)]");
+}
+
+// relative residual |b - A x| / |b| with A = D (op 0) or D D^dagger (op 1)
+static void krylov_relres(Ctx& c, int nrhs, int op, double* relres) {
+    const size_t tot = c.nvec(0) * nrhs;
+    c.kbuf.alloc(sizeof(c64) * tot * 3);
+    c64* t1 = c.kbuf.as<c64>();
+    c64* t2 = t1 + tot;
+    c64* t3 = t2 + tot;
+    KryRun K(c, nrhs);
+    if (op == 1) {
+        K.stencil<true>(c.ox.as<c64>(), t1, nullptr, 0);
+        K.stencil<false>(t1, t2, nullptr, 0);
+    } else {
+        K.stencil<false>(c.ox.as<c64>(), t2, nullptr, 0);
+    }
+    for (int q = 0; q < nrhs; q++) K.k.a[q] = c64{1.0, 0.0};
+    HIP_CHECK(hipMemcpyAsync(t1, c.ob.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    K.vec(KOP_CGXR, t3, t3, t1, t2, nullptr, nullptr, true);  // t1 = b - A x, |t1|^2 (t3 is scratch)
+    K.fold();
+    std::vector<double> e(nrhs);
+    for (int q = 0; q < nrhs; q++) e[q] = K.q(q, 0).real();
+    K.vec(KOP_DOT, c.ob.as<c64>(), c.ob.as<c64>(), nullptr, nullptr, nullptr, nullptr, true);
+    K.fold();
+    for (int q = 0; q < nrhs; q++) {
+        const double nb = K.q(q, 0).real();
+        relres[q] = nb > 0 ? std::sqrt(e[q] / nb) : 0.0;
+    }
+}
+
+extern "C" int amgs_bicgstab(amgs_ctx* ctx, int nrhs, const double* b, double* x, int max_iter, double tol,
+                             int* iters, double* relres, double* hist, int hist_cap, int* nhist) {
+    if (!ctx || !b || !x || !iters || !relres || max_iter < 1) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    krylov_prologue(c, nrhs);
+    const size_t tot = c.nvec(0) * nrhs;
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, b, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    bicgstab_impl(c, nrhs, max_iter, tol, iters, hist_cap > 0 ? hist : nullptr, hist_cap, nhist);
+    krylov_relres(c, nrhs, 0, relres);
+    HIP_CHECK(hipMemcpy(x, c.ox.p, sizeof(c64) * tot, hipMemcpyDeviceToHost));
+    for (int q = 0; q < nrhs; q++)
+        if (!(relres[q] < tol * 1.0000001)) return AMGS_NOT_CONVERGED;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_cg_normal(amgs_ctx* ctx, int nrhs, const double* phi, double* x, int max_iter, double tol,
+                              int* iters, double* relres) {
+    if (!ctx || !phi || !x || !iters || !relres || max_iter < 1) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    krylov_prologue(c, nrhs);
+    const size_t tot = c.nvec(0) * nrhs;
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, phi, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    cg_impl(c, nrhs, max_iter, tol, iters);
+    krylov_relres(c, nrhs, 1, relres);
+    HIP_CHECK(hipMemcpy(x, c.ox.p, sizeof(c64) * tot, hipMemcpyDeviceToHost));
+    for (int q = 0; q < nrhs; q++)
+        if (iters[q] < 0) return AMGS_NOT_CONVERGED;
+    return AMGS_OK;
+    API_END
+}
+
+// outer FGMRES with another preconditioner (1: one SAP sweep, 2: none)
+static int solve_with(amgs_ctx* ctx, int prec, int nrhs, const double* b, double* x, int* iters, double* relres,
+                      double* hist, int hist_cap, int* nhist) {
+    Ctx& c = ctx->c;
+    need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    need(nrhs >= 1 && nrhs <= c.P.max_rhs && (prec != 1 || nrhs <= c.e64.cap), AMGS_ERR_ARG, "bad nrhs");
+    const size_t n0 = c.nvec(0);
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, b, sizeof(c64) * n0 * nrhs, hipMemcpyHostToDevice, c.stream));
+    c.outer_prec = prec;
+    int rc;
+    try {
+        rc = solve_impl(c, nrhs, iters, relres, hist, hist_cap, nhist);
+    } catch (...) {
+        c.outer_prec = 0;
+        throw;
+    }
+    c.outer_prec = 0;
+    HIP_CHECK(hipMemcpy(x, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+extern "C" int amgs_fgmres_sap(amgs_ctx* ctx, int nrhs, const double* b, double* x, int* iters, double* relres,
+                               double* hist, int hist_cap, int* nhist) {
+    if (!ctx || !b || !x) return AMGS_ERR_ARG;
+    API_BEGIN
+    return solve_with(ctx, 1, nrhs, b, x, iters, relres, hist, hist_cap, nhist);
+    API_END
+}
+
+extern "C" int amgs_gmres(amgs_ctx* ctx, int nrhs, const double* b, double* x, int* iters, double* relres,
+                          double* hist, int hist_cap, int* nhist) {
+    if (!ctx || !b || !x) return AMGS_ERR_ARG;
+    API_BEGIN
+    return solve_with(ctx, 2, nrhs, b, x, iters, relres, hist, hist_cap, nhist);
     API_END
 }
 

@@ -132,6 +132,28 @@ int amgs_kcycle(amgs_ctx* c, int nrhs, const double* eta, double* psi);
  * converge (then AMGS_NOT_CONVERGED); relres[r] = its relative residual. */
 int amgs_multigrid(amgs_ctx* c, int nrhs, const double* b, double* x, int max_cycles, double tol, int* cycles,
                    double* relres);
+/* ---- comparison solvers on the fine lattice (FP64; nrhs <= min(max_rhs, 32);
+ *      they need amgs_set_gauge only, no set-up) ---- */
+/* bi_cgstab(D_phi, ..., x0 = 0, max_iter, tol) (src/bi_cgstab.cpp:5-87):
+ * iters[r] = k+1 at convergence or max_iter; hist [nrhs][hist_cap] gets |s| of
+ * every iteration (the Residuals_BiCG file); relres = |b - D x| / |b|. */
+int amgs_bicgstab(amgs_ctx* c, int nrhs, const double* b, double* x, int max_iter, double tol, int* iters,
+                  double* relres, double* hist, int hist_cap, int* nhist);
+/* conjugate_gradient(U, phi, x, m0) on D D^dagger from x0 = phi
+ * (src/conjugate_gradient.cpp:5-65): iters[r] = loop passes at convergence
+ * (the reference prints this minus one) or -max_iter; relres = |phi - D D^dagger x| / |phi|. */
+int amgs_cg_normal(amgs_ctx* c, int nrhs, const double* phi, double* x, int max_iter, double tol, int* iters,
+                   double* relres);
+/* FGMRES_SAP::fgmres(b, 0, x) (include/sap.h:194-225): outer FGMRES(out_m,
+ * out_restarts, out_tol) with one SAP sweep (level-0 Schwarz blocks) as
+ * preconditioner.  Same outputs as amgs_solve. */
+int amgs_fgmres_sap(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, double* relres, double* hist,
+                    int hist_cap, int* nhist);
+/* FGMRES_fine_level::fgmres(b, 0, x) (include/fgmres.h:126-148): GMRES on D
+ * without preconditioning (out_m, out_restarts, out_tol). */
+int amgs_gmres(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, double* relres, double* hist,
+               int hist_cap, int* nhist);
+
 /* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
  * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as
  * [nsites_l][dof_l][dof_l] (row = 2c+alpha, col = 2b+beta). */

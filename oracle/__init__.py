@@ -68,6 +68,10 @@ def lib():
         L.or_get_links.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
         L.or_kcycle0.argtypes = [vp, P, P]
         L.or_vcycle.argtypes = [vp, ctypes.c_int, P, P]
+        L.or_bicgstab.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.or_cg.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double]
+        L.or_fgmres_sap.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.or_multigrid.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.or_solve.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double)]
@@ -175,6 +179,28 @@ class Oracle:
 
     def kcycle0(self, eta):
         return self._un(lib().or_kcycle0, eta, self.size(0))
+
+    def bicgstab(self, b, max_iter=10000, tol=1e-10, hist_cap=100000):
+        b = _c(b).reshape(-1)
+        x = np.zeros_like(b)
+        h = np.zeros(hist_cap)
+        nh = ctypes.c_int(0)
+        it = lib().or_bicgstab(self.h, _p(b), _p(x), max_iter, tol, _p(h), hist_cap, ctypes.byref(nh))
+        return x, it, h[: nh.value].copy()
+
+    def cg(self, phi, max_iter=100000, tol=1e-10):
+        phi = _c(phi).reshape(-1)
+        x = np.zeros_like(phi)
+        it = lib().or_cg(self.h, _p(phi), _p(x), max_iter, tol)
+        return x, it
+
+    def fgmres_sap(self, b, hist_cap=4096):
+        b = _c(b).reshape(-1)
+        x = np.zeros_like(b)
+        h = np.zeros(hist_cap)
+        nh = ctypes.c_int(0)
+        it = lib().or_fgmres_sap(self.h, _p(b), _p(x), _p(h), hist_cap, ctypes.byref(nh))
+        return x, it, h[: nh.value].copy()
 
     def multigrid(self, b, x0=None, it=100, tol=1e-10):
         """Stand-alone multigrid; returns x, cycles (-it if not converged), relres."""

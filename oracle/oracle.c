@@ -892,3 +892,112 @@ int or_gmres_fine(or_ctx* c, const double* b, double* x, int m, int restarts, do
     ws_free(&W);
     return it;
 }
+
+/* ------------------------------------------------------------------ */
+/* Comparison solvers on the fine lattice                             */
+/* ------------------------------------------------------------------ */
+
+/* Bi-CGstab on D_phi from x0 = 0 (src/bi_cgstab.cpp:5-87).  hist gets |s| of
+ * every iteration.  Returns the iterations (k+1 at convergence) or max_iter. */
+int or_bicgstab(or_ctx* c, const double* bd, double* xd, int max_iter, double tol, double* hist, int hist_cap,
+                int* nhist) {
+    const int n = c->lv[0].ns * 2;
+    const cx* b = (const cx*)bd;
+    cx* x = (cx*)xd;
+    cx *r = xcalloc(n, sizeof(cx)), *rt = xcalloc(n, sizeof(cx)), *d = xcalloc(n, sizeof(cx)),
+       *s = xcalloc(n, sizeof(cx)), *t = xcalloc(n, sizeof(cx)), *Ad = xcalloc(n, sizeof(cx));
+    memset(x, 0, sizeof(cx) * n);
+    dphi(c, x, Ad);
+    for (int i = 0; i < n; i++) r[i] = b[i] - Ad[i];
+    memcpy(rt, r, sizeof(cx) * n);
+    const double nb = vnorm(b, n);
+    cx alpha = 0, omega = 0, rho = 0, rho_prev = 0, beta;
+    int k = 0, ret = max_iter, nh = 0;
+    while (k < max_iter) {
+        rho = vdot(r, rt, n);
+        if (k == 0) memcpy(d, r, sizeof(cx) * n);
+        else {
+            beta = alpha * rho / (omega * rho_prev);
+            for (int i = 0; i < n; i++) d[i] = r[i] + beta * (d[i] - omega * Ad[i]);
+        }
+        dphi(c, d, Ad);
+        alpha = rho / vdot(Ad, rt, n);
+        for (int i = 0; i < n; i++) s[i] = r[i] - alpha * Ad[i];
+        const double err = vnorm(s, n);
+        if (hist && nh < hist_cap) hist[nh] = err;
+        nh++;
+        if (err < tol * nb) {
+            for (int i = 0; i < n; i++) x[i] = x[i] + alpha * d[i];
+            ret = k + 1;
+            break;
+        }
+        dphi(c, s, t);
+        omega = vdot(s, t, n) / vdot(t, t, n);
+        for (int i = 0; i < n; i++) r[i] = s[i] - omega * t[i];
+        for (int i = 0; i < n; i++) x[i] = x[i] + alpha * d[i] + omega * s[i];
+        rho_prev = rho;
+        k++;
+    }
+    if (nhist) *nhist = nh < hist_cap ? nh : hist_cap;
+    free(r); free(rt); free(d); free(s); free(t); free(Ad);
+    return ret;
+}
+
+/* Conjugate gradient for (D D^dagger) x = phi from x0 = phi
+ * (src/conjugate_gradient.cpp:5-65).  Returns the number of D D^dagger
+ * applications inside the loop (the reference prints one less) or -max_iter. */
+int or_cg(or_ctx* c, const double* pd, double* xd, int max_iter, double tol) {
+    const int n = c->lv[0].ns * 2;
+    const cx* phi = (const cx*)pd;
+    cx* x = (cx*)xd;
+    cx *r = xcalloc(n, sizeof(cx)), *d = xcalloc(n, sizeof(cx)), *Ad = xcalloc(n, sizeof(cx)),
+       *tmp = xcalloc(n, sizeof(cx));
+    memcpy(x, phi, sizeof(cx) * n);
+    ddag(c, x, tmp);
+    dphi(c, tmp, Ad);
+    for (int i = 0; i < n; i++) r[i] = phi[i] - Ad[i];
+    memcpy(d, r, sizeof(cx) * n);
+    cx rn = vdot(r, r, n);
+    const double np = vnorm(phi, n);
+    int k = 0, ret = -max_iter;
+    while (k < max_iter) {
+        ddag(c, d, tmp);
+        dphi(c, tmp, Ad);
+        const cx alpha = rn / vdot(d, Ad, n);
+        for (int i = 0; i < n; i++) x[i] += alpha * d[i];
+        for (int i = 0; i < n; i++) r[i] -= alpha * Ad[i];
+        const double e2 = creal(vdot(r, r, n));
+        if (sqrt(e2) < tol * np) {
+            ret = k + 1;
+            break;
+        }
+        const cx beta = e2 / rn;
+        for (int i = 0; i < n; i++) {
+            d[i] *= beta;
+            d[i] += r[i];
+        }
+        rn = e2;
+        k++;
+    }
+    free(r); free(d); free(Ad); free(tmp);
+    return ret;
+}
+
+/* FGMRES with one SAP sweep as preconditioner, from x0 = 0
+ * (include/sap.h:194-225): the preconditioner zeroes its output, then
+ * SAP(in, out, 1) on the fine level. */
+static void pre_sap(or_ctx* c, const void* a, const cx* in, cx* out) {
+    (void)a;
+    memset(out, 0, sizeof(cx) * (size_t)c->lv[0].ns * 2);
+    sap(c, 0, in, out, 1);
+}
+
+int or_fgmres_sap(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, int* nhist) {
+    size_t n = (size_t)c->lv[0].ns * 2;
+    fg_ws W;
+    ws_init(&W, (int)n, c->p.out_m, c->p.out_restarts, c->p.out_tol);
+    memset(c->tmp0, 0, n * sizeof(cx));
+    int it = fgmres(c, &W, op_dphi, NULL, pre_sap, NULL, (const cx*)b, c->tmp0, (cx*)x, hist, hist_cap, nhist);
+    ws_free(&W);
+    return it;
+}

@@ -85,6 +85,14 @@ void or_kcycle0(or_ctx* c, const double* eta, double* psi);
 /* Stand-alone multigrid (src/amg.cpp:181-231) from the x given; returns the
  * cycles used or -it. */
 int or_multigrid(or_ctx* c, const double* b, double* x, int it, double tol, double* relres);
+/* Comparison solvers on the fine lattice (FP64): Bi-CGstab on D from 0
+ * (src/bi_cgstab.cpp:5-87), CG on D D^dagger from x0 = phi
+ * (src/conjugate_gradient.cpp:5-65), FGMRES with one SAP sweep as
+ * preconditioner (include/sap.h:194-225; out_m/out_restarts/out_tol). */
+int or_bicgstab(or_ctx* c, const double* b, double* x, int max_iter, double tol, double* hist, int hist_cap,
+                int* nhist);
+int or_cg(or_ctx* c, const double* phi, double* x, int max_iter, double tol);
+int or_fgmres_sap(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, int* nhist);
 /* V-cycle (src/amg.cpp:90-132) at level l from psi as given. */
 void or_vcycle(or_ctx* c, int l, const double* eta, double* psi);
 

@@ -24,6 +24,10 @@
 #include "params.h"
 #include "boundary.h"
 #include "amg.h"
+#include "bi_cgstab.h"
+#include "conjugate_gradient.h"
+#include <iostream>
+#include <sstream>
 #include "mpi.h"
 #include <cstdio>
 #include <cstdlib>
@@ -187,12 +191,58 @@ static int mode_multigrid(int argc, char** argv) {
     return 0;
 }
 
+// comparison solvers on the fine lattice (src/bi_cgstab.cpp:5-87,
+// src/conjugate_gradient.cpp:5-65, include/sap.h:194-225, include/fgmres.h:126-148):
+// writes x.bin and result.txt (iterations), residual files where the
+// reference writes them
+static int mode_krylov(int argc, char** argv) {
+    if (argc < 10) return 1;
+    const std::string kind = argv[1];
+    double m0 = atof(argv[3]);
+    common_init(argv[2], m0, atoi(argv[4]));
+    GaugeConf G = load_conf(argv[5]);
+    spinor b = to_spinor(read_bin(argv[6], (size_t)LV::Ntot * 2), LV::Ntot, 2);
+    std::string od = argv[7];
+    const int p1 = atoi(argv[8]), p2 = atoi(argv[9]);
+    if (chdir(od.c_str()) != 0) return 2;
+    spinor x0(LV::Ntot, c_vector(2, 0)), x(LV::Ntot, c_vector(2, 0));
+    long iters = -1;
+    if (kind == "bicg") {
+        x = bi_cgstab(&D_phi, LV::Ntot, 2, G.Conf, b, x0, m0, p1, 1e-10, false, true);
+    } else if (kind == "cg") {
+        CG::max_iter = p1;
+        std::ostringstream cap;
+        std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+        const int ok = conjugate_gradient(G.Conf, b, x, m0);
+        std::cout.rdbuf(old);
+        std::istringstream in(cap.str());
+        std::string w1, w2;
+        in >> w1 >> w2 >> iters;  // "Converged in k iterations"
+        if (!ok) iters = -1;
+    } else if (kind == "fgmres_sap") {
+        sap.set_params(G.Conf, m0);
+        FGMRES_SAP f(LV::Ntot, 2, p1, p2, 1e-10, G.Conf, m0);
+        iters = f.fgmres(b, x0, x, false, true);
+    } else if (kind == "gmres") {
+        FGMRES_fine_level f(LV::Ntot, 2, p1, p2, 1e-10, G.Conf, m0);
+        iters = f.fgmres(b, x0, x, false, true);
+    } else {
+        return 1;
+    }
+    write_spinor("x.bin", x);
+    FILE* fo = fopen("result.txt", "w");
+    fprintf(fo, "iterations %ld\nsetup_s 0\nsolve_s 0\n", iters);
+    fclose(fo);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     MPI_Init(&argc, &argv);
     int rc = 1;
     if (argc > 1 && std::string(argv[1]) == "setup") rc = mode_setup(argc, argv);
     else if (argc > 1 && std::string(argv[1]) == "solve") rc = mode_solve(argc, argv);
     else if (argc > 1 && std::string(argv[1]) == "multigrid") rc = mode_multigrid(argc, argv);
+    else if (argc > 1) rc = mode_krylov(argc, argv);
     if (rc == 1) fprintf(stderr, "usage: ref_harness setup|solve ...\n");
     MPI_Finalize();
     return rc;

@@ -47,6 +47,11 @@ CASES = {
     # stand-alone multigrid from a non-zero guess, a fixed number of cycles
     "mg_k": dict(mode="multigrid", rows=ROWS_A, m0=-0.05, tv=4, field=(11, 0.5), vec=12, cycles=3, cycle=1),
     "mg_v": dict(mode="multigrid", rows=ROWS_A, m0=-0.05, tv=4, field=(11, 0.5), vec=12, cycles=3, cycle=0),
+    # comparison solvers on the fine lattice (p1, p2: max_iter or restart length, restarts)
+    "kry_bicg": dict(mode="bicg", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, p=(10000, 0)),
+    "kry_cg": dict(mode="cg", rows=ROWS_A, m0=0.3, tv=4, field=(11, 0.5), vec=12, p=(100000, 0)),
+    "kry_fsap": dict(mode="fgmres_sap", rows=ROWS_A, m0=0.05, tv=4, field=(11, 0.5), vec=12, p=(20, 50)),
+    "kry_gmres": dict(mode="gmres", rows=ROWS_A, m0=0.3, tv=4, field=(11, 0.5), vec=12, p=(20, 100)),
 }
 
 NSAMP = 64
@@ -83,6 +88,9 @@ def run_case(name, cfg):
             f.write("\n".join(" ".join(str(x) for x in r) for r in rows) + "\n")
         if cfg["mode"] == "setup":
             cmd = [HARNESS, "setup", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin", "vec", "out"]
+        elif cfg["mode"] in ("bicg", "cg", "fgmres_sap", "gmres"):
+            cmd = [HARNESS, cfg["mode"], "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin", "vec/b.bin", "out",
+                   str(cfg["p"][0]), str(cfg["p"][1])]
         elif cfg["mode"] == "multigrid":
             cmd = [HARNESS, "multigrid", "params.dat", repr(cfg["m0"]), str(cfg["tv"]), "U.bin", "vec/b.bin",
                    "vec/x0.bin", "out", str(cfg["cycles"]), "1e-14"]

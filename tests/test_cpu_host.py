@@ -234,3 +234,22 @@ def test_cli_argument_errors(tmp_path):
     r = subprocess.run([exe, "--nx", "8", "--nt", "8", "--params", str(tmp_path / "none.dat")], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "cannot read" in r.stderr
+
+
+def test_comparison_solvers_against_dense():
+    """Oracle Bi-CGstab, CG on D D^dagger, FGMRES+SAP and GMRES against
+    numpy's dense solves on an 8x8 lattice."""
+    U, _ = amgs.gauge_metropolis(8, 8, 2.0, 10, 3)
+    o = O.Oracle(O.default_params(8, 8, [(0, 2, 2, 2, 2, 2)], m0=0.1), U)
+    D = dense(o.dphi, 128)
+    b = np.random.default_rng(4).standard_normal(128) + 1j * np.random.default_rng(5).standard_normal(128)
+    xe = np.linalg.solve(D, b)
+    x, it, hist = o.bicgstab(b)
+    assert it == len(hist) and hist[-1] < 1e-10 * np.linalg.norm(b)
+    np.testing.assert_allclose(x, xe, atol=1e-8)
+    x, it = o.cg(b)
+    assert it > 0
+    np.testing.assert_allclose(x, np.linalg.solve(D @ D.conj().T, b), atol=1e-8)
+    for x, it in (o.fgmres_sap(b)[:2], o.gmres_fine(b)):
+        assert it > 0
+        np.testing.assert_allclose(x, xe, atol=1e-8)

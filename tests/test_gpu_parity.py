@@ -227,3 +227,34 @@ def test_standalone_multigrid(cycle):
         assert cyco > 0 and abs(int(cycs[i]) - cyco) <= 1, (cycs[i], cyco)
         assert rrs[i] < 1e-10
         assert rel(X[i], xo) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["32_3lvl", "64_2lvl"])
+def test_comparison_solvers(case):
+    """Bi-CGstab, CG on D D^dagger, FGMRES+SAP and plain GMRES on the fine
+    lattice against the oracle (no multigrid set-up needed), batched."""
+    nx, nt, rows, m0, beta = CASES[case]
+    m0 = 0.1  # well-conditioned enough for the unpreconditioned methods
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, max_rhs=2), U)
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    B = np.stack([cvec(s.size(0), 110 + i) for i in range(2)])
+    X, its, rrs, hists = s.bicgstab(B)
+    for i in range(2):
+        xo, ito, ho = o.bicgstab(B[i])
+        assert abs(int(its[i]) - ito) <= max(1, ito // 50), (its[i], ito)
+        assert rrs[i] < 1.5e-10 and rel(X[i], xo) < 1e-8
+        k = min(len(hists[i]), len(ho), 10)
+        np.testing.assert_allclose(hists[i][:k], ho[:k], rtol=1e-6)
+    X, its, rrs = s.cg_normal(B)
+    for i in range(2):
+        xo, ito = o.cg(B[i])
+        assert ito > 0 and abs(int(its[i]) - ito) <= max(1, ito // 50), (its[i], ito)
+        assert rrs[i] < 1.5e-10 and rel(X[i], xo) < 1e-8
+    for meth, ometh in ((s.fgmres_sap, o.fgmres_sap), (s.gmres, o.gmres_fine)):
+        X, its, rrs, _ = meth(B)
+        for i in range(2):
+            res = ometh(B[i])
+            xo, ito = res[0], res[1]
+            assert abs(int(its[i]) - ito) <= 1, (meth.__name__, its[i], ito)
+            assert rrs[i] < 1e-10 and rel(X[i], xo) < 1e-9

@@ -118,3 +118,46 @@ def test_standalone_multigrid(case):
     x, cyc, _ = o.multigrid(v["b"], v["x0"], it=meta["cycles"], tol=1e-14)
     assert cyc == -meta["cycles"]
     check(z, "x", x)
+
+
+def test_bicgstab():
+    """bi_cgstab (src/bi_cgstab.cpp:5-87): iterations, |s| history and x."""
+    z, meta = load("kry_bicg")
+    o, v, _ = build(meta)
+    x, it, hist = o.bicgstab(v["b"], max_iter=meta["p"][0], tol=1e-10)
+    assert it == len(z["residuals"])
+    np.testing.assert_array_equal(hist, z["residuals"])
+    check(z, "x", x)
+
+
+def test_cg_normal_equations():
+    """conjugate_gradient on D D^dagger (src/conjugate_gradient.cpp:5-65); the
+    reference prints the loop counter, one less than the passes counted here."""
+    z, meta = load("kry_cg")
+    o, v, _ = build(meta)
+    x, it = o.cg(v["b"], max_iter=meta["p"][0], tol=1e-10)
+    assert it == int(z["iterations"]) + 1
+    check(z, "x", x)
+
+
+def test_gmres_fine():
+    """FGMRES_fine_level (include/fgmres.h:126-148): GMRES(20) on D."""
+    z, meta = load("kry_gmres")
+    o, v, _ = build(meta)
+    x, it = o.gmres_fine(v["b"], m=meta["p"][0], restarts=meta["p"][1])
+    assert it == int(z["iterations"])
+    check(z, "x", x)
+
+
+def test_fgmres_sap():
+    """FGMRES_SAP (include/sap.h:194-225).  The reference's fine-level SAP
+    applies D in the D_phi form while the oracle's level-0 SAP uses the stencil
+    form (the same operator, other rounding), so x agrees to 1e-10, not bitwise."""
+    z, meta = load("kry_fsap")
+    o, v, _ = build(meta, out_m=meta["p"][0], out_restarts=meta["p"][1])
+    x, it, hist = o.fgmres_sap(v["b"])
+    assert it == int(z["iterations"])
+    np.testing.assert_allclose(hist, z["residuals"], rtol=1e-6)
+    idx = z["x__idx"]
+    assert np.max(np.abs(x[idx] - z["x__val"])) < 1e-10 * float(z["x__norm"]) / np.sqrt(x.size) * 100
+    assert abs(np.linalg.norm(x) - float(z["x__norm"])) < 1e-10 * float(z["x__norm"])
