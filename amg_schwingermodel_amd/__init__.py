@@ -101,6 +101,7 @@ SIGNATURES = {
     "amgs_cg_normal": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_fgmres_sap": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
     "amgs_gmres": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
+    "amgs_force": ([_V, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
@@ -296,6 +297,18 @@ class Solver:
     def gmres(self, b, hist_cap=1000):
         """GMRES without preconditioning (include/fgmres.h:126-148)."""
         return self._krylov(lib().amgs_gmres, b, hist_cap=hist_cap)
+
+    def force(self, phi):
+        """S = phi^+ (D D^+)^-1 phi and dS/dtheta [.., ns, 2] (adjoint method, two solves)."""
+        phi = _c(phi)
+        single = phi.ndim == 1
+        phi = phi.reshape(-1, self.size(0))
+        nrhs = phi.shape[0]
+        F = np.zeros((nrhs, self.size(0) // 2, 2))
+        S = np.zeros(nrhs)
+        self._chk(lib().amgs_force(self.h, nrhs, _p(phi), F.ctypes.data_as(_P), _p(S)), "amgs_force",
+                  ok=(AMGS_OK, AMGS_NOT_CONVERGED))
+        return (float(S[0]), F[0]) if single else (S, F)
 
     def multigrid(self, b, x0=None, max_cycles=100, tol=1e-10):
         """Stand-alone multigrid (src/amg.cpp:181-231); returns x, cycles, relres."""

@@ -1318,6 +1318,41 @@ extern "C" int amgs_gmres(amgs_ctx* ctx, int nrhs, const double* b, double* x, i
     API_END
 }
 
+// ---- fermion force (new: adjoint of the solve) ----
+// S = phi^dagger (D D^dagger)^-1 phi = |Y|^2 with Y = D^-1 phi; X = D^dagger^-1 Y
+// = s3 D^-1 s3 Y (D^dagger = s3 D s3); dS/dtheta from k_force.  Two solves of
+// the configured kind (plain or mixed), both to out_tol.
+extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, const double* phi, double* dsdtheta, double* action) {
+    if (!ctx || !phi || !dsdtheta || !action) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(nrhs >= 1 && nrhs <= std::min(c.P.max_rhs, KMAXB), AMGS_ERR_ARG, "nrhs must be in [1, min(max_rhs, 32)]");
+    const Geo& g = c.geo[0];
+    const size_t tot = c.nvec(0) * nrhs, nsr = (size_t)g.ns * nrhs;
+    DevBuf Yb, Fb;
+    Yb.alloc(sizeof(c64) * tot);
+    Fb.alloc(sizeof(double) * 2 * nsr);
+    HIP_CHECK(hipMemcpyAsync(c.ob.p, phi, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    std::vector<int> it(nrhs);
+    std::vector<double> rr(nrhs);
+    int rc1 = solve_impl(c, nrhs, it.data(), rr.data(), nullptr, 0, nullptr);  // Y = D^-1 phi
+    HIP_CHECK(hipMemcpyAsync(Yb.p, c.ox.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_sigma3, dim3(cdiv_u(nsr, 256)), dim3(256), 0, c.stream, nsr, (const c64*)Yb.as<c64>(),
+                       c.ob.as<c64>());
+    int rc2 = solve_impl(c, nrhs, it.data(), rr.data(), nullptr, 0, nullptr);  // s3 X = D^-1 s3 Y
+    hipLaunchKernelGGL(k_sigma3, dim3(cdiv_u(nsr, 256)), dim3(256), 0, c.stream, nsr, (const c64*)c.ox.as<c64>(),
+                       c.ox.as<c64>());
+    hipLaunchKernelGGL(k_force, dim3(cdiv_u(g.ns, FT), nrhs), dim3(FT), 0, c.stream, g, (const c64*)c.U64.as<c64>(),
+                       (const c64*)c.ox.as<c64>(), (const c64*)Yb.as<c64>(), Fb.as<double>());
+    KryRun K(c, nrhs);
+    K.vec(KOP_DOT, Yb.as<c64>(), Yb.as<c64>(), nullptr, nullptr, nullptr, nullptr, true);
+    K.fold();
+    for (int q = 0; q < nrhs; q++) action[q] = K.q(q, 0).real();
+    HIP_CHECK(hipMemcpy(dsdtheta, Fb.p, sizeof(double) * 2 * nsr, hipMemcpyDeviceToHost));
+    return (rc1 == AMGS_OK && rc2 == AMGS_OK) ? AMGS_OK : AMGS_NOT_CONVERGED;
+    API_END
+}
+
 // ---- component entry points (FP64 engine) ----
 static size_t level_n(Ctx& c, int l) { return c.nvec(l); }
 

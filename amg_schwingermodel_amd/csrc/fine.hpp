@@ -643,6 +643,44 @@ __global__ void k_convert(size_t n, const cplx<From>* __restrict__ in, cplx<To>*
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = cvt<To>(in[i]);
 }
+// out = sigma_3 in: the spin-1 components change sign (D^dagger = s3 D s3)
+__global__ void k_sigma3(size_t nsites_total, const c64* in, c64* out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsites_total) return;
+    out[2 * i] = in[2 * i];
+    out[2 * i + 1] = -in[2 * i + 1];
+}
+
+// Fermion force of S = phi^dagger (D D^dagger)^-1 phi with Y = D^-1 phi and
+// X = D^dagger^-1 Y: dS/dtheta_mu(n) = -2 Re X^dagger (dD/dtheta_mu(n)) Y with
+// dU/dtheta = iU, i.e. Re[i s U_mu(n) X(n)^+(1-s_mu)Y(n+mu) - i s U_mu(n)^* X(n+mu)^+(1+s_mu)Y(n)]
+// (s = -1 on the antiperiodic time hop).  F: [nrhs][ns][2] real.
+__global__ void __launch_bounds__(FT) k_force(Geo g, const c64* __restrict__ U, const c64* __restrict__ X,
+                                              const c64* __restrict__ Y, double* __restrict__ F) {
+    const int n = blockIdx.x * FT + threadIdx.x;
+    if (n >= g.ns) return;
+    const size_t off = (size_t)blockIdx.y * g.ns * 2;
+    const c64* x = X + off;
+    const c64* y = Y + off;
+    const c64 xn0 = x[2 * n], xn1 = x[2 * n + 1], yn0 = y[2 * n], yn1 = y[2 * n + 1];
+    for (int mu = 0; mu < 2; mu++) {
+        const int r = nb_right(n, mu, g.nx, g.nt);
+        const double s = wraps_right(n, mu, g.nt) ? -1.0 : 1.0;
+        const c64 u = U[2 * n + mu];
+        const c64 xr0 = x[2 * r], xr1 = x[2 * r + 1], yr0 = y[2 * r], yr1 = y[2 * r + 1];
+        c64 t1, t2;
+        if (mu == 0) {  // sigma_x: (1-s)v = (v0-v1)(1,-1), (1+s)v = (v0+v1)(1,1)
+            t1 = mulc(yr0 - yr1, xn0 - xn1);
+            t2 = mulc(yn0 + yn1, xr0 + xr1);
+        } else {  // sigma_y: (1-s)v = (v0+iv1)(1,-i), (1+s)v = (v0-iv1)(1,i)
+            t1 = mulc(yr0 + mul_i(yr1), xn0 + mul_i(xn1));
+            t2 = mulc(yn0 - mul_i(yn1), xr0 - mul_i(xr1));
+        }
+        const c64 z = u * t1 - conj(u) * t2;  // F = Re(i s z) = -s Im z
+        F[(size_t)blockIdx.y * g.ns * 2 + 2 * n + mu] = -s * z.im;
+    }
+}
+
 // out = a x + b y (real a, b; out may alias x or y)
 template <typename R>
 __global__ void k_lin(size_t n, double a, const cplx<R>* x, double b, const cplx<R>* y, cplx<R>* out) {

@@ -154,6 +154,15 @@ int amgs_fgmres_sap(amgs_ctx* c, int nrhs, const double* b, double* x, int* iter
 int amgs_gmres(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, double* relres, double* hist,
                int hist_cap, int* nhist);
 
+/* ---- fermion force (new; no counterpart in the reference, which stops at
+ *      the solve; the D^dagger it needs is D_dagger_phi, src/dirac_operator.cpp:30-54) ----
+ * S = phi^dagger (D D^dagger)^-1 phi and dS/dtheta_mu(n) for U_mu(n) = exp(i theta_mu(n)),
+ * by the adjoint method: Y = D^-1 phi, X = (D^dagger)^-1 Y (two multigrid
+ * solves), dS/dtheta = -2 Re X^dagger (dD/dtheta) Y.  phi [nrhs][nx*nt][2]
+ * complex; dsdtheta [nrhs][nx*nt][2] real (mu = 0 time, 1 space); action[nrhs].
+ * Needs amgs_setup; nrhs <= min(max_rhs, 32). */
+int amgs_force(amgs_ctx* c, int nrhs, const double* phi, double* dsdtheta, double* action);
+
 /* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
  * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as
  * [nsites_l][dof_l][dof_l] (row = 2c+alpha, col = 2b+beta). */

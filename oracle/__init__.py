@@ -72,6 +72,8 @@ def lib():
                                   ctypes.POINTER(ctypes.c_int)]
         L.or_cg.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double]
         L.or_fgmres_sap.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.or_force.argtypes = [vp, P, P]
+        L.or_force.restype = ctypes.c_double
         L.or_multigrid.argtypes = [vp, P, P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.or_solve.argtypes = [vp, P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double)]
@@ -201,6 +203,13 @@ class Oracle:
         nh = ctypes.c_int(0)
         it = lib().or_fgmres_sap(self.h, _p(b), _p(x), _p(h), hist_cap, ctypes.byref(nh))
         return x, it, h[: nh.value].copy()
+
+    def force(self, phi):
+        """(S, dS/dtheta [ns, 2]) for S = phi^+ (D D^+)^-1 phi (needs setup)."""
+        phi = _c(phi).reshape(-1)
+        F = np.zeros((self.size(0) // 2, 2))
+        S = lib().or_force(self.h, _p(phi), F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return S, F
 
     def multigrid(self, b, x0=None, it=100, tol=1e-10):
         """Stand-alone multigrid; returns x, cycles (-it if not converged), relres."""

@@ -1001,3 +1001,36 @@ int or_fgmres_sap(or_ctx* c, const double* b, double* x, double* hist, int hist_
     ws_free(&W);
     return it;
 }
+
+/* Fermion force of S = phi^+ (D D^+)^-1 phi (FP64 CPU gradient): Y = D^-1 phi,
+ * X = D^+^-1 Y = s3 D^-1 s3 Y by two multigrid solves, then
+ * dS/dtheta_mu(n) = Re[i s U_mu(n) X(n)^+(1-sigma_mu)Y(n+mu) - i s U_mu(n)^* X(n+mu)^+(1+sigma_mu)Y(n)]
+ * from dU/dtheta = iU on the hops of src/dirac_operator.cpp:3-27.  Returns S. */
+double or_force(or_ctx* c, const double* phid, double* F) {
+    const int ns = c->lv[0].ns, n = 2 * ns;
+    const lvl_t* v = &c->lv[0];
+    cx *Y = xcalloc(n, sizeof(cx)), *X = xcalloc(n, sizeof(cx)), *t = xcalloc(n, sizeof(cx));
+    double rr;
+    or_solve(c, phid, (double*)Y, NULL, 0, NULL, &rr);
+    for (int i = 0; i < ns; i++) { t[2 * i] = Y[2 * i]; t[2 * i + 1] = -Y[2 * i + 1]; }
+    or_solve(c, (const double*)t, (double*)X, NULL, 0, NULL, &rr);
+    for (int i = 0; i < ns; i++) X[2 * i + 1] = -X[2 * i + 1];
+    for (int s = 0; s < ns; s++)
+        for (int mu = 0; mu < 2; mu++) {
+            const int r = nb_right(s, mu, v->nx, v->nt);
+            const double sg = sg_right(s, mu, v->nt);
+            const cx u = c->U[2 * s + mu];
+            cx t1, t2;
+            if (mu == 0) {
+                t1 = conj(X[2 * s] - X[2 * s + 1]) * (Y[2 * r] - Y[2 * r + 1]);
+                t2 = conj(X[2 * r] + X[2 * r + 1]) * (Y[2 * s] + Y[2 * s + 1]);
+            } else {
+                t1 = conj(X[2 * s] + I * X[2 * s + 1]) * (Y[2 * r] + I * Y[2 * r + 1]);
+                t2 = conj(X[2 * r] - I * X[2 * r + 1]) * (Y[2 * s] - I * Y[2 * s + 1]);
+            }
+            F[2 * s + mu] = creal(I * sg * (u * t1 - conj(u) * t2));
+        }
+    double S = creal(vdot(Y, Y, n));
+    free(Y); free(X); free(t);
+    return S;
+}

@@ -93,6 +93,9 @@ int or_bicgstab(or_ctx* c, const double* b, double* x, int max_iter, double tol,
                 int* nhist);
 int or_cg(or_ctx* c, const double* phi, double* x, int max_iter, double tol);
 int or_fgmres_sap(or_ctx* c, const double* b, double* x, double* hist, int hist_cap, int* nhist);
+/* Fermion force of S = phi^+ (D D^+)^-1 phi: F [ns][2] real = dS/dtheta_mu(n);
+ * returns S.  Two multigrid solves (out_tol). */
+double or_force(or_ctx* c, const double* phi, double* F);
 /* V-cycle (src/amg.cpp:90-132) at level l from psi as given. */
 void or_vcycle(or_ctx* c, int l, const double* eta, double* psi);
 

@@ -253,3 +253,34 @@ def test_comparison_solvers_against_dense():
     for x, it in (o.fgmres_sap(b)[:2], o.gmres_fine(b)):
         assert it > 0
         np.testing.assert_allclose(x, xe, atol=1e-8)
+
+
+def test_force_against_finite_differences():
+    """dS/dtheta of S = phi^+ (D D^+)^-1 phi from the oracle's adjoint
+    formula against central differences (h = 1e-5) of S from dense solves,
+    on every link of an 8x8 lattice (pins the formula, signs and the
+    antiperiodic boundary)."""
+    nx = nt = 8
+    U, _ = amgs.gauge_metropolis(nx, nt, 2.0, 10, 11)
+    phi = np.random.default_rng(12).standard_normal(128) + 1j * np.random.default_rng(13).standard_normal(128)
+    rows = [(0, 2, 2, 2, 2, 2)]
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=0.1), U)
+    o.setup(draw_order=1)
+    S, F = o.force(phi)
+
+    def action(th):
+        ob = O.Oracle(O.default_params(nx, nt, rows, m0=0.1), np.exp(1j * th))
+        y = np.linalg.solve(dense(ob.dphi, 128), phi)
+        return np.vdot(y, y).real
+
+    th = np.angle(U)
+    assert abs(S - action(th)) < 1e-8 * S
+    h = 1e-5
+    Ffd = np.zeros_like(F)
+    for n in range(nx * nt):
+        for mu in range(2):
+            tp, tm = th.copy(), th.copy()
+            tp[n, mu] += h
+            tm[n, mu] -= h
+            Ffd[n, mu] = (action(tp) - action(tm)) / (2 * h)
+    np.testing.assert_allclose(F, Ffd, atol=1e-6 * np.abs(Ffd).max())

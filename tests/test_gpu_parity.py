@@ -258,3 +258,39 @@ def test_comparison_solvers(case):
             xo, ito = res[0], res[1]
             assert abs(int(its[i]) - ito) <= 1, (meth.__name__, its[i], ito)
             assert rrs[i] < 1e-10 and rel(X[i], xo) < 1e-9
+
+
+@pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED])
+def test_force(prec):
+    """Adjoint fermion force on the GPU against the oracle's FP64 CPU gradient,
+    and against central differences (h = 1e-5) of the GPU action on 16
+    random links."""
+    nx, nt, rows, m0, beta = CASES["32_3lvl"]
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
+    # tight solves so that the finite differences resolve the force
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec, max_rhs=2, out_tol=1e-13), U)
+    s.setup()
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    o.setup(draw_order=1)
+    phi = np.stack([cvec(s.size(0), 120 + i) for i in range(2)])
+    S, F = s.force(phi)
+    for i in range(2):
+        So, Fo = o.force(phi[i])
+        assert abs(S[i] - So) < 1e-9 * So
+        assert np.abs(F[i] - Fo).max() < 1e-8 * np.abs(Fo).max()
+    th = np.angle(U)
+    rng = np.random.default_rng(121)
+    h = 1e-5
+    for _ in range(16):
+        n, mu = int(rng.integers(nx * nt)), int(rng.integers(2))
+        Sp = []
+        for sg in (1, -1):
+            t = th.copy()
+            t[n, mu] += sg * h
+            sp = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec, out_tol=1e-13),
+                             np.exp(1j * t))
+            sp.setup()
+            Sp.append(sp.force(phi[0])[0])
+            sp.close()
+        fd = (Sp[0] - Sp[1]) / (2 * h)
+        assert abs(fd - F[0][n, mu]) < 1e-4 * np.abs(F[0]).max() + 1e-6 * abs(fd), (n, mu, fd, F[0][n, mu])
