@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libamgs.so")
 MAXL = 6
 
 AMGS_OK, AMGS_NOT_CONVERGED = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED16 = 0, 1, 2
+FORCE_DDAG_D, FORCE_D_DDAG = 0, 1
 
 
 class AmgsError(RuntimeError):
@@ -53,7 +54,8 @@ class Params(ctypes.Structure):
 
 def build(arch="gfx950"):
     """Compile libamgs.so in place with hipcc (no GPU needed)."""
-    subprocess.run(["make", "-s", "-C", _HERE, f"ARCH={arch}", "-B"], check=True)
+    jobs = str(max(1, min(8, os.cpu_count() or 1)))
+    subprocess.run(["make", "-s", "-C", _HERE, f"ARCH={arch}", "-B", "-j", jobs], check=True)
 
 
 def cli_path():
@@ -101,8 +103,11 @@ SIGNATURES = {
     "amgs_cg_normal": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_fgmres_sap": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
     "amgs_gmres": ([_V, ctypes.c_int, _P, _P, _I, _P, _P, ctypes.c_int, _I], ctypes.c_int),
-    "amgs_force": ([_V, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "amgs_force": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
+    "amgs_set_graph": ([_V, ctypes.c_int], ctypes.c_int),
+    "amgs_solve_multi": ([ctypes.c_int, ctypes.POINTER(_V), _I, ctypes.POINTER(_V), ctypes.POINTER(_V), _I, _P, _I],
+                         ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
 }
@@ -174,6 +179,28 @@ def gaussian(n, seed):
     return out
 
 
+def solve_multi(solvers, b_ptrs, x_ptrs, nrhs=None):
+    """Concurrent solves of independent contexts (amgs_solve_multi): solver i
+    solves nrhs[i] right-hand sides from device address b_ptrs[i] into
+    x_ptrs[i] (e.g. torch tensors' data_ptr()).  Returns (status, [iters_i],
+    [relres_i])."""
+    k = len(solvers)
+    nrhs = list(nrhs) if nrhs is not None else [1] * k
+    tot = sum(nrhs)
+    ctxs = (_V * k)(*[s.h for s in solvers])
+    nr = (ctypes.c_int * k)(*nrhs)
+    bp = (_V * k)(*b_ptrs)
+    xp = (_V * k)(*x_ptrs)
+    it = np.zeros(tot, np.int32)
+    rr = np.zeros(tot)
+    st = np.zeros(k, np.int32)
+    rc = lib().amgs_solve_multi(k, ctxs, nr, bp, xp, it.ctypes.data_as(_I), _p(rr), st.ctypes.data_as(_I))
+    if rc < 0:
+        raise AmgsError(f"amgs_solve_multi failed: {rc} (per-context status {st.tolist()})")
+    off = np.cumsum([0] + nrhs)
+    return st, [it[off[i]:off[i + 1]] for i in range(k)], [rr[off[i]:off[i + 1]] for i in range(k)]
+
+
 class Solver:
     """One hierarchy on the GPU (one amgs context)."""
 
@@ -214,6 +241,10 @@ class Solver:
 
     def setup(self):
         self._chk(lib().amgs_setup(self.h), "amgs_setup")
+
+    def set_graph(self, enable=True):
+        """Replay the level-0 preconditioner from a captured HIP graph."""
+        self._chk(lib().amgs_set_graph(self.h, 1 if enable else 0), "amgs_set_graph")
 
     def setup_from_tv(self, tvs, smooth=True):
         arrs = [_c(t).reshape(-1) for t in tvs]
@@ -298,15 +329,20 @@ class Solver:
         """GMRES without preconditioning (include/fgmres.h:126-148)."""
         return self._krylov(lib().amgs_gmres, b, hist_cap=hist_cap)
 
-    def force(self, phi):
-        """S = phi^+ (D D^+)^-1 phi and dS/dtheta [.., ns, 2] (adjoint method, two solves)."""
+    def force(self, phi, form="DdagD"):
+        """Pseudofermion action and dS/dtheta [.., ns, 2] by the adjoint method
+        (two solves): form "DdagD" is S = phi^+ (D^+ D)^-1 phi, "DDdag" is
+        S = phi^+ (D D^+)^-1 phi."""
+        forms = {"DdagD": FORCE_DDAG_D, "DDdag": FORCE_D_DDAG}
+        if form not in forms:
+            raise ValueError(f"form must be one of {sorted(forms)}")
         phi = _c(phi)
         single = phi.ndim == 1
         phi = phi.reshape(-1, self.size(0))
         nrhs = phi.shape[0]
         F = np.zeros((nrhs, self.size(0) // 2, 2))
         S = np.zeros(nrhs)
-        self._chk(lib().amgs_force(self.h, nrhs, _p(phi), F.ctypes.data_as(_P), _p(S)), "amgs_force",
+        self._chk(lib().amgs_force(self.h, nrhs, forms[form], _p(phi), F.ctypes.data_as(_P), _p(S)), "amgs_force",
                   ok=(AMGS_OK, AMGS_NOT_CONVERGED))
         return (float(S[0]), F[0]) if single else (S, F)
 

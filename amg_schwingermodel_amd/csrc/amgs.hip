@@ -21,179 +21,20 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/amgs.h"
-#include "coarse.hpp"
 #include "common.hpp"
+#include "engine.hpp"
 #include "fine.hpp"
 #include "host_util.hpp"
 #include "krylov.hpp"
 #include "outer.hpp"
 #include "setup.hpp"
 
-struct HipError : std::runtime_error {
-    hipError_t code;
-    HipError(hipError_t e, const char* what, const char* file, int line)
-        : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")"),
-          code(e) {}
-};
-struct ApiError : std::runtime_error {
-    int code;
-    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
 namespace amgs {
-
-// ---------------------------------------------------------------------------
-// device buffer
-// ---------------------------------------------------------------------------
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t b) {
-        if (b <= bytes && p) return;
-        release();
-        HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
-        bytes = std::max<size_t>(b, 256);
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-static inline unsigned cdiv_u(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
-
-struct Ctx;
-
-// ---------------------------------------------------------------------------
-// Per-launch timing of one kernel class inside solves: a HIP event pair on the
-// library stream around every launch, and a device counter of the items that
-// did work in that launch (for the algorithmic byte count).
-// ---------------------------------------------------------------------------
-struct KTimer {
-    static constexpr int CAP = 8192;  // launches recorded between two resolves
-    bool on = false;
-    std::vector<hipEvent_t> ev;
-    std::vector<double> fixed, per_item;  // bytes = fixed + per_item * active items
-    DevBuf actlog;
-    long long launches = 0;
-    double ms = 0, bytes = 0;
-
-    ~KTimer() {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-    void reset() {
-        launches = 0;
-        ms = bytes = 0;
-        fixed.clear();
-        per_item.clear();
-    }
-    // returns the counter for this launch, or nullptr when not recording
-    int* begin(hipStream_t s, double fx, double pi) {
-        if (!on || (int)fixed.size() >= CAP) return nullptr;
-        if (!actlog.p) {
-            actlog.alloc(sizeof(int) * CAP);
-            HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
-        }
-        const size_t i = fixed.size();
-        while (ev.size() < 2 * i + 2) {
-            hipEvent_t e;
-            HIP_CHECK(hipEventCreate(&e));
-            ev.push_back(e);
-        }
-        HIP_CHECK(hipEventRecord(ev[2 * i], s));
-        fixed.push_back(fx);
-        per_item.push_back(pi);
-        return actlog.as<int>() + i;
-    }
-    void end(hipStream_t s) { HIP_CHECK(hipEventRecord(ev[2 * (fixed.size() - 1) + 1], s)); }
-    // fold the pending records into the totals (synchronises the stream)
-    void resolve(hipStream_t s) {
-        if (fixed.empty()) return;
-        HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<int> act(fixed.size());
-        HIP_CHECK(hipMemcpy(act.data(), actlog.p, sizeof(int) * act.size(), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < fixed.size(); i++) {
-            float t = 0;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
-            ms += t;
-            launches++;
-            if (act[i] > 0) bytes += fixed[i] + per_item[i] * act[i];
-        }
-        HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
-        fixed.clear();
-        per_item.clear();
-    }
-};
-enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_CLASSES = 2 };
-
-// ---------------------------------------------------------------------------
-// per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments
-// ---------------------------------------------------------------------------
-template <typename R> struct Engine {
-    Ctx* c = nullptr;
-    int cap = 0;
-    // hierarchy in precision R
-    const cplx<R>* U = nullptr;
-    const cplx<R>* tv[MAXL] = {};
-    const cplx<R>* links[MAXL] = {};
-    // fine SAP workspace
-    DevBuf Vt, xl, r, st_mem, skip, part_r, part_v, zero;
-    SapState st{};
-    int nchunk_max = 0;
-    // K-cycle(0) temporaries
-    DevBuf eta1, psi1, res0;
-    // coarse workspace; kernel arguments live in device memory (one slot per launch in flight)
-    DevBuf cws, prof, cargs;
-    int carg_slot = 0;
-    static constexpr int CARG_SLOTS = 64;
-    CArgs<R> ca{};
-
-    void init(Ctx* ctx, int capacity);
-    void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
-    void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi, bool from_zero = true);
-    void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
-                int nu = 0);
-    void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
-};
-
-struct Ctx {
-    amgs_params P{};
-    int L = 0;
-    Geo geo[MAXL]{};
-    std::vector<int> red[MAXL], black[MAXL];
-    hipStream_t stream = nullptr;
-    int device = 0;
-    std::string err;
-    bool gauge_set = false, setup_done = false;
-    // FP64 hierarchy (always), FP32 copy in mixed mode
-    DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], redl[MAXL], blackl[MAXL];
-    Engine<double> e64;
-    Engine<float> e32;
-    // outer FGMRES
-    DevBuf oV, oZ, ox, ob, or_, ow, ost_mem, oeta32, opsi32;
-    OuterState ost{};
-    int ocap = 0;
-    // io staging
-    DevBuf io_a, io_b;
-    // comparison solvers: Krylov vectors, partial sums
-    DevBuf kbuf, kpart, ksum;
-    int outer_prec = 0;  // outer FGMRES preconditioner: 0 multigrid cycle, 1 one SAP sweep, 2 none
-    double last_ms[3] = {0, 0, 0};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    KTimer kt[KT_CLASSES];
-
-    size_t nvec(int l) const { return (size_t)geo[l].ns * geo[l].dof; }
-    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
-};
 
 // ---------------------------------------------------------------------------
 // geometry (include/params.h:30-65, include/level.h:147-155, include/sap.h:64-78)
@@ -259,7 +100,9 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     const Geo& g0 = c->geo[0];
     const size_t n0 = c->nvec(0);
     const int m = c->P.sap_m;
-    Vt.alloc(sizeof(cplx<R>) * (m + 1) * cap * n0);
+    const bool h16 = std::is_same<R, float>::value && c->h16();
+    Vt.alloc(sizeof(cplx<R>) * (h16 ? 2 : m + 1) * cap * n0);
+    if (h16) Vh.alloc(sizeof(f16x4) * m * cap * g0.ns);
     xl.alloc(sizeof(cplx<R>) * cap * n0);
     r.alloc(sizeof(cplx<R>) * cap * n0);
     res0.alloc(sizeof(cplx<R>) * cap * n0);
@@ -351,7 +194,6 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
         }
     }
     ca.ws_per_rhs = (wo + 63) & ~size_t(63);
-    cargs.alloc(sizeof(CArgs<R>) * CARG_SLOTS);
     const char* pe = std::getenv("AMGS_PROFILE");
     if (pe && pe[0] && pe[0] != '0') {
         prof.alloc(sizeof(unsigned long long) * 32 * cap);
@@ -359,52 +201,6 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     }
     cws.alloc(sizeof(cplx<R>) * ca.ws_per_rhs * cap);
     ca.ws = cws.as<cplx<R>>();
-}
-
-template <typename R>
-void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out,
-                       size_t out_stride, int nu) {
-    if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
-    CArgs<R> a = ca;
-    a.mm = R(c->P.m0 + 2.0);
-    for (int l = 1; l < c->L; l++) {
-        a.lv[l].links = links[l];
-        a.lv[l].w = l < c->L - 1 ? tv[l] : nullptr;
-    }
-    a.mode = mode;
-    a.level = level;
-    a.nu = nu;
-    // coarsest GMRES vectors in dynamic LDS if they fit beside the static part
-    const size_t ncrs = c->nvec(c->L - 1);
-    const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
-    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
-    a.crs_lds = dyn <= lds_cap ? 1 : 0;
-    size_t dyn_bytes = a.crs_lds ? dyn : 0;
-    // SAP block vectors of a level share that region (never live at the same time)
-    for (int l = 0; l < MAXL; l++) {
-        const size_t need = l >= 1 && l < c->L - 1 ? CW * a.lv[l].blkStride * sizeof(cplx<R>) : 0;
-        a.sap_lds[l] = need > 0 && need <= lds_cap ? 1 : 0;
-        if (a.sap_lds[l]) dyn_bytes = std::max(dyn_bytes, need);
-    }
-    a.prof = prof.p ? prof.as<unsigned long long>() : nullptr;
-    a.in = in;
-    a.out = out;
-    a.in_stride = in_stride;
-    a.out_stride = out_stride;
-    if (dyn_bytes > 0) {
-        static bool attr_set[2] = {false, false};
-        bool& done = attr_set[sizeof(R) == 8];
-        if (!done) {
-            HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)(160 * 1024 - sizeof(CLds))));
-            done = true;
-        }
-    }
-    CArgs<R>* dargs = cargs.as<CArgs<R>>() + carg_slot;
-    carg_slot = (carg_slot + 1) % CARG_SLOTS;
-    HIP_CHECK(hipMemcpyAsync(dargs, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, (const CArgs<R>*)dargs);
-    HIP_CHECK(hipGetLastError());
 }
 
 template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out) {
@@ -433,11 +229,14 @@ static int sap_chunks(const Geo& g, int nrhs) {
     return std::max(1, std::min(maxc, (int)cdiv_u(target, items)));
 }
 
-template <typename R>
+// FP16 Krylov storage exists for the FP32 engine only
+template <typename R> constexpr bool kH16 = std::is_same<R, float>::value;
+
+template <typename R, bool H>
 static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act) {
     switch (j) {
 #define AMGS_SAP_A(J) \
-    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J>), grid, dim3(FT), 0, s, a, k, act); break;
+    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J, H>), grid, dim3(FT), 0, s, a, k, act); break;
         AMGS_SAP_A(0) AMGS_SAP_A(1) AMGS_SAP_A(2) AMGS_SAP_A(3)
         AMGS_SAP_A(4) AMGS_SAP_A(5) AMGS_SAP_A(6) AMGS_SAP_A(7)
 #undef AMGS_SAP_A
@@ -452,7 +251,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     const R mm = R(c->P.m0 + 2.0);
     const unsigned ntile = cdiv_u(g.ns, FT);
     hipStream_t s = c->stream;
-    HIP_CHECK(hipMemsetAsync(skip.p, 0, sizeof(int) * nrhs, s));
+    hipLaunchKernelGGL(k_zero_int, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, nrhs, skip.as<int>());
     if (nu <= 0) return;
     hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
@@ -473,6 +272,11 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     a.xl = xl.as<cplx<R>>();
     a.skip = skip.as<int>();
     a.st = st;
+    const bool h16 = std::is_same<R, float>::value && c->h16();
+    a.Vh = h16 ? Vh.as<f16x4>() : nullptr;
+    // power-of-two scale of the FP16 slots: a unit-norm block vector has
+    // entries of order 1 / sqrt(2 sspb), stored as entries of order 1
+    a.sig = h16 ? std::ldexp(1.0f, (int)std::lround(0.5 * std::log2(2.0 * g.sspb))) : 1.0f;
     const dim3 grid(g.ncol * a.nchunk, nrhs);
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
@@ -481,18 +285,25 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
             for (int k = 0; k < c->P.sap_restarts; k++) {
                 for (int j = 0; j < a.m; j++) {
                     // algorithmic bytes: links of the colour's sites once; per active
-                    // item Vt_j read, w written, Vt_i (i < j) read for the dots
+                    // item Vt_j read, w written, Vt_i (i < j) read for the dots.
+                    // With FP16 slots: w and slot 0 complex FP32, slots >= 1 FP16.
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
-                    int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * 2 * S * (2 + j));
-                    sap_arnoldi_a(grid, s, a, j, k, act);
+                    const double pa = h16 ? 32.0 + 8.0 * j : 2 * S * (2 + j);
+                    int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * pa);
+                    if (h16) sap_arnoldi_a<R, kH16<R>>(grid, s, a, j, k, act);
+                    else sap_arnoldi_a<R, false>(grid, s, a, j, k, act);
                     if (act) c->kt[KT_SAP_A].end(s);
                     // w and Vt_i (i <= j) read, w written
-                    act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * 2 * S * (3 + j));
-                    hipLaunchKernelGGL(k_sap_arnoldi_b<R>, grid, dim3(FT), 0, s, a, j, act);
+                    const double pb = h16 ? 40.0 + 8.0 * j : 2 * S * (3 + j);
+                    act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * pb);
+                    if (h16) hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
+                    else hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
                     if (act) c->kt[KT_SAP_B].end(s);
                 }
                 const bool last = k + 1 == c->P.sap_restarts;
-                hipLaunchKernelGGL(k_sap_update<R>, grid, dim3(FT), 0, s, a, last ? x : (cplx<R>*)nullptr);
+                cplx<R>* xacc = last ? x : (cplx<R>*)nullptr;
+                if (h16) hipLaunchKernelGGL((k_sap_update<R, kH16<R>>), grid, dim3(FT), 0, s, a, xacc);
+                else hipLaunchKernelGGL((k_sap_update<R, false>), grid, dim3(FT), 0, s, a, xacc);
                 if (!last) hipLaunchKernelGGL(k_sap_resid<R>, grid, dim3(FT), 0, s, a);
             }
             hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
@@ -605,13 +416,14 @@ static void bootstrap(Ctx& c) {
 }
 
 static void setup_common(Ctx& c, bool smooth) {
+    c.clear_graphs();
     bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
     for (int l = 0; l < c.L - 1; l++) {
         if (smooth) smooth_level(c, l);
         ortho_links(c, l);
     }
     if (c.P.nit > 0) bootstrap(c);
-    if (c.P.precision == AMGS_PREC_MIXED) {
+    if (c.mixed()) {
         convert64to32(c, c.U64, c.U32, c.nvec(0));
         for (int l = 0; l < c.L; l++) {
             if (l < c.L - 1) convert64to32(c, c.tv64[l], c.tv32[l], (size_t)c.geo[l].ntest * c.nvec(l));
@@ -665,9 +477,12 @@ static void outer_alloc(Ctx& c, int cap) {
     c.ob.alloc(sizeof(c64) * cap * n0);
     c.or_.alloc(sizeof(c64) * cap * n0);
     c.ow.alloc(sizeof(c64) * cap * n0);
-    if (c.P.precision == AMGS_PREC_MIXED) {
+    if (c.mixed()) {
         c.oeta32.alloc(sizeof(c32) * cap * n0);
         c.opsi32.alloc(sizeof(c32) * cap * n0);
+    } else {
+        c.oeta64.alloc(sizeof(c64) * cap * n0);
+        c.opsi64.alloc(sizeof(c64) * cap * n0);
     }
     const int ntile = (int)cdiv_u(c.geo[0].ns, FT);
     const int hcap = M * c.P.out_restarts;
@@ -707,8 +522,52 @@ static void outer_alloc(Ctx& c, int cap) {
 }
 
 // preconditioner Z_j = K(V_j) for all rhs of the batch
+// The multigrid cycle at level 0 on the fixed staging buffers (oeta32 ->
+// opsi32 in mixed precision, oeta64 -> opsi64 in FP64): the part of the
+// preconditioner that is captured into a HIP graph.
+static void precond_body(Ctx& c, int nrhs) {
+    if (c.mixed()) c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
+    else c.e64.kcycle0(nrhs, c.oeta64.as<c64>(), c.opsi64.as<c64>());
+}
+
+// Executable graph of precond_body for this batch size (captured on first use).
+static hipGraphExec_t precond_graph(Ctx& c, int nrhs) {
+    for (auto& g : c.pgraphs)
+        if (g.first == nrhs) return g.second;
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        precond_body(c, nrhs);
+    } catch (...) {
+        if (hipStreamEndCapture(c.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(e);
+    c.pgraphs.emplace_back(nrhs, exec);
+    return exec;
+}
+
 static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
+    // graph replay (kernel timers record eager launches only)
+    if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on) {
+        if (c.mixed())
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
+                               c.oeta32.as<c32>());
+        else
+            HIP_CHECK(hipMemcpyAsync(c.oeta64.p, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs), c.stream));
+        if (c.mixed())
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                               (const c32*)c.opsi32.as<c32>(), zj);
+        else
+            HIP_CHECK(hipMemcpyAsync(zj, c.opsi64.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        return;
+    }
     if (c.outer_prec == 1) {
         // FGMRES_SAP::preconditioner (include/sap.h:212-224): zero, one SAP sweep
         hipLaunchKernelGGL(k_zero<double>, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, zj);
@@ -719,7 +578,7 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
         HIP_CHECK(hipMemcpyAsync(zj, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
         return;
     }
-    if (c.P.precision == AMGS_PREC_MIXED) {
+    if (c.mixed()) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                            c.oeta32.as<c32>());
         c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
@@ -887,6 +746,8 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
         Ctx& c = ctx->c;
         c.P = *p;
         if (c.P.max_rhs < 1) c.P.max_rhs = 1;
+        if (c.P.precision != AMGS_PREC_FP64 && c.P.precision != AMGS_PREC_MIXED && c.P.precision != AMGS_PREC_MIXED16)
+            throw ApiError(AMGS_ERR_ARG, "precision must be AMGS_PREC_FP64, AMGS_PREC_MIXED or AMGS_PREC_MIXED16");
         build_geometry(c);
         HIP_CHECK(hipGetDevice(&c.device));
         HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -905,7 +766,7 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
         }
         int ntest_max = 1;
         for (int l = 0; l < c.L - 1; l++) ntest_max = std::max(ntest_max, c.geo[l].ntest);
-        const bool mixed = c.P.precision == AMGS_PREC_MIXED;
+        const bool mixed = c.mixed();
         c.e64.init(&c, mixed ? ntest_max : std::max(ntest_max, c.P.max_rhs));
         if (mixed) c.e32.init(&c, c.P.max_rhs);
         outer_alloc(c, c.P.max_rhs);
@@ -930,6 +791,7 @@ extern "C" void amgs_destroy(amgs_ctx* ctx) {
     Ctx& c = ctx->c;
     hipStream_t s = c.stream;
     hipEvent_t a = c.ev0, b = c.ev1;
+    c.clear_graphs();
     delete ctx;  // frees device buffers
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
@@ -1038,6 +900,45 @@ extern "C" int amgs_solve_dev(amgs_ctx* ctx, int nrhs, const void* b_dev, void* 
     c.sync();
     return rc;
     API_END
+}
+
+extern "C" int amgs_set_graph(amgs_ctx* ctx, int enable) {
+    if (!ctx) return AMGS_ERR_ARG;
+    ctx->c.graph = enable != 0;
+    return AMGS_OK;
+}
+
+// ---- uneven multi-configuration batch ----
+// Independent contexts (each its own lattice size, hierarchy, mass and HIP
+// stream) solve concurrently: one host thread per context drives its solve, so
+// the kernels of all configurations overlap on the device.
+extern "C" int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev,
+                                void* const* x_dev, int* iters, double* relres, int* status) {
+    if (nctx < 1 || !ctxs || !nrhs || !b_dev || !x_dev || !iters || !relres || !status) return AMGS_ERR_ARG;
+    std::vector<size_t> off(nctx + 1, 0);
+    for (int i = 0; i < nctx; i++) {
+        if (!ctxs[i] || nrhs[i] < 1) return AMGS_ERR_ARG;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return AMGS_ERR_ARG;  // one solve per context at a time
+        off[i + 1] = off[i] + nrhs[i];
+    }
+    std::vector<std::thread> th;
+    th.reserve(nctx);
+    for (int i = 0; i < nctx; i++)
+        th.emplace_back([&, i]() {
+            if (hipSetDevice(ctxs[i]->c.device) != hipSuccess) {
+                status[i] = AMGS_ERR_HIP;
+                return;
+            }
+            status[i] = amgs_solve_dev(ctxs[i], nrhs[i], b_dev[i], x_dev[i], iters + off[i], relres + off[i]);
+        });
+    for (std::thread& t : th) t.join();
+    int rc = AMGS_OK;
+    for (int i = 0; i < nctx; i++) {
+        if (status[i] < 0) return status[i];
+        if (status[i] == AMGS_NOT_CONVERGED) rc = AMGS_NOT_CONVERGED;
+    }
+    return rc;
 }
 
 // ---- comparison solvers on the fine lattice (FP64) ----
@@ -1320,10 +1221,12 @@ extern "C" int amgs_gmres(amgs_ctx* ctx, int nrhs, const double* b, double* x, i
 
 // ---- fermion force (new: adjoint of the solve) ----
 // S = phi^dagger (D D^dagger)^-1 phi = |Y|^2 with Y = D^-1 phi; X = D^dagger^-1 Y
-// = s3 D^-1 s3 Y (D^dagger = s3 D s3); dS/dtheta from k_force.  Two solves of
-// the configured kind (plain or mixed), both to out_tol.
-extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, const double* phi, double* dsdtheta, double* action) {
+// = s3 D^-1 s3 Y (D^dagger = s3 D s3); dS/dtheta from k_force.  The form
+// phi^dagger (D^dagger D)^-1 phi is the same at s3 phi.  Two solves of the
+// configured kind (plain or mixed), both to out_tol.
+extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, int form, const double* phi, double* dsdtheta, double* action) {
     if (!ctx || !phi || !dsdtheta || !action) return AMGS_ERR_ARG;
+    if (form != AMGS_FORCE_DDAG_D && form != AMGS_FORCE_D_DDAG) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
     need(nrhs >= 1 && nrhs <= std::min(c.P.max_rhs, KMAXB), AMGS_ERR_ARG, "nrhs must be in [1, min(max_rhs, 32)]");
@@ -1333,6 +1236,9 @@ extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, const double* phi, double* ds
     Yb.alloc(sizeof(c64) * tot);
     Fb.alloc(sizeof(double) * 2 * nsr);
     HIP_CHECK(hipMemcpyAsync(c.ob.p, phi, sizeof(c64) * tot, hipMemcpyHostToDevice, c.stream));
+    if (form == AMGS_FORCE_DDAG_D)
+        hipLaunchKernelGGL(k_sigma3, dim3(cdiv_u(nsr, 256)), dim3(256), 0, c.stream, nsr, (const c64*)c.ob.as<c64>(),
+                           c.ob.as<c64>());
     std::vector<int> it(nrhs);
     std::vector<double> rr(nrhs);
     int rc1 = solve_impl(c, nrhs, it.data(), rr.data(), nullptr, 0, nullptr);  // Y = D^-1 phi
@@ -1585,7 +1491,7 @@ extern "C" int amgs_time_kernel(amgs_ctx* ctx, int kernel, int nrhs, int reps, d
             hipLaunchKernelGGL((k_wilson<double, false>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
                                (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), c.ow.as<c64>());
         else if (kernel == 1) {
-            need(c.P.precision == AMGS_PREC_MIXED && c.setup_done, AMGS_ERR_STATE, "FP32 operator needs mixed setup");
+            need(c.mixed() && c.setup_done, AMGS_ERR_STATE, "FP32 operator needs mixed setup");
             hipLaunchKernelGGL((k_wilson<float, false>), grid, dim3(FT), 0, c.stream, g, float(c.P.m0 + 2.0),
                                (const c32*)c.U32.as<c32>(), (const c32*)c.oeta32.as<c32>(), c.opsi32.as<c32>());
         } else

@@ -32,7 +32,7 @@ void usage() {
                  "  --m0 M             bare mass (default -0.18840579710144945)\n"
                  "  --cycle K|V        multigrid cycle (default K)\n"
                  "  --nit N            bootstrap iterations of the set-up (default 0)\n"
-                 "  --precision fp64|mixed\n"
+                 "  --precision fp64|mixed|mixed16\n"
                  "  --restart M --restarts K --tol T   outer FGMRES (default 20, 50, 1e-10)\n"
                  "  --x-out FILE       write the solution of right-hand side 0\n"
                  "  --quiet            no parameter banner\n");
@@ -78,7 +78,8 @@ bool parse(int argc, char** argv, Opts& o) {
         else if (a == "--precision") {
             if (!std::strcmp(v, "fp64")) o.precision = AMGS_PREC_FP64;
             else if (!std::strcmp(v, "mixed")) o.precision = AMGS_PREC_MIXED;
-            else { std::fprintf(stderr, "amgs_solve: --precision fp64 or mixed\n"); return false; }
+            else if (!std::strcmp(v, "mixed16")) o.precision = AMGS_PREC_MIXED16;
+            else { std::fprintf(stderr, "amgs_solve: --precision fp64, mixed or mixed16\n"); return false; }
         } else if (a == "--restart") o.restart = std::atoi(v);
         else if (a == "--restarts") o.restarts = std::atoi(v);
         else if (a == "--tol") o.tol = std::atof(v);

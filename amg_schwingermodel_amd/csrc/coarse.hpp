@@ -12,6 +12,8 @@
 // order).  SAP blocks of one colour run concurrently, one wavefront per block,
 // each with its own control flow and wave-level reductions (no block barrier).
 #pragma once
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace amgs {
@@ -76,12 +78,22 @@ struct WaveScal {
     int brk;
 };
 constexpr int VIN_BYTES = 16384;  // LDS staging of stencil inputs
+// A suspended solve of the nested K-cycle: FGMRES at a level (in = b, out = x),
+// k_cycle at a level (in = eta, out = psi) or the coarsest GMRES.
+struct CFrame {
+    const void* in;
+    void* out;
+    int pc, k, j, maxIt;
+    double nb, err;
+};
 struct CLds {
     c64 red[CT];
     alignas(16) char vin[VIN_BYTES];
     FgScal<KC_MAXM> kc[MAXL];
     FgScal<CRS_MAXM> crs;
     WaveScal wv[CW];
+    CFrame fg[MAXL], kcf[MAXL], crsf;  // frames per level and kind
+    int stk[2 * MAXL + 2], sp;         // frame stack: kind * 16 + level
 };
 
 // ---------------------------------------------------------------------------
@@ -139,10 +151,9 @@ template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { 
 
 // Row r of site x: the five link blocks (column stride D) against the on-site
 // and the four neighbour vectors, five independent accumulators.
-template <typename R, typename VP>
-__device__ __forceinline__ void stencil_row_p(gptr<cplx<R>> L0, gptr<cplx<R>> L1, gptr<cplx<R>> L2, gptr<cplx<R>> L3,
-                                              gptr<cplx<R>> L4, int D, VP v0, VP vr0, VP vr1, VP vl0, VP vl1, R f1, R f2,
-                                              R f3, R f4, cplx<R>& acc) {
+template <typename R, typename LP, typename VP>
+__device__ __forceinline__ void stencil_row_p(LP L0, LP L1, LP L2, LP L3, LP L4, int D, VP v0, VP vr0, VP vr1, VP vl0,
+                                              VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
     cplx<R> a0 = czero<R>(), a1 = czero<R>(), a2 = czero<R>(), a3 = czero<R>(), a4 = czero<R>();
 #pragma unroll 4
     for (int c = 0; c < D; c++) {
@@ -158,21 +169,20 @@ __device__ __forceinline__ void stencil_row_p(gptr<cplx<R>> L0, gptr<cplx<R>> L1
     acc -= f2 * a2 + f4 * a4;
 }
 
-template <typename R, typename VP>
-__device__ __forceinline__ void stencil_row(gptr<cplx<R>> links, int ksz, int msz, int D, int x, int r, VP v0, VP vr0,
-                                            VP vr1, VP vl0, VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
-    gptr<cplx<R>> L0 = links + x * msz + r;
+template <typename R, typename LP, typename VP>
+__device__ __forceinline__ void stencil_row(LP links, int ksz, int msz, int D, int x, int r, VP v0, VP vr0, VP vr1,
+                                            VP vl0, VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
+    LP L0 = links + x * msz + r;
     stencil_row_p<R>(L0, L0 + ksz, L0 + 2 * ksz, L0 + 3 * ksz, L0 + 4 * ksz, D, v0, vr0, vr1, vl0, vl1, f1, f2, f3, f4,
                      acc);
 }
 
-template <typename R, typename VP>
-__device__ __forceinline__ void dop_rows(const CArgs<R>& A, int l, VP src, cplx<R>* out) {
+template <typename R, typename LP, typename VP>
+__device__ __forceinline__ void dop_rows_l(const CArgs<R>& A, int l, LP links, VP src, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, n = g.ns * D;
     const int msz = D * D, ksz = g.ns * msz;
-    gptr<cplx<R>> links = as_global(L.links);
     for (int e = threadIdx.x; e < n; e += CT) {
         const int x = e / D, r = e - x * D;
         const int xr0 = nb_right(x, 0, g.nx, g.nt), xr1 = nb_right(x, 1, g.nx, g.nt);
@@ -184,6 +194,10 @@ __device__ __forceinline__ void dop_rows(const CArgs<R>& A, int l, VP src, cplx<
                        f1, R(1), f3, R(1), acc);
         out[e] = acc;
     }
+}
+template <typename R, typename VP>
+__device__ __forceinline__ void dop_rows(const CArgs<R>& A, int l, VP src, cplx<R>* out) {
+    dop_rows_l<R>(A, l, as_global(A.lv[l].links), src, out);
 }
 
 // out = D_l in (src/level.cpp:161-180): (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu)).
@@ -268,13 +282,12 @@ template <typename R> __device__ __forceinline__ c64 w_dot(const cplx<R>* a, con
 // block-local stencil, neighbours outside block blk read as zero: multiplier
 // 0, and both the vector and the link reads are redirected to the site's own
 // (already loaded) entries, so no link block outside the block is fetched
-template <typename R>
-__device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
+template <typename R, typename LP>
+__device__ __forceinline__ void w_dlocal_l(const CArgs<R>& A, int l, int blk, LP links, const cplx<R>* in, cplx<R>* out) {
     const CLevel<R>& L = A.lv[l];
     const Geo& g = L.g;
     const int D = g.dof, nb = g.sspb * D, lane = threadIdx.x & 63;
     const int msz = D * D, ksz = g.ns * msz;
-    gptr<cplx<R>> links = as_global(L.links);
     for (int e = lane; e < nb; e += 64) {
         const int m = e / D, r = e - m * D;
         const int n = sap_site(g, blk, m);
@@ -282,7 +295,7 @@ __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, cons
                       nb_left(n, 1, g.nx, g.nt)};
         R f[4];
         const cplx<R>* vp[4];
-        gptr<cplx<R>> lk[5];
+        LP lk[5];
         lk[0] = links + n * msz + r;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -298,6 +311,10 @@ __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, cons
         out[e] = acc;
     }
     wave_sync();
+}
+template <typename R>
+__device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
+    w_dlocal_l<R>(A, l, blk, as_global(A.lv[l].links), in, out);
 }
 
 // Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
@@ -442,20 +459,32 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
 }
 
 // ---------------------------------------------------------------------------
-// workgroup FGMRES (src/fgmres.cpp:5-88) and the K-cycle recursion
+// workgroup FGMRES (src/fgmres.cpp:5-88) and the K-cycle recursion, run as a
+// state machine over per-level frames in LDS: FGMRES_k_cycle[l].fgmres ->
+// k_cycle(l) -> FGMRES_k_cycle[l+1].fgmres -> ... -> the coarsest GMRES.  Each
+// level body exists once (run-time level), so the kernel stays compact; a
+// frame that needs a nested solve saves its loop state, pushes the child and
+// is resumed when the child pops.  The order of operations within each solve
+// is that of the recursive formulation.
 // ---------------------------------------------------------------------------
-template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi);
+enum { FR_FG = 0, FR_KC = 1, FR_CRS = 2 };
 
-// Restarted FGMRES from x0 = b.  crs: the coarsest GMRES (identity
-// preconditioner, GMRES_level_l); otherwise the K-cycle FGMRES whose
-// preconditioner is k_cycle(LV) started from the previous cycle's Z_j
+// thread 0 only; the run loop's barrier publishes the change
+__device__ __forceinline__ void frame_push(CLds& S, int kind, int l) { S.stk[++S.sp] = kind * 16 + l; }
+__device__ __forceinline__ void frame_pop(CLds& S) { S.sp--; }
+
+// Restarted FGMRES from x0 = b (frame F, level l).  CRS: the coarsest GMRES
+// (identity preconditioner, GMRES_level_l); otherwise the K-cycle FGMRES whose
+// preconditioner is k_cycle(l) started from the previous cycle's Z_j
 // (include/amg.h:37-39).  One-pass MGS with the Gram correction:
 //   H_ij = <w, V_i> - sum_{k<i} H_kj <V_k, V_i>.
 // Scalars (Hessenberg column, Givens, residual estimate) are computed by
-// thread 0 in FP64 and broadcast through LDS.
-template <typename R, int LV, int M>
-__device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>& S, bool crs, const cplx<R>* b, cplx<R>* x) {
-    const CLevel<R>& L = A.lv[LV];
+// thread 0 in FP64 and broadcast through LDS.  Loop state that survives a
+// nested solve (k, j, maxIt, |b|, |r|) lives in the frame.
+template <typename R, int M, bool CRS>
+__device__ __forceinline__ void fg_run(const CArgs<R>& A, CK<R>& K, int l, FgScal<M>& S, CFrame& F) {
+    constexpr bool crs = CRS;
+    const CLevel<R>& L = A.lv[l];
     const int n = L.g.ns * L.g.dof;
     const int m = crs ? A.crs_m : A.kc_m;
     const int restarts = crs ? A.crs_restarts : A.kc_restarts;
@@ -469,30 +498,58 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
     cplx<R>* Dx = lds ? K.dyn + (size_t)(m + 3) * n : K.W + (crs ? L.crDx : L.kcDx);
     auto V = [&](int j) { return Vb + (size_t)j * n; };
     auto Z = [&](int j) { return Zb + (size_t)j * n; };
-    // setZeros (include/fgmres.h:93-120): Z_j start from zero in the first cycle
-    __syncthreads();
-    if (!crs)
-        for (int e = threadIdx.x; e < m * n; e += CT) Zb[e] = czero<R>();
-    for (int e = threadIdx.x; e < n; e += CT) x[e] = b[e];
-    c_dop(A, LV, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
-    for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
-    const double nb = sqrt(c_dot(K.S, b, b, n).re);
-    double err = sqrt(c_dot(K.S, r, r, n).re);
-    if (err == 0.0) return;
-    int maxIt = m;
-    for (int k = 0; k < restarts; k++) {
-        const R ib = R(1.0 / err);
-        for (int e = threadIdx.x; e < n; e += CT) V(0)[e] = ib * r[e];
-        if (threadIdx.x == 0) S.g[0] = c64{err, 0.0};
-        for (int j = 0; j < m; j++) {
-            cplx<R>* zj = Z(j);
+    const cplx<R>* b = (const cplx<R>*)F.in;
+    cplx<R>* x = (cplx<R>*)F.out;
+    int pc = F.pc, k = F.k, j = F.j, maxIt = F.maxIt;
+    double nb = F.nb, err = F.err;
+    __syncthreads();  // every thread has read the frame before thread 0 rewrites it
+    for (;;) {
+        if (pc == 0) {
+            // setZeros (include/fgmres.h:93-120): Z_j start from zero in the first cycle
+            if (!crs)
+                for (int e = threadIdx.x; e < m * n; e += CT) Zb[e] = czero<R>();
+            for (int e = threadIdx.x; e < n; e += CT) x[e] = b[e];
+            c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
+            nb = sqrt(c_dot(K.S, b, b, n).re);
+            err = sqrt(c_dot(K.S, r, r, n).re);
+            if (err == 0.0) break;
+            maxIt = m;
+            k = 0;
+            pc = 1;
+        }
+        if (pc == 1) {  // cycle k starts
+            const R ib = R(1.0 / err);
+            for (int e = threadIdx.x; e < n; e += CT) V(0)[e] = ib * r[e];
+            if (threadIdx.x == 0) S.g[0] = c64{err, 0.0};
+            j = 0;
+            pc = 2;
+        }
+        if (pc == 2) {  // step j: the preconditioner, a nested solve
             if (!crs) {
-                if constexpr (LV < MAXL) c_kcycle<R, LV>(A, K, V(j), zj);
+                if (threadIdx.x == 0) {
+                    F.pc = 3;
+                    F.k = k;
+                    F.j = j;
+                    F.maxIt = maxIt;
+                    F.nb = nb;
+                    F.err = err;
+                    CFrame& C = K.S.kcf[l];
+                    C.in = V(j);
+                    C.out = Z(j);
+                    C.pc = 0;
+                    frame_push(K.S, FR_KC, l);
+                }
+                return;
             }
-            const int pb = crs ? 0 : 8 + 4 * (LV - 1 > 2 ? 2 : LV - 1);  // timer slots
+            pc = 3;
+        }
+        if (pc == 3) {  // step j after the preconditioner
+            cplx<R>* zj = Z(j);
+            const int pb = crs ? 0 : 8 + 4 * (l - 1 > 2 ? 2 : l - 1);  // timer slots
             PhaseTimer T(K.prof);
             T.count(crs ? 7 : pb + 3);
-            c_dop(A, LV, zj, w, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            c_dop(A, l, zj, w, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
             T.lap(pb + 0);
             // one pass: <w, V_i> (i <= j) and <V_j, V_k> (k < j); nq dots,
             // CT/nq threads per dot, partial sums combined in fixed order
@@ -564,68 +621,111 @@ __device__ __forceinline__ void c_fgmres(const CArgs<R>& A, CK<R>& K, FgScal<M>&
             T.lap(crs ? 4 : pb + 2);
             if (S.brk) {
                 maxIt = j + 1;
-                break;
+                pc = 4;
+            } else if (++j < m) {
+                pc = 2;
+                continue;
+            } else {
+                pc = 4;
             }
         }
-        if (threadIdx.x == 0) {
-            for (int i = maxIt - 1; i >= 0; i--) {
-                c64 e = S.g[i];
-                for (int q = i + 1; q < maxIt; q++) e -= S.R[i][q] * S.eta[q];
-                S.eta[i] = cdiv(e, S.R[i][i]);
+        if (pc == 4) {  // end of cycle k: x += Z eta, true residual
+            if (threadIdx.x == 0) {
+                for (int i = maxIt - 1; i >= 0; i--) {
+                    c64 e = S.g[i];
+                    for (int q = i + 1; q < maxIt; q++) e -= S.R[i][q] * S.eta[q];
+                    S.eta[i] = cdiv(e, S.R[i][i]);
+                }
             }
+            __syncthreads();
+            for (int e = threadIdx.x; e < n; e += CT) {
+                c64 s = cvt<double>(x[e]);
+                for (int q = 0; q < maxIt; q++) s += S.eta[q] * cvt<double>(Z(q)[e]);
+                x[e] = cvt<R>(s);
+            }
+            c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
+            err = sqrt(c_dot(K.S, r, r, n).re);
+            if (err < tol * nb) break;
+            if (++k < restarts) {
+                pc = 1;
+                continue;
+            }
+            break;
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < n; e += CT) {
-            c64 s = cvt<double>(x[e]);
-            for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(Z(j)[e]);
-            x[e] = cvt<R>(s);
-        }
-        c_dop(A, LV, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
-        for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
-        err = sqrt(c_dot(K.S, r, r, n).re);
-        if (err < tol * nb) return;
     }
+    if (threadIdx.x == 0) frame_pop(K.S);
 }
 
 // k_cycle(l, eta, psi) (src/amg.cpp:135-178), or v_cycle(l, eta, psi)
 // (src/amg.cpp:90-132) when A.vcycle: the coarse correction is then a
 // recursive cycle from a zero guess instead of the level's K-cycle FGMRES.
-template <typename R, int LV> __device__ __forceinline__ void c_kcycle(const CArgs<R>& A, CK<R>& K, const cplx<R>* eta, cplx<R>* psi) {
-    if (LV == A.L - 1) {
-        c_fgmres<R, LV, CRS_MAXM>(A, K, K.S.crs, true, eta, psi);
+// At the coarsest level the frame becomes the coarsest GMRES from x0 = eta.
+template <typename R> __device__ __forceinline__ void kc_run(const CArgs<R>& A, CK<R>& K, int l) {
+    CFrame& F = K.S.kcf[l];
+    const cplx<R>* eta = (const cplx<R>*)F.in;
+    cplx<R>* psi = (cplx<R>*)F.out;
+    const int pc = F.pc;
+    __syncthreads();  // every thread has read the frame before thread 0 rewrites it
+    if (l == A.L - 1) {
+        if (threadIdx.x == 0) {
+            K.S.crsf.in = eta;
+            K.S.crsf.out = psi;
+            K.S.crsf.pc = 0;
+            K.S.stk[K.S.sp] = FR_CRS * 16 + l;  // tail call
+        }
         return;
     }
-    if constexpr (LV + 1 < MAXL) {
-        const CLevel<R>& L = A.lv[LV];
-        const int n = L.g.ns * L.g.dof;
-        cplx<R>* Dpsi = K.W + L.kDpsi;
-        cplx<R>* r = K.W + L.kR;
-        cplx<R>* eta1 = K.W + L.kEta1;
-        cplx<R>* psi1 = K.W + L.kPsi1;
-        for (int ph = 0; ph < 2; ph++) {
-            if (ph == 1) {
-                // coarse-grid correction
-                c_dop(A, LV, psi, Dpsi, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
-                for (int e = threadIdx.x; e < n; e += CT) r[e] = eta[e] - Dpsi[e];
-                c_restrict(A, LV, r, eta1);
-                if (A.vcycle) {
-                    const int nc = A.lv[LV + 1].g.ns * A.lv[LV + 1].g.dof;
-                    for (int e = threadIdx.x; e < nc; e += CT) psi1[e] = czero<R>();
-                    __syncthreads();
-                    c_kcycle<R, LV + 1>(A, K, eta1, psi1);
-                } else {
-                    c_fgmres<R, LV + 1, KC_MAXM>(A, K, K.S.kc[LV + 1], false, eta1, psi1);
-                }
-                c_prolong(A, LV, psi1, psi, true);
-            }
-            const int nu = ph == 0 ? A.nu1 : A.nu2;  // pre- / post-smoothing
-            if (nu > 0) c_sap(A, K, LV, eta, psi, nu);
+    const CLevel<R>& L = A.lv[l];
+    const int n = L.g.ns * L.g.dof;
+    cplx<R>* Dpsi = K.W + L.kDpsi;
+    cplx<R>* rr = K.W + L.kR;
+    cplx<R>* eta1 = K.W + L.kEta1;
+    cplx<R>* psi1 = K.W + L.kPsi1;
+    if (pc == 0) {
+        if (A.nu1 > 0) c_sap(A, K, l, eta, psi, A.nu1);  // pre-smoothing
+        // coarse-grid correction
+        c_dop(A, l, psi, Dpsi, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+        for (int e = threadIdx.x; e < n; e += CT) rr[e] = eta[e] - Dpsi[e];
+        c_restrict(A, l, rr, eta1);
+        if (A.vcycle) {
+            const int nc = A.lv[l + 1].g.ns * A.lv[l + 1].g.dof;
+            for (int e = threadIdx.x; e < nc; e += CT) psi1[e] = czero<R>();
         }
+        if (threadIdx.x == 0) {
+            F.pc = 1;
+            CFrame& C = A.vcycle ? K.S.kcf[l + 1] : K.S.fg[l + 1];
+            C.in = eta1;
+            C.out = psi1;
+            C.pc = 0;
+            frame_push(K.S, A.vcycle ? FR_KC : FR_FG, l + 1);
+        }
+        return;
+    }
+    c_prolong(A, l, psi1, psi, true);
+    if (A.nu2 > 0) c_sap(A, K, l, eta, psi, A.nu2);  // post-smoothing
+    if (threadIdx.x == 0) frame_pop(K.S);
+}
+
+// Runs the frame stack until it is empty.
+template <typename R> __device__ __forceinline__ void c_run(const CArgs<R>& A, CK<R>& K) {
+    for (;;) {
+        __syncthreads();
+        const int sp = K.S.sp;
+        if (sp < 0) break;
+        const int top = K.S.stk[sp], kind = top >> 4, l = top & 15;
+        if (kind == FR_FG) fg_run<R, KC_MAXM, false>(A, K, l, K.S.kc[l], K.S.fg[l]);
+        else if (kind == FR_CRS) fg_run<R, CRS_MAXM, true>(A, K, l, K.S.crs, K.S.crsf);
+        else kc_run<R>(A, K, l);
     }
 }
 
-template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R>* __restrict__ Ap) {
-    const CArgs<R>& A = *Ap;
+// The arguments travel by value in the kernel-argument segment (no per-launch
+// copy, capturable in a HIP graph) and are read in place through the segment
+// pointer (scalar loads; a copy of the parameter would land in scratch).
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R> Aval) {
+    (void)Aval;
+    const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ CLds S;
     extern __shared__ char dyn_lds[];
     const int rhs = blockIdx.x;
@@ -637,7 +737,14 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
     const int l = A.level;
     switch (A.mode) {
         case CM_KC_SOLVE:  // FGMRES_k_cycle[1].fgmres(eta, eta, psi) (src/amg.cpp:161)
-            c_fgmres<R, 1, KC_MAXM>(A, K, S.kc[1], false, in, out);
+            if (threadIdx.x == 0) {
+                S.fg[1].in = in;
+                S.fg[1].out = out;
+                S.fg[1].pc = 0;
+                S.sp = -1;
+                frame_push(S, FR_FG, 1);
+            }
+            c_run(A, K);
             break;
         case CM_SAP: c_sap(A, K, l, in, out, A.nu); break;
         case CM_DOP: c_dop(A, l, in, out); break;
@@ -646,13 +753,14 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
         case CM_CYCLE: {  // the cycle at level l from psi = 0
             const int n = A.lv[l].g.ns * A.lv[l].g.dof;
             for (int e = threadIdx.x; e < n; e += CT) out[e] = czero<R>();
-            __syncthreads();
-            switch (l) {
-                case 1: c_kcycle<R, 1>(A, K, in, out); break;
-                case 2: c_kcycle<R, 2>(A, K, in, out); break;
-                case 3: c_kcycle<R, 3>(A, K, in, out); break;
-                case 4: c_kcycle<R, 4>(A, K, in, out); break;
+            if (threadIdx.x == 0) {
+                S.kcf[l].in = in;
+                S.kcf[l].out = out;
+                S.kcf[l].pc = 0;
+                S.sp = -1;
+                frame_push(S, FR_KC, l);
             }
+            c_run(A, K);
             break;
         }
     }

@@ -1,0 +1,53 @@
+// Host launcher of the persistent coarse-level kernel (levels >= 1), kept in
+// translation units of its own (one per precision) so that the large kernel
+// compiles in parallel with the rest of the library and only when it changes.
+#pragma once
+#include "engine.hpp"
+
+namespace amgs {
+
+template <typename R>
+void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out,
+                       size_t out_stride, int nu) {
+    if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
+    static_assert(sizeof(CArgs<R>) <= 4096, "coarse-kernel arguments exceed the kernel-argument segment");
+    CArgs<R> a = ca;
+    a.mm = R(c->P.m0 + 2.0);
+    for (int l = 1; l < c->L; l++) {
+        a.lv[l].links = links[l];
+        a.lv[l].w = l < c->L - 1 ? tv[l] : nullptr;
+    }
+    a.mode = mode;
+    a.level = level;
+    a.nu = nu;
+    // coarsest GMRES vectors in dynamic LDS if they fit beside the static part
+    const size_t ncrs = c->nvec(c->L - 1);
+    const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
+    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
+    a.crs_lds = dyn <= lds_cap ? 1 : 0;
+    size_t dyn_bytes = a.crs_lds ? dyn : 0;
+    // SAP block vectors of a level share that region (never live at the same time)
+    for (int l = 0; l < MAXL; l++) {
+        const size_t need = l >= 1 && l < c->L - 1 ? CW * a.lv[l].blkStride * sizeof(cplx<R>) : 0;
+        a.sap_lds[l] = need > 0 && need <= lds_cap ? 1 : 0;
+        if (a.sap_lds[l]) dyn_bytes = std::max(dyn_bytes, need);
+    }
+    a.prof = prof.p ? prof.as<unsigned long long>() : nullptr;
+    a.in = in;
+    a.out = out;
+    a.in_stride = in_stride;
+    a.out_stride = out_stride;
+    if (dyn_bytes > 0) {
+        static bool attr_set[2] = {false, false};
+        bool& done = attr_set[sizeof(R) == 8];
+        if (!done) {
+            HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(160 * 1024 - sizeof(CLds))));
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace amgs

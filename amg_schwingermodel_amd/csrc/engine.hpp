@@ -1,0 +1,191 @@
+// Host-side state of libamgs shared by its translation units: error types,
+// device buffers, per-launch kernel timers, the per-precision engine (fine
+// SAP, K-cycle at level 0, coarse-kernel arguments) and the context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/amgs.h"
+#include "coarse.hpp"
+#include "common.hpp"
+#include "layout.hpp"
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t e, const char* what, const char* file, int line)
+        : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")"),
+          code(e) {}
+};
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+namespace amgs {
+
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
+        bytes = std::max<size_t>(b, 256);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+static inline unsigned cdiv_u(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+struct Ctx;
+
+// ---------------------------------------------------------------------------
+// Per-launch timing of one kernel class inside solves: a HIP event pair on the
+// library stream around every launch, and a device counter of the items that
+// did work in that launch (for the algorithmic byte count).
+// ---------------------------------------------------------------------------
+struct KTimer {
+    static constexpr int CAP = 8192;  // launches recorded between two resolves
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> fixed, per_item;  // bytes = fixed + per_item * active items
+    DevBuf actlog;
+    long long launches = 0;
+    double ms = 0, bytes = 0;
+
+    ~KTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void reset() {
+        launches = 0;
+        ms = bytes = 0;
+        fixed.clear();
+        per_item.clear();
+    }
+    // returns the counter for this launch, or nullptr when not recording
+    int* begin(hipStream_t s, double fx, double pi) {
+        if (!on || (int)fixed.size() >= CAP) return nullptr;
+        if (!actlog.p) {
+            actlog.alloc(sizeof(int) * CAP);
+            HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+        }
+        const size_t i = fixed.size();
+        while (ev.size() < 2 * i + 2) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        HIP_CHECK(hipEventRecord(ev[2 * i], s));
+        fixed.push_back(fx);
+        per_item.push_back(pi);
+        return actlog.as<int>() + i;
+    }
+    void end(hipStream_t s) { HIP_CHECK(hipEventRecord(ev[2 * (fixed.size() - 1) + 1], s)); }
+    // fold the pending records into the totals (synchronises the stream)
+    void resolve(hipStream_t s) {
+        if (fixed.empty()) return;
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<int> act(fixed.size());
+        HIP_CHECK(hipMemcpy(act.data(), actlog.p, sizeof(int) * act.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < fixed.size(); i++) {
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            ms += t;
+            launches++;
+            if (act[i] > 0) bytes += fixed[i] + per_item[i] * act[i];
+        }
+        HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+        fixed.clear();
+        per_item.clear();
+    }
+};
+enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_CLASSES = 2 };
+
+// ---------------------------------------------------------------------------
+// per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments
+// ---------------------------------------------------------------------------
+template <typename R> struct Engine {
+    Ctx* c = nullptr;
+    int cap = 0;
+    // hierarchy in precision R
+    const cplx<R>* U = nullptr;
+    const cplx<R>* tv[MAXL] = {};
+    const cplx<R>* links[MAXL] = {};
+    // fine SAP workspace
+    DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero;
+    SapState st{};
+    int nchunk_max = 0;
+    // K-cycle(0) temporaries
+    DevBuf eta1, psi1, res0;
+    // coarse workspace; the kernel arguments (ca + per-launch fields) travel by value
+    DevBuf cws, prof;
+    CArgs<R> ca{};
+
+    void init(Ctx* ctx, int capacity);
+    void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
+    void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi, bool from_zero = true);
+    void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
+                int nu = 0);
+    void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
+};
+
+struct Ctx {
+    amgs_params P{};
+    int L = 0;
+    Geo geo[MAXL]{};
+    std::vector<int> red[MAXL], black[MAXL];
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    bool gauge_set = false, setup_done = false;
+    // FP64 hierarchy (always), FP32 copy in mixed mode
+    DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], redl[MAXL], blackl[MAXL];
+    Engine<double> e64;
+    Engine<float> e32;
+    // outer FGMRES
+    DevBuf oV, oZ, ox, ob, or_, ow, ost_mem, oeta32, opsi32;
+    OuterState ost{};
+    int ocap = 0;
+    // io staging
+    DevBuf io_a, io_b;
+    // comparison solvers: Krylov vectors, partial sums
+    DevBuf kbuf, kpart, ksum;
+    int outer_prec = 0;  // outer FGMRES preconditioner: 0 multigrid cycle, 1 one SAP sweep, 2 none
+    double last_ms[3] = {0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    KTimer kt[KT_CLASSES];
+    // HIP-graph replay of the level-0 preconditioner, one executable graph per
+    // batch size; FP64 inputs/outputs are staged through fixed buffers
+    bool graph = false;
+    std::vector<std::pair<int, hipGraphExec_t>> pgraphs;
+    DevBuf oeta64, opsi64;
+    void clear_graphs() {
+        for (auto& g : pgraphs) (void)hipGraphExecDestroy(g.second);
+        pgraphs.clear();
+    }
+
+    size_t nvec(int l) const { return (size_t)geo[l].ns * geo[l].dof; }
+    // FP32 inner levels (AMGS_PREC_MIXED, AMGS_PREC_MIXED16)
+    bool mixed() const { return P.precision != AMGS_PREC_FP64; }
+    // FP16 Krylov basis in the level-0 SAP block GMRES (AMGS_PREC_MIXED16)
+    bool h16() const { return P.precision == AMGS_PREC_MIXED16; }
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+}  // namespace amgs

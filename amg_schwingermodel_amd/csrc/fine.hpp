@@ -4,12 +4,10 @@
 // 256 threads per workgroup; the right-hand side is blockIdx.y.
 #pragma once
 #include "common.hpp"
+#include "layout.hpp"
 
 namespace amgs {
 
-constexpr int FT = 256;       // threads per workgroup for fine-lattice kernels
-constexpr int SAP_MAXM = 8;   // max restart length of the SAP block GMRES
-constexpr int OUT_MAXM = 32;  // max restart length of the outer FGMRES
 
 // Workgroup -> tile map that keeps consecutive tiles on one XCD: the hardware
 // deals workgroups round-robin over the 8 XCDs, so tile = (wg % 8) * per + wg / 8
@@ -30,32 +28,63 @@ __device__ __forceinline__ int xcd_tile(int wg, int nwg) {
 //   (for D^dagger the same with the forward/backward hops exchanged).
 // MASK(n') says whether neighbour n' contributes (zero Dirichlet for SAP blocks).
 // ---------------------------------------------------------------------------
-template <typename R, bool DAG, typename Mask>
-__device__ __forceinline__ void wilson_site(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
-                                            int n, int nx, int nt, R mm, Mask mask, cplx<R>& o0, cplx<R>& o1) {
+// Site readers: the two spin components of site n of one vector.
+template <typename R> struct SpinPtr {  // complex FP32/FP64 storage [ns][2]
+    const cplx<R>* p;
+    __device__ __forceinline__ void operator()(int n, cplx<R>& a, cplx<R>& b) const {
+        a = p[2 * n];
+        b = p[2 * n + 1];
+    }
+};
+// FP16 storage: (re0, im0, re1, im1) of a site in 8 bytes; values are read as
+// stored (the power-of-two scale lives in the Krylov scale factors)
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+struct SpinH16 {
+    const f16x4* p;
+    __device__ __forceinline__ void operator()(int n, c32& a, c32& b) const {
+        const f16x4 v = p[n];
+        a = c32{float(v.x), float(v.y)};
+        b = c32{float(v.z), float(v.w)};
+    }
+};
+__device__ __forceinline__ f16x4 to_h16(c32 a, c32 b) {
+    return f16x4{(_Float16)a.re, (_Float16)a.im, (_Float16)b.re, (_Float16)b.im};
+}
+
+template <typename R, bool DAG, typename Mask, typename In>
+__device__ __forceinline__ void wilson_site_in(const cplx<R>* __restrict__ U, In in, int n, int nx, int nt, R mm,
+                                               Mask mask, cplx<R>& o0, cplx<R>& o1) {
     const int r0 = nb_right(n, 0, nx, nt), r1 = nb_right(n, 1, nx, nt);
     const int l0 = nb_left(n, 0, nx, nt), l1 = nb_left(n, 1, nx, nt);
-    const cplx<R> c0 = in[2 * n], c1 = in[2 * n + 1];
+    cplx<R> c0, c1;
+    in(n, c0, c1);
     cplx<R> f0 = U[2 * n], f1 = U[2 * n + 1];
     cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
     if (wraps_right(n, 0, nt)) f0 = -f0;
     if (wraps_left(n, 0, nt)) b0 = -b0;
     cplx<R> A = czero<R>(), B = czero<R>(), C = czero<R>(), E = czero<R>();
+    cplx<R> p0, p1;
     // forward/backward neighbour reads
     if (!DAG) {
-        if (mask(r0)) { cplx<R> p0 = in[2 * r0], p1 = in[2 * r0 + 1]; A = f0 * (p0 - p1); }
-        if (mask(r1)) { cplx<R> p0 = in[2 * r1], p1 = in[2 * r1 + 1]; B = f1 * (p0 + mul_i(p1)); }
-        if (mask(l0)) { cplx<R> p0 = in[2 * l0], p1 = in[2 * l0 + 1]; C = b0 * (p0 + p1); }
-        if (mask(l1)) { cplx<R> p0 = in[2 * l1], p1 = in[2 * l1 + 1]; E = b1 * (p0 - mul_i(p1)); }
+        if (mask(r0)) { in(r0, p0, p1); A = f0 * (p0 - p1); }
+        if (mask(r1)) { in(r1, p0, p1); B = f1 * (p0 + mul_i(p1)); }
+        if (mask(l0)) { in(l0, p0, p1); C = b0 * (p0 + p1); }
+        if (mask(l1)) { in(l1, p0, p1); E = b1 * (p0 - mul_i(p1)); }
     } else {
-        if (mask(l0)) { cplx<R> p0 = in[2 * l0], p1 = in[2 * l0 + 1]; A = b0 * (p0 - p1); }
-        if (mask(l1)) { cplx<R> p0 = in[2 * l1], p1 = in[2 * l1 + 1]; B = b1 * (p0 + mul_i(p1)); }
-        if (mask(r0)) { cplx<R> p0 = in[2 * r0], p1 = in[2 * r0 + 1]; C = f0 * (p0 + p1); }
-        if (mask(r1)) { cplx<R> p0 = in[2 * r1], p1 = in[2 * r1 + 1]; E = f1 * (p0 - mul_i(p1)); }
+        if (mask(l0)) { in(l0, p0, p1); A = b0 * (p0 - p1); }
+        if (mask(l1)) { in(l1, p0, p1); B = b1 * (p0 + mul_i(p1)); }
+        if (mask(r0)) { in(r0, p0, p1); C = f0 * (p0 + p1); }
+        if (mask(r1)) { in(r1, p0, p1); E = f1 * (p0 - mul_i(p1)); }
     }
     const R h = R(0.5);
     o0 = mm * c0 - h * (A + B + C + E);
     o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+}
+
+template <typename R, bool DAG, typename Mask>
+__device__ __forceinline__ void wilson_site(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
+                                            int n, int nx, int nt, R mm, Mask mask, cplx<R>& o0, cplx<R>& o1) {
+    wilson_site_in<R, DAG>(U, SpinPtr<R>{in}, n, nx, nt, mm, mask, o0, o1);
 }
 
 struct AllSites {
@@ -132,17 +161,6 @@ __global__ void __launch_bounds__(FT) k_resid(Geo g, R mm, const cplx<R>* __rest
 // in exact arithmetic.  Basis vectors are stored unnormalised (Vt_j) with
 // V_j = s_j Vt_j, s_j = 1/H_{j,j-1}, so no separate normalisation pass runs.
 // ---------------------------------------------------------------------------
-struct SapState {
-    int *maxIt, *jbreak, *done;
-    double *normb, *s;          // s: [item][M+1]
-    c64 *Hraw, *Hrot;           // [item][(M+1)*M]  (row i, column j at i*M+j)
-    c64 *G;                     // [item][(M+1)*(M+1)]  G[i][k] = <V_i, V_k>, k < i
-    c64 *gpre, *gpost, *cn, *sn;
-    c64 *qa;                    // [item][nchunk][2M]   <w,Vt_i>, <Vt_j,Vt_k>
-    double *qb;                 // [item][nchunk]       |w|^2
-    double *qr;                 // [item][nchunk][2]    |r|^2, |b|^2
-};
-
 template <typename R> struct SapArgs {
     Geo g;
     R mm;
@@ -151,7 +169,9 @@ template <typename R> struct SapArgs {
     int ncol, nchunk, cs, nrhs, m;  // cs: sites per chunk (a multiple of FT)
     double tol;
     const cplx<R>* rglob;  // b of each item = global residual restricted to the block
-    cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]
+    cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]; with FP16 storage only slot 0 (V_0) and slot 1 (w)
+    f16x4* Vh;             // FP16 storage: slots 1..m of [nrhs][ns] (both spins of a site in 8 bytes)
+    float sig;             // FP16 storage: power-of-two scale of the stored slots (1 otherwise)
     cplx<R>* xl;           // [nrhs][ns][2] block solutions
     const int* skip;       // per-rhs: SAP already converged
     SapState st;
@@ -175,6 +195,21 @@ template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R
 
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
     return a.Vt + ((size_t)j * a.nrhs + rhs) * a.g.ns * 2;
+}
+// FP16 slot j >= 1
+template <typename R> __device__ __forceinline__ f16x4* hslot(const SapArgs<R>& a, int j, int rhs) {
+    return a.Vh + ((size_t)(j - 1) * a.nrhs + rhs) * a.g.ns;
+}
+// both spins of Krylov slot i at site n: FP16 slots 1..m when H, else complex R
+template <typename R, bool H>
+__device__ __forceinline__ void ld_slot(const SapArgs<R>& a, int i, int rhs, int n, cplx<R>& b0, cplx<R>& b1) {
+    if constexpr (H) {
+        if (i > 0) {
+            SpinH16{hslot(a, i, rhs)}(n, b0, b1);
+            return;
+        }
+    }
+    SpinPtr<R>{slot(a, i, rhs)}(n, b0, b1);
 }
 
 // Per-item scalar state held in LDS during a kernel (matrices with row
@@ -242,9 +277,11 @@ __device__ __forceinline__ void reduce_chunks(const T* base, int stride, int nq,
 // Finalise Arnoldi step j (src/fgmres.cpp:35-49): H_{j+1,j} = |w|, Givens
 // rotation of column j, residual estimate, break test.  Thread 0 on LDS
 // state; chunk 0 also stores the results.
+// hn2 is |sig w|^2 of the stored vector: H_{j+1,j} = |w| = sqrt(hn2) / sig and the
+// scale of the stored vector is s_{j+1} = 1 / sqrt(hn2).
 __device__ void sap_finalize_step(const SapState& st, int item, double hn2, int M, int j, double tol, bool store,
-                                  ItemLds& L) {
-    const double hn = sqrt(hn2);
+                                  ItemLds& L, double sig) {
+    const double hs = sqrt(hn2), hn = hs / sig;
     c64* col = L.Hcol;
     for (int i = 0; i <= j; i++) col[i] = L.Hraw[i * M + j];
     col[j + 1] = c64{hn, 0.0};
@@ -266,7 +303,7 @@ __device__ void sap_finalize_step(const SapState& st, int item, double hn2, int 
     L.sn[j] = snj;
     L.gpost[j] = gp;
     L.gpre[j + 1] = gn;
-    L.s[j + 1] = hn > 0 ? 1.0 / hn : 0.0;
+    L.s[j + 1] = hs > 0 ? 1.0 / hs : 0.0;
     L.broke = broke;
     if (store) {
         c64* Ro = st.Hrot + (size_t)item * (M + 1) * M;
@@ -324,8 +361,9 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
 }
 
 // Arnoldi step J, part A: prologue finalises step J-1 (or starts cycle k),
-// body w = s_J D_B Vt_J -> Vt_{J+1}; partials <w, Vt_i> (i <= J), <Vt_J, Vt_i> (i < J).
-template <typename R, int J>
+// body w = s_J D_B Vt_J -> Vt_{J+1} (slot 1 with FP16 storage H);
+// partials <w, Vt_i> (i <= J), <Vt_J, Vt_i> (i < J).
+template <typename R, int J, bool H>
 __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
@@ -361,7 +399,7 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
         } else if (active) {
             if (L.jbreak >= 0) active = 0;
             else {
-                sap_finalize_step(st, t.item, L.dr[0], M, J - 1, a.tol, store, L);
+                sap_finalize_step(st, t.item, L.dr[0], M, J - 1, a.tol, store, L, H ? a.sig : 1.0);
                 if (L.broke) active = 0;
             }
         }
@@ -371,8 +409,7 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
     __syncthreads();
     if (!L.active) return;
     const R sj = R(L.s[J]);
-    const cplx<R>* vj = slot(a, J, t.rhs);
-    cplx<R>* w = slot(a, J + 1, t.rhs);
+    cplx<R>* w = slot(a, H ? 1 : J + 1, t.rhs);
     // per-thread partial sums in the working precision, combined in FP64
     cplx<R> pw[J + 1], pv[J + 1];
 #pragma unroll
@@ -381,16 +418,21 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
         cplx<R> o0, o1;
-        wilson_site<R, false>(a.U, vj, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        if constexpr (H && J > 0)
+            wilson_site_in<R, false>(a.U, SpinH16{hslot(a, J, t.rhs)}, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0,
+                                     o1);
+        else
+            wilson_site<R, false>(a.U, slot(a, J, t.rhs), n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
         o0 = sj * o0;
         o1 = sj * o1;
         w[2 * n] = o0;
         w[2 * n + 1] = o1;
-        const cplx<R> a0 = vj[2 * n], a1 = vj[2 * n + 1];
+        cplx<R> a0, a1;
+        ld_slot<R, H>(a, J, t.rhs, n, a0, a1);
 #pragma unroll
         for (int i = 0; i <= J; i++) {
-            const cplx<R>* vi = slot(a, i, t.rhs);
-            const cplx<R> b0 = vi[2 * n], b1 = vi[2 * n + 1];
+            cplx<R> b0, b1;
+            ld_slot<R, H>(a, i, t.rhs, n, b0, b1);
             pw[i] += mulc(o0, b0) + mulc(o1, b1);
             if (i < J) pv[i] += mulc(a0, b0) + mulc(a1, b1);
         }
@@ -409,8 +451,9 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
 }
 
 // Arnoldi step j, part B: prologue builds Hessenberg column j (one-pass MGS),
-// body w -= sum_i H_ij V_i (in place in Vt_{j+1}); partial |w|^2.
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j, int* act) {
+// body w -= sum_i H_ij V_i (in place in Vt_{j+1}; with FP16 storage H from
+// slot 1 into FP16 slot j+1 as sig w); partial |w|^2 of the stored values.
+template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapArgs<R> a, int j, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
@@ -442,21 +485,30 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
     }
     __syncthreads();
     if (!L.active) return;
-    cplx<R>* w = slot(a, j + 1, t.rhs);
+    cplx<R>* w = slot(a, H ? 1 : j + 1, t.rhs);
     double nn = 0;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = sap_site(a.g, t.blk, m);
         c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
         for (int i = 0; i <= j; i++) {
-            const cplx<R>* vi = slot(a, i, t.rhs);
+            cplx<R> v0, v1;
+            ld_slot<R, H>(a, i, t.rhs, n, v0, v1);
             const c64 c = L.eta[i];
-            w0 -= c * cvt<double>(vi[2 * n]);
-            w1 -= c * cvt<double>(vi[2 * n + 1]);
+            w0 -= c * cvt<double>(v0);
+            w1 -= c * cvt<double>(v1);
         }
-        const cplx<R> r0 = cvt<R>(w0), r1 = cvt<R>(w1);
-        w[2 * n] = r0;
-        w[2 * n + 1] = r1;
-        nn += double(norm2(r0)) + double(norm2(r1));
+        if constexpr (H) {
+            const double sg = a.sig;
+            const f16x4 h = to_h16(c32{float(sg * w0.re), float(sg * w0.im)}, c32{float(sg * w1.re), float(sg * w1.im)});
+            hslot(a, j + 1, t.rhs)[n] = h;
+            nn += double(h.x) * double(h.x) + double(h.y) * double(h.y) + double(h.z) * double(h.z) +
+                  double(h.w) * double(h.w);
+        } else {
+            const cplx<R> r0 = cvt<R>(w0), r1 = cvt<R>(w1);
+            w[2 * n] = r0;
+            w[2 * n + 1] = r1;
+            nn += double(norm2(r0)) + double(norm2(r1));
+        }
     }
     c64 q[1] = {{nn, 0.0}};
     block_sum<1>(q, L.tmp);
@@ -467,7 +519,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_arnoldi_b(SapA
 // (src/fgmres.cpp:52-59, 107-115) and update x_B += sum_j eta_j V_j.  With
 // xacc set (last cycle) the block solution is added into the global x
 // instead (src/sap.cpp:117-125), also for items that converged earlier.
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a, cplx<R>* xacc) {
+template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a, cplx<R>* xacc) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
@@ -480,7 +532,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
         if (active) {
             maxIt = L.maxIt;
             if (L.jbreak < 0) {
-                sap_finalize_step(a.st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L);
+                sap_finalize_step(a.st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L, H ? a.sig : 1.0);
                 if (L.broke) maxIt = M;
             }
             for (int i = maxIt - 1; i >= 0; i--) {
@@ -501,10 +553,11 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs
         const int n = sap_site(a.g, t.blk, m);
         c64 x0 = cvt<double>(a.xl[off + 2 * n]), x1 = cvt<double>(a.xl[off + 2 * n + 1]);
         for (int jj = 0; jj < mi; jj++) {
-            const cplx<R>* vj = slot(a, jj, t.rhs);
+            cplx<R> v0, v1;
+            ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
             const c64 c = L.eta[jj];
-            x0 += c * cvt<double>(vj[2 * n]);
-            x1 += c * cvt<double>(vj[2 * n + 1]);
+            x0 += c * cvt<double>(v0);
+            x1 += c * cvt<double>(v1);
         }
         if (xacc) {
             xacc[off + 2 * n] += cvt<R>(x0);
@@ -690,6 +743,12 @@ __global__ void k_lin(size_t n, double a, const cplx<R>* x, double b, const cplx
 template <typename R> __global__ void k_zero(size_t n, cplx<R>* out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = czero<R>();
+}
+// int flags to zero (a kernel rather than a memset node, so that a captured
+// K-cycle replays correctly)
+__global__ void k_zero_int(int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 0;
 }
 
 }  // namespace amgs

@@ -7,16 +7,6 @@
 
 namespace amgs {
 
-struct OuterState {
-    int *maxIt, *jbreak, *done, *active, *iters, *nhist;
-    double *normb, *s, *hist;  // s [rhs][M+1]; hist [rhs][hcap]
-    c64 *R, *G, *g, *cn, *sn, *eta;  // R [rhs][M][M], G [rhs][M+1][M+1]
-    c64* qa;     // [rhs][ntile][2M+1]
-    double* qb;  // [rhs][ntile]
-    double* qr;  // [rhs][ntile][2]
-    int M, ntile, hcap;
-};
-
 // r = b - D x with partials |r|^2, |b|^2 (src/fgmres.cpp:10-11, 60-62)
 __global__ void __launch_bounds__(FT) k_outer_resid(Geo g, double mm, const c64* __restrict__ U,
                                                     const c64* __restrict__ b, const c64* __restrict__ x,

@@ -2,8 +2,10 @@
 """Benchmark: time per right-hand side to a 1e-10 FP64 relative residual on a
 512x512 lattice with a 3-level K-cycle DD-alpha-AMG preconditioner: the mixed-
 precision batched solve (32 right-hand sides at once; SAP, interpolation and
-coarse levels in FP32, outer FGMRES in FP64) next to the plain FP64 GPU solve
-(one right-hand side), on one MI355X.
+coarse levels in FP32, the level-0 SAP Krylov basis in FP16 with a power-of-two
+scale (--precision mixed16, default) or in FP32 (--precision mixed), outer
+FGMRES in FP64; the level-0 preconditioner replayed from a captured HIP graph)
+next to the plain FP64 GPU solve (one right-hand side), on one MI355X.
 
 Workload: quenched U(1) links from 500 Metropolis sweeps of the plaquette
 action at beta = 3 (cold start, seeded), parameters.dat rows "0 8 8 10 4 4"
@@ -60,6 +62,12 @@ def parse():
                     help="outer iterations of the sample at the named mass m0=-0.20 (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="outer iterations of the CPU-baseline sample (0: the full 1-rhs solve to 1e-10)")
+    ap.add_argument("--precision", choices=["mixed", "mixed16"], default="mixed16",
+                    help="inner arithmetic of the batched solve: FP32 (mixed) or FP32 with an FP16 Krylov "
+                         "basis in the level-0 SAP block GMRES (mixed16)")
+    ap.add_argument("--no-alt", action="store_true", help="skip timing the other inner precision beside it")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: replay the level-0 preconditioner from a captured HIP graph in the timed solves")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -118,6 +126,7 @@ def main():
         t = time.perf_counter()
         sp.setup()
         setup_plain = (time.perf_counter() - t) * 1e3
+        sp.set_graph(bool(a.graph))
         ms_plain, it_plain, rr_plain = timed(sp, 1, a.steps, a.warmup, "plain fp64 1 rhs")
         log(f"plain FP64: setup {setup_plain:.0f} ms, solve {ms_plain:.1f} ms, iterations {it_plain[0]}, "
             f"relres {rr_plain[0]:.2e}")
@@ -127,26 +136,41 @@ def main():
         sp.close()
 
     # ---- mixed precision, batched, to 1e-10 (the timed value) ---------------
-    sm = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=a.rhs, precision=amgs.PREC_MIXED), U)
+    precs = {"mixed": amgs.PREC_MIXED, "mixed16": amgs.PREC_MIXED16}
+    if not a.no_alt:
+        alt = "mixed" if a.precision == "mixed16" else "mixed16"
+        sa = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=a.rhs, precision=precs[alt]), U)
+        sa.setup()
+        sa.set_graph(bool(a.graph))
+        ms_alt, its_alt, rr_alt = timed(sa, a.rhs, a.steps, a.warmup, f"{alt} {a.rhs} rhs")
+        out[f"{alt}_batched_alt"] = {"ms_per_rhs": round(ms_alt / a.rhs, 3), "iterations_min": int(its_alt.min()),
+                                     "iterations_max": int(its_alt.max()), "relres_max": float(rr_alt.max())}
+        sa.close()
+    sm = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=a.rhs, precision=precs[a.precision]), U)
     t = time.perf_counter()
     sm.setup()
     setup_mixed = (time.perf_counter() - t) * 1e3
-    ms_step, its, rrs = timed(sm, a.rhs, a.steps, a.warmup, f"mixed {a.rhs} rhs",
-                              on_start=lambda: sm.kernel_timing(0, True))
+    sm.set_graph(bool(a.graph))
+    ms_step, its, rrs = timed(sm, a.rhs, a.steps, a.warmup, f"mixed {a.rhs} rhs")
+    ms_rhs = ms_step / a.rhs
+    # roofline pass: the same batched solve with every launch of SAP step A
+    # bracketed by HIP events on the library stream (eager launches)
+    sm.kernel_timing(0, True)
+    ms_roof, _, _ = timed(sm, a.rhs, 1, 0, f"mixed {a.rhs} rhs, kernel timing")
     n_launch, k_ms, k_bytes = sm.kernel_stats(0)
     sm.kernel_timing(0, False)
-    ms_rhs = ms_step / a.rhs
     log(f"mixed batched: setup {setup_mixed:.0f} ms, solve {ms_step:.1f} ms for {a.rhs} rhs -> {ms_rhs:.2f} ms/rhs, "
         f"iterations {its.min()}..{its.max()}, max relres {rrs.max():.2e}")
-    mixed = {"setup_ms": round(setup_mixed, 1), "iterations_min": int(its.min()), "iterations_max": int(its.max()),
-             "relres_max": float(rrs.max())}
+    mixed = {"precision": a.precision, "graph": bool(a.graph), "setup_ms": round(setup_mixed, 1),
+             "iterations_min": int(its.min()), "iterations_max": int(its.max()), "relres_max": float(rrs.max()),
+             "eager_timed_solve_ms_per_rhs": round(ms_roof / a.rhs, 3)}
     if not a.no_plain:
         x0 = Xd[0].cpu().numpy().view(np.complex128)
         mixed["x_relerr_vs_fp64_rhs0"] = float(np.linalg.norm(x0 - xs_plain) / np.linalg.norm(xs_plain))
         out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] / ms_rhs, 2)
     out["mixed_batched"] = mixed
 
-    # ---- roofline: SAP Arnoldi step A over the timed solves -----------------
+    # ---- roofline: SAP Arnoldi step A over the kernel-timing solve ----------
     avg_us = k_ms * 1e3 / max(n_launch, 1)
     bytes_per_launch = k_bytes / max(n_launch, 1)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
@@ -164,7 +188,7 @@ def main():
     # ---- the named mass m0 = -0.20: bounded sample --------------------------
     if a.exact_iters > 0:
         ex = {"m0": M0_NAMED, "outer_iterations": a.exact_iters}
-        for prec, nrhs, tag in ((amgs.PREC_MIXED, a.rhs, "mixed_batched"), (amgs.PREC_FP64, 1, "plain_fp64")):
+        for prec, nrhs, tag in ((precs[a.precision], a.rhs, "mixed_batched"), (amgs.PREC_FP64, 1, "plain_fp64")):
             if tag == "plain_fp64" and a.no_plain:
                 continue
             s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=M0_NAMED, max_rhs=nrhs, precision=prec,
@@ -206,7 +230,8 @@ def main():
 
     res = {"metric": METRIC, "value": round(ms_rhs, 3), "unit": "ms/RHS", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": False, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "vs_baseline": None,
+           "dtype": "f32+f64" if a.precision == "mixed" else "f16+f32+f64", "data": "synthetic",
            "config": {"workload": f"{nx}x{nt} 3-level K-cycle FGMRES(20,50) to 1e-10, {a.rhs} Gaussian rhs "
                                   f"batched, beta={BETA} Metropolis ({SWEEPS} sweeps), m0={a.m0} "
                                   f"(named m0={M0_NAMED} stagnates; sampled in named_mass_sample), "

@@ -47,6 +47,9 @@ extern "C" {
  * least-squares problem and the true-residual check are FP64 in every mode. */
 #define AMGS_PREC_FP64 0   /* plain FP64 everywhere: the drop-in */
 #define AMGS_PREC_MIXED 1  /* SAP, P/P^dagger and coarse levels in FP32 */
+#define AMGS_PREC_MIXED16 2 /* as MIXED, and the level-0 SAP block GMRES keeps its
+                             * Krylov basis in FP16 with a power-of-two scale
+                             * (FP32 arithmetic, FP64 scalars) */
 
 typedef struct {
     /* lattice and hierarchy: CMakeLists.txt:22-27 (NS, NT, LEVELS) and the
@@ -109,6 +112,27 @@ int amgs_solve(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, do
                double* hist, int hist_cap, int* nhist);
 int amgs_solve_dev(amgs_ctx* c, int nrhs, const void* b_dev, void* x_dev, int* iters, double* relres);
 
+/* HIP-graph replay of the preconditioner (new): with enable != 0, later
+ * solves capture the level-0 multigrid cycle (restriction, the persistent
+ * coarse-level kernel, prolongation, SAP; several hundred launches) once per
+ * batch size into a HIP graph and replay it in every outer iteration.  Results
+ * are identical to the eager launches.  Graphs are rebuilt after a new set-up;
+ * solves with kernel timing enabled (amgs_kernel_timing) launch eagerly. */
+int amgs_set_graph(amgs_ctx* c, int enable);
+
+/* Uneven multi-configuration batch (new; the reference solves one gauge
+ * configuration per process): the solves of nctx independent contexts - each
+ * with its own lattice size, blocking, mass and gauge field, set up
+ * beforehand - run concurrently, one host thread per context, their kernels
+ * overlapping on the device.  Context i solves nrhs[i] right-hand sides from
+ * device buffers b_dev[i] into x_dev[i] exactly as amgs_solve_dev would;
+ * iters and relres are concatenated over the contexts in order, status[i] is
+ * that context's return value.  A context may appear only once.  Returns the
+ * first negative status, else AMGS_NOT_CONVERGED if any did not converge,
+ * else AMGS_OK. */
+int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev, void* const* x_dev,
+                     int* iters, double* relres, int* status);
+
 /* ---- component entry points (the reference's building blocks, used by the
  *      parity tests and by callers that assemble their own solvers) ---- */
 
@@ -156,12 +180,18 @@ int amgs_gmres(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, do
 
 /* ---- fermion force (new; no counterpart in the reference, which stops at
  *      the solve; the D^dagger it needs is D_dagger_phi, src/dirac_operator.cpp:30-54) ----
- * S = phi^dagger (D D^dagger)^-1 phi and dS/dtheta_mu(n) for U_mu(n) = exp(i theta_mu(n)),
- * by the adjoint method: Y = D^-1 phi, X = (D^dagger)^-1 Y (two multigrid
- * solves), dS/dtheta = -2 Re X^dagger (dD/dtheta) Y.  phi [nrhs][nx*nt][2]
- * complex; dsdtheta [nrhs][nx*nt][2] real (mu = 0 time, 1 space); action[nrhs].
+ * The pseudofermion action and its derivative dS/dtheta_mu(n) for
+ * U_mu(n) = exp(i theta_mu(n)), by the adjoint method (two multigrid solves):
+ *   form AMGS_FORCE_DDAG_D: S = phi^dagger (D^dagger D)^-1 phi,
+ *   form AMGS_FORCE_D_DDAG: S = phi^dagger (D D^dagger)^-1 phi
+ *     = |Y|^2 with Y = D^-1 phi, X = (D^dagger)^-1 Y, dS/dtheta = -2 Re X^dagger (dD/dtheta) Y.
+ * Since D^dagger = s3 D s3, (D^dagger D)^-1 = s3 (D D^dagger)^-1 s3: the first
+ * form is the second at s3 phi.  phi [nrhs][nx*nt][2] complex; dsdtheta
+ * [nrhs][nx*nt][2] real (mu = 0 time, 1 space); action[nrhs].
  * Needs amgs_setup; nrhs <= min(max_rhs, 32). */
-int amgs_force(amgs_ctx* c, int nrhs, const double* phi, double* dsdtheta, double* action);
+#define AMGS_FORCE_DDAG_D 0
+#define AMGS_FORCE_D_DDAG 1
+int amgs_force(amgs_ctx* c, int nrhs, int form, const double* phi, double* dsdtheta, double* action);
 
 /* Level::interpolator_columns and the coarse links G1/G2/G3 (include/level.h:143-164).
  * tv: [nsites_l][dof_l] of test vector i;  which 0/1/2 = A/B_mu/C_mu as

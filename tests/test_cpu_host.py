@@ -255,22 +255,27 @@ def test_comparison_solvers_against_dense():
         np.testing.assert_allclose(x, xe, atol=1e-8)
 
 
-def test_force_against_finite_differences():
-    """dS/dtheta of S = phi^+ (D D^+)^-1 phi from the oracle's adjoint
-    formula against central differences (h = 1e-5) of S from dense solves,
-    on every link of an 8x8 lattice (pins the formula, signs and the
-    antiperiodic boundary)."""
+@pytest.mark.parametrize("form", ["DDdag", "DdagD"])
+def test_force_against_finite_differences(form):
+    """dS/dtheta from the oracle's adjoint formula against central differences
+    (h = 1e-5) of S from dense solves, on every link of an 8x8 lattice (pins
+    the formula, signs and the antiperiodic boundary).  "DDdag":
+    S = phi^+ (D D^+)^-1 phi = |D^-1 phi|^2, the oracle's own form.  "DdagD":
+    S = phi^+ (D^+ D)^-1 phi = |D^+^-1 phi|^2 from dense solves with D^+, which
+    the library evaluates as the first form at sigma_3 phi."""
     nx = nt = 8
     U, _ = amgs.gauge_metropolis(nx, nt, 2.0, 10, 11)
     phi = np.random.default_rng(12).standard_normal(128) + 1j * np.random.default_rng(13).standard_normal(128)
     rows = [(0, 2, 2, 2, 2, 2)]
     o = O.Oracle(O.default_params(nx, nt, rows, m0=0.1), U)
     o.setup(draw_order=1)
-    S, F = o.force(phi)
+    s3 = np.tile([1.0, -1.0], nx * nt)
+    S, F = o.force(phi if form == "DDdag" else s3 * phi)
 
     def action(th):
         ob = O.Oracle(O.default_params(nx, nt, rows, m0=0.1), np.exp(1j * th))
-        y = np.linalg.solve(dense(ob.dphi, 128), phi)
+        Dm = dense(ob.dphi, 128)
+        y = np.linalg.solve(Dm if form == "DDdag" else Dm.conj().T, phi)
         return np.vdot(y, y).real
 
     th = np.angle(U)

@@ -145,6 +145,25 @@ def test_mixed_precision_converges():
 
 
 @pytest.mark.parametrize("case", list(CASES))
+def test_mixed16_precision_converges(case):
+    """FP16 Krylov basis in the level-0 SAP block GMRES (FP32 arithmetic,
+    FP64 outer FGMRES): the batched solve reaches the FP64 tolerance, stays
+    within 1e-8 of the oracle's solution and needs at most 3 more outer
+    iterations; every right-hand side of the batch matches its single solve."""
+    s, o, _ = make(case, precision=amgs.PREC_MIXED16, max_rhs=3)
+    B = np.stack([cvec(s.size(0), 85 + i) for i in range(3)])
+    X, its, rrs, _ = s.solve(B)
+    for i in range(3):
+        xo, ito, _, _ = o.solve(B[i])
+        assert rrs[i] < 1e-10
+        assert rel(X[i], xo) < 1e-8
+        assert its[i] <= ito + 3, (its[i], ito)
+    x1, it1, rr1, _ = s.solve(B[1])
+    assert abs(it1 - its[1]) <= 1 and rr1 < 1e-10
+    assert rel(x1, X[1]) < 1e-8
+
+
+@pytest.mark.parametrize("case", list(CASES))
 def test_vcycle(case):
     """V-cycle preconditioner (src/amg.cpp:90-132): one application at level 0
     and the full solve."""
@@ -260,11 +279,13 @@ def test_comparison_solvers(case):
             assert rrs[i] < 1e-10 and rel(X[i], xo) < 1e-9
 
 
+@pytest.mark.parametrize("form", ["DdagD", "DDdag"])
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED])
-def test_force(prec):
+def test_force(prec, form):
     """Adjoint fermion force on the GPU against the oracle's FP64 CPU gradient,
     and against central differences (h = 1e-5) of the GPU action on 16
-    random links."""
+    random links.  The oracle evaluates phi^+ (D D^+)^-1 phi; the D^+ D form
+    equals it at sigma_3 phi."""
     nx, nt, rows, m0, beta = CASES["32_3lvl"]
     U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 1234)
     # tight solves so that the finite differences resolve the force
@@ -273,9 +294,10 @@ def test_force(prec):
     o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
     o.setup(draw_order=1)
     phi = np.stack([cvec(s.size(0), 120 + i) for i in range(2)])
-    S, F = s.force(phi)
+    s3 = np.tile([1.0, -1.0], nx * nt)
+    S, F = s.force(phi, form)
     for i in range(2):
-        So, Fo = o.force(phi[i])
+        So, Fo = o.force(phi[i] if form == "DDdag" else s3 * phi[i])
         assert abs(S[i] - So) < 1e-9 * So
         assert np.abs(F[i] - Fo).max() < 1e-8 * np.abs(Fo).max()
     th = np.angle(U)
@@ -290,7 +312,60 @@ def test_force(prec):
             sp = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec, out_tol=1e-13),
                              np.exp(1j * t))
             sp.setup()
-            Sp.append(sp.force(phi[0])[0])
+            Sp.append(sp.force(phi[0], form)[0])
             sp.close()
         fd = (Sp[0] - Sp[1]) / (2 * h)
         assert abs(fd - F[0][n, mu]) < 1e-4 * np.abs(F[0]).max() + 1e-6 * abs(fd), (n, mu, fd, F[0][n, mu])
+
+
+def test_solve_multi_uneven_configurations():
+    """Concurrent solves of independent configurations of different lattice
+    sizes, blockings, masses and couplings (amgs_solve_multi): each equals its
+    own solve on the same context, bit for bit, and converges."""
+    import torch
+
+    confs = [(32, [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)], -0.10, 2.0, amgs.PREC_FP64, 1),
+             (64, [(0, 4, 4, 10, 4, 4)], -0.15, 3.0, amgs.PREC_MIXED, 2),
+             (48, [(0, 4, 4, 6, 4, 4), (1, 2, 2, 6, 2, 2)], -0.05, 1.0, amgs.PREC_MIXED16, 1)]
+    solvers, bs, xs, nr = [], [], [], []
+    for i, (n, rows, m0, beta, prec, k) in enumerate(confs):
+        U, _ = amgs.gauge_metropolis(n, n, beta, 30, 300 + i)
+        s = amgs.Solver(amgs.default_params(n, n, rows, m0=m0, precision=prec, max_rhs=k), U)
+        s.setup()
+        solvers.append(s)
+        b = np.stack([cvec(s.size(0), 310 + 10 * i + r) for r in range(k)])
+        bs.append(torch.from_numpy(b.view(np.float64).copy()).to("cuda:0"))
+        xs.append(torch.zeros_like(bs[-1]))
+        nr.append(k)
+    torch.cuda.synchronize()
+    st, its, rrs = amgs.solve_multi(solvers, [b.data_ptr() for b in bs], [x.data_ptr() for x in xs], nr)
+    assert list(st) == [amgs.AMGS_OK] * len(confs)
+    for i, s in enumerate(solvers):
+        x_multi = xs[i].cpu().numpy().view(np.complex128).reshape(nr[i], -1)
+        x1, it1, rr1, _ = s.solve(bs[i].cpu().numpy().view(np.complex128).reshape(nr[i], -1))
+        assert np.array_equal(np.asarray(it1).reshape(-1), its[i])
+        assert np.array_equal(x1.reshape(nr[i], -1), x_multi)
+        assert np.all(rrs[i] < 1e-10)
+        s.close()
+
+
+@pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
+def test_graph_replay_matches_eager(prec):
+    """The level-0 preconditioner replayed from a captured HIP graph gives the
+    eager result bit for bit, over many replays (every outer iteration of
+    several solves), with eager solves in between and for two batch sizes."""
+    s, o, _ = make("32_3lvl", precision=prec, max_rhs=2)
+    B = np.stack([cvec(s.size(0), 400 + i) for i in range(2)])
+    xe1, ite1, rre1, _ = s.solve(B[0])
+    xe2, ite2, _, _ = s.solve(B)
+    s.set_graph(True)
+    for rep in range(2):
+        xg1, itg1, rrg1, _ = s.solve(B[0])
+        assert itg1 == ite1 and np.array_equal(xg1, xe1), rep
+        xg2, itg2, _, _ = s.solve(B)
+        assert np.array_equal(itg2, ite2) and np.array_equal(xg2, xe2), rep
+        s.set_graph(False)
+        xe, ite, _, _ = s.solve(B[0])
+        assert np.array_equal(xe, xe1)
+        s.set_graph(True)
+    assert rre1 < 1e-10
