@@ -72,6 +72,8 @@ static void build_geometry(Ctx& c) {
         g.stel = g.nt / g.sbt;
         g.sspb = g.sxel * g.stel;
         g.nsb = g.sbx * g.sbt;
+        if (l == 0 && g.sspb > (1 << 20))
+            throw ApiError(AMGS_ERR_GEOMETRY, "level-0 Schwarz blocks are limited to 2^20 sites");
         g.ncol = g.nsb != 1 ? g.nsb / 2 : 1;
         // red/black lists (src/sap.cpp:26-36): red iff (X + T) even; block b
         // goes to slot b / 2; a single block serves as both colours
@@ -232,11 +234,11 @@ static int sap_chunks(const Geo& g, int nrhs) {
 // FP16 Krylov storage exists for the FP32 engine only
 template <typename R> constexpr bool kH16 = std::is_same<R, float>::value;
 
-template <typename R, bool H>
+template <typename R, bool H, bool SQ>
 static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act) {
     switch (j) {
 #define AMGS_SAP_A(J) \
-    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J, H>), grid, dim3(FT), 0, s, a, k, act); break;
+    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J, H, SQ>), grid, dim3(FT), 0, s, a, k, act); break;
         AMGS_SAP_A(0) AMGS_SAP_A(1) AMGS_SAP_A(2) AMGS_SAP_A(3)
         AMGS_SAP_A(4) AMGS_SAP_A(5) AMGS_SAP_A(6) AMGS_SAP_A(7)
 #undef AMGS_SAP_A
@@ -277,11 +279,15 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     // power-of-two scale of the FP16 slots: a unit-norm block vector has
     // entries of order 1 / sqrt(2 sspb), stored as entries of order 1
     a.sig = h16 ? std::ldexp(1.0f, (int)std::lround(0.5 * std::log2(2.0 * g.sspb))) : 1.0f;
+    a.rcp_stel = 1.0f / (float)g.stel;
+    // square lattice: block membership of a neighbour from local coordinates
+    const bool sq = g.nx == g.nt;
     const dim3 grid(g.ncol * a.nchunk, nrhs);
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
-            hipLaunchKernelGGL(k_sap_init<R>, grid, dim3(FT), 0, s, a);
+            if (sq) hipLaunchKernelGGL((k_sap_init<R, true>), grid, dim3(FT), 0, s, a);
+            else hipLaunchKernelGGL((k_sap_init<R, false>), grid, dim3(FT), 0, s, a);
             for (int k = 0; k < c->P.sap_restarts; k++) {
                 for (int j = 0; j < a.m; j++) {
                     // algorithmic bytes: links of the colour's sites once; per active
@@ -290,8 +296,13 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
                     const double pa = h16 ? 32.0 + 8.0 * j : 2 * S * (2 + j);
                     int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * pa);
-                    if (h16) sap_arnoldi_a<R, kH16<R>>(grid, s, a, j, k, act);
-                    else sap_arnoldi_a<R, false>(grid, s, a, j, k, act);
+                    if (h16) {
+                        if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act);
+                        else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act);
+                    } else {
+                        if (sq) sap_arnoldi_a<R, false, true>(grid, s, a, j, k, act);
+                        else sap_arnoldi_a<R, false, false>(grid, s, a, j, k, act);
+                    }
                     if (act) c->kt[KT_SAP_A].end(s);
                     // w and Vt_i (i <= j) read, w written
                     const double pb = h16 ? 40.0 + 8.0 * j : 2 * S * (3 + j);
@@ -304,7 +315,10 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                 cplx<R>* xacc = last ? x : (cplx<R>*)nullptr;
                 if (h16) hipLaunchKernelGGL((k_sap_update<R, kH16<R>>), grid, dim3(FT), 0, s, a, xacc);
                 else hipLaunchKernelGGL((k_sap_update<R, false>), grid, dim3(FT), 0, s, a, xacc);
-                if (!last) hipLaunchKernelGGL(k_sap_resid<R>, grid, dim3(FT), 0, s, a);
+                if (!last) {
+                    if (sq) hipLaunchKernelGGL((k_sap_resid<R, true>), grid, dim3(FT), 0, s, a);
+                    else hipLaunchKernelGGL((k_sap_resid<R, false>), grid, dim3(FT), 0, s, a);
+                }
             }
             hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                                colour == 1 ? part_r.as<double>() : (double*)nullptr, skip.as<int>());

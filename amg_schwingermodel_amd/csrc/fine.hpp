@@ -51,34 +51,44 @@ __device__ __forceinline__ f16x4 to_h16(c32 a, c32 b) {
     return f16x4{(_Float16)a.re, (_Float16)a.im, (_Float16)b.re, (_Float16)b.im};
 }
 
+// Stencil core on given neighbours r0 = n+t, r1 = n+x, l0 = n-t, l1 = n-x,
+// their masks and the antiperiodic wraps of the time hops.
+template <typename R, bool DAG, typename In>
+__device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in, int n, int r0, int r1, int l0, int l1,
+                                            bool mr0, bool mr1, bool ml0, bool ml1, bool wr, bool wl, R mm,
+                                            cplx<R>& o0, cplx<R>& o1) {
+    cplx<R> c0, c1;
+    in(n, c0, c1);
+    cplx<R> f0 = U[2 * n], f1 = U[2 * n + 1];
+    cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
+    if (wr) f0 = -f0;
+    if (wl) b0 = -b0;
+    cplx<R> A = czero<R>(), B = czero<R>(), C = czero<R>(), E = czero<R>();
+    cplx<R> p0, p1;
+    // forward/backward neighbour reads
+    if (!DAG) {
+        if (mr0) { in(r0, p0, p1); A = f0 * (p0 - p1); }
+        if (mr1) { in(r1, p0, p1); B = f1 * (p0 + mul_i(p1)); }
+        if (ml0) { in(l0, p0, p1); C = b0 * (p0 + p1); }
+        if (ml1) { in(l1, p0, p1); E = b1 * (p0 - mul_i(p1)); }
+    } else {
+        if (ml0) { in(l0, p0, p1); A = b0 * (p0 - p1); }
+        if (ml1) { in(l1, p0, p1); B = b1 * (p0 + mul_i(p1)); }
+        if (mr0) { in(r0, p0, p1); C = f0 * (p0 + p1); }
+        if (mr1) { in(r1, p0, p1); E = f1 * (p0 - mul_i(p1)); }
+    }
+    const R h = R(0.5);
+    o0 = mm * c0 - h * (A + B + C + E);
+    o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+}
+
 template <typename R, bool DAG, typename Mask, typename In>
 __device__ __forceinline__ void wilson_site_in(const cplx<R>* __restrict__ U, In in, int n, int nx, int nt, R mm,
                                                Mask mask, cplx<R>& o0, cplx<R>& o1) {
     const int r0 = nb_right(n, 0, nx, nt), r1 = nb_right(n, 1, nx, nt);
     const int l0 = nb_left(n, 0, nx, nt), l1 = nb_left(n, 1, nx, nt);
-    cplx<R> c0, c1;
-    in(n, c0, c1);
-    cplx<R> f0 = U[2 * n], f1 = U[2 * n + 1];
-    cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
-    if (wraps_right(n, 0, nt)) f0 = -f0;
-    if (wraps_left(n, 0, nt)) b0 = -b0;
-    cplx<R> A = czero<R>(), B = czero<R>(), C = czero<R>(), E = czero<R>();
-    cplx<R> p0, p1;
-    // forward/backward neighbour reads
-    if (!DAG) {
-        if (mask(r0)) { in(r0, p0, p1); A = f0 * (p0 - p1); }
-        if (mask(r1)) { in(r1, p0, p1); B = f1 * (p0 + mul_i(p1)); }
-        if (mask(l0)) { in(l0, p0, p1); C = b0 * (p0 + p1); }
-        if (mask(l1)) { in(l1, p0, p1); E = b1 * (p0 - mul_i(p1)); }
-    } else {
-        if (mask(l0)) { in(l0, p0, p1); A = b0 * (p0 - p1); }
-        if (mask(l1)) { in(l1, p0, p1); B = b1 * (p0 + mul_i(p1)); }
-        if (mask(r0)) { in(r0, p0, p1); C = f0 * (p0 + p1); }
-        if (mask(r1)) { in(r1, p0, p1); E = f1 * (p0 - mul_i(p1)); }
-    }
-    const R h = R(0.5);
-    o0 = mm * c0 - h * (A + B + C + E);
-    o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+    wilson_core<R, DAG>(U, in, n, r0, r1, l0, l1, mask(r0), mask(r1), mask(l0), mask(l1), wraps_right(n, 0, nt),
+                        wraps_left(n, 0, nt), mm, o0, o1);
 }
 
 template <typename R, bool DAG, typename Mask>
@@ -172,6 +182,7 @@ template <typename R> struct SapArgs {
     cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]; with FP16 storage only slot 0 (V_0) and slot 1 (w)
     f16x4* Vh;             // FP16 storage: slots 1..m of [nrhs][ns] (both spins of a site in 8 bytes)
     float sig;             // FP16 storage: power-of-two scale of the stored slots (1 otherwise)
+    float rcp_stel;        // 1 / stel (block-local index -> coordinates)
     cplx<R>* xl;           // [nrhs][ns][2] block solutions
     const int* skip;       // per-rhs: SAP already converged
     SapState st;
@@ -179,6 +190,7 @@ template <typename R> struct SapArgs {
 
 struct ItemIdx {
     int item, chunk, rhs, blk, m_lo, m_hi;
+    int x0, t0;  // lattice coordinates of the block's first site
 };
 
 template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R>& a) {
@@ -188,9 +200,50 @@ template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R
     t.rhs = blockIdx.y;
     t.item = t.rhs * a.ncol + i;
     t.blk = a.blocks[i];
+    const int X = t.blk / a.g.sbt;
+    t.x0 = X * a.g.sxel;
+    t.t0 = (t.blk - X * a.g.sbt) * a.g.stel;
     t.m_lo = t.chunk * a.cs;
     t.m_hi = min(a.g.sspb, t.m_lo + a.cs);
     return t;
+}
+
+// Site m (x-major local index, src/sap.cpp:3-24) of the item's Schwarz block:
+// global site and coordinates.  m / stel through the FP32 reciprocal, exact for
+// blocks of up to 2^20 sites.
+struct BSite {
+    int n, x, t, mx, mt;
+};
+template <typename R> __device__ __forceinline__ BSite bsite(const SapArgs<R>& a, const ItemIdx& t, int m) {
+    BSite b;
+    b.mx = (int)((float(m) + 0.5f) * a.rcp_stel);
+    b.mt = m - b.mx * a.g.stel;
+    b.x = t.x0 + b.mx;
+    b.t = t.t0 + b.mt;
+    b.n = b.x * a.g.nt + b.t;
+    return b;
+}
+
+// D_B at a site of the item's block, zero Dirichlet outside the block
+// (src/level.cpp:304-368).  On square lattices a neighbour's membership
+// follows from the local coordinates; otherwise it is the reference's
+// getMandBlock test with its x = n / Nx (include/level.h:48-59).
+template <bool SQ, typename R, typename In>
+__device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx& t, const BSite& b, In in, cplx<R>& o0,
+                                              cplx<R>& o1) {
+    const Geo& g = a.g;
+    if constexpr (SQ) {
+        const bool tr = b.t + 1 == g.nt, tl = b.t == 0;
+        const int r0 = tr ? b.n - b.t : b.n + 1;
+        const int l0 = tl ? b.n + g.nt - 1 : b.n - 1;
+        const int r1 = b.x + 1 == g.nx ? b.t : b.n + g.nt;
+        const int l1 = b.x == 0 ? b.n + (g.nx - 1) * g.nt : b.n - g.nt;
+        const bool wt = g.stel == g.nt, wx = g.sxel == g.nx;  // block spans the extent
+        wilson_core<R, false>(a.U, in, b.n, r0, r1, l0, l1, wt || b.mt + 1 < g.stel, wx || b.mx + 1 < g.sxel,
+                              wt || b.mt > 0, wx || b.mx > 0, tr, tl, a.mm, o0, o1);
+    } else {
+        wilson_site_in<R, false>(a.U, in, b.n, g.nx, g.nt, a.mm, InBlock{g, t.blk}, o0, o1);
+    }
 }
 
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
@@ -328,7 +381,7 @@ __device__ __forceinline__ void park(c64 v, int q, c64* tmp, int nqmax) {
 }
 
 // init: x_B = b_B (x0 = b, src/sap.cpp:61), Vt_0 = b_B - D_B b_B; partials |r|^2, |b|^2
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R> a) {
+template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R> a) {
     __shared__ c64 red[8];
     const ItemIdx t = item_of(a);
     if (a.skip && a.skip[t.rhs]) return;
@@ -336,9 +389,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
     cplx<R>* v0 = slot(a, 0, t.rhs);
     c64 q[2] = {{0, 0}, {0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
-        const int n = sap_site(a.g, t.blk, m);
+        const BSite b = bsite(a, t, m);
+        const int n = b.n;
         cplx<R> o0, o1;
-        wilson_site<R, false>(a.U, a.rglob + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        block_stencil<SQ>(a, t, b, SpinPtr<R>{a.rglob + off}, o0, o1);
         const cplx<R> b0 = a.rglob[off + 2 * n], b1 = a.rglob[off + 2 * n + 1];
         a.xl[off + 2 * n] = b0;
         a.xl[off + 2 * n + 1] = b1;
@@ -363,7 +417,7 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R
 // Arnoldi step J, part A: prologue finalises step J-1 (or starts cycle k),
 // body w = s_J D_B Vt_J -> Vt_{J+1} (slot 1 with FP16 storage H);
 // partials <w, Vt_i> (i <= J), <Vt_J, Vt_i> (i < J).
-template <typename R, int J, bool H>
+template <typename R, int J, bool H, bool SQ>
 __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
@@ -416,13 +470,13 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
     for (int i = 0; i <= J; i++) pw[i] = pv[i] = czero<R>();
 #pragma unroll 2
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
-        const int n = sap_site(a.g, t.blk, m);
+        const BSite b = bsite(a, t, m);
+        const int n = b.n;
         cplx<R> o0, o1;
         if constexpr (H && J > 0)
-            wilson_site_in<R, false>(a.U, SpinH16{hslot(a, J, t.rhs)}, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0,
-                                     o1);
+            block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1);
         else
-            wilson_site<R, false>(a.U, slot(a, J, t.rhs), n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+            block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1);
         o0 = sj * o0;
         o1 = sj * o1;
         w[2 * n] = o0;
@@ -488,7 +542,7 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
     cplx<R>* w = slot(a, H ? 1 : j + 1, t.rhs);
     double nn = 0;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
-        const int n = sap_site(a.g, t.blk, m);
+        const int n = bsite(a, t, m).n;
         c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
         for (int i = 0; i <= j; i++) {
             cplx<R> v0, v1;
@@ -550,7 +604,7 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
     const int mi = L.maxIt;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
-        const int n = sap_site(a.g, t.blk, m);
+        const int n = bsite(a, t, m).n;
         c64 x0 = cvt<double>(a.xl[off + 2 * n]), x1 = cvt<double>(a.xl[off + 2 * n + 1]);
         for (int jj = 0; jj < mi; jj++) {
             cplx<R> v0, v1;
@@ -570,7 +624,7 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
 }
 
 // True block residual after a cycle: Vt_0 = b_B - D_B x_B, partial |r|^2.
-template <typename R> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<R> a) {
+template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<R> a) {
     __shared__ c64 red[8];
     __shared__ int act;
     const ItemIdx t = item_of(a);
@@ -582,9 +636,10 @@ template <typename R> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<
     cplx<R>* v0 = slot(a, 0, t.rhs);
     c64 q[1] = {{0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
-        const int n = sap_site(a.g, t.blk, m);
+        const BSite b = bsite(a, t, m);
+        const int n = b.n;
         cplx<R> o0, o1;
-        wilson_site<R, false>(a.U, a.xl + off, n, a.g.nx, a.g.nt, a.mm, InBlock{a.g, t.blk}, o0, o1);
+        block_stencil<SQ>(a, t, b, SpinPtr<R>{a.xl + off}, o0, o1);
         const cplx<R> r0 = a.rglob[off + 2 * n] - o0, r1 = a.rglob[off + 2 * n + 1] - o1;
         v0[2 * n] = r0;
         v0[2 * n + 1] = r1;
