@@ -13,7 +13,8 @@ WRITE_SIZE, both in KiB per dispatch), run from the repository root:
 The workload first copies a 256 MiB device buffer three times (a plain copy
 kernel of known size: 256 MiB read and 256 MiB written per launch), then runs
 two outer iterations of the mixed-precision batched solve (512x512, 3 levels,
-32 right-hand sides, m0 = -0.12, beta = 3).  Each counter is scaled by the
+32 right-hand sides, m0 = -0.12, beta = 3; FP16 SAP Krylov basis, eager
+launches), the configuration of the benchmark's roofline pass.  Each counter is scaled by the
 ratio of the copy's known bytes to what the counter reads for the copy.
 """
 import csv
@@ -41,7 +42,7 @@ def sample(nx=512, nrhs=32, m0=-0.12):
     torch.cuda.synchronize()
     U, _ = amgs.gauge_metropolis(nx, nx, 3.0, 500, 20260512)
     B = np.stack([amgs.gaussian(nx * nx * 2, 1000 + i) for i in range(nrhs)])
-    s = amgs.Solver(amgs.default_params(nx, nx, ROWS, m0=m0, max_rhs=nrhs, precision=amgs.PREC_MIXED,
+    s = amgs.Solver(amgs.default_params(nx, nx, ROWS, m0=m0, max_rhs=nrhs, precision=amgs.PREC_MIXED16,
                                         out_m=2, out_restarts=1), U)
     s.setup()
     s.kernel_timing(0, True)
