@@ -155,14 +155,21 @@ template <typename R, typename LP, typename VP>
 __device__ __forceinline__ void stencil_row_p(LP L0, LP L1, LP L2, LP L3, LP L4, int D, VP v0, VP vr0, VP vr1, VP vl0,
                                               VP vl1, R f1, R f2, R f3, R f4, cplx<R>& acc) {
     cplx<R> a0 = czero<R>(), a1 = czero<R>(), a2 = czero<R>(), a3 = czero<R>(), a4 = czero<R>();
-#pragma unroll 4
-    for (int c = 0; c < D; c++) {
-        const int o = c * D;
-        a0 += ld<R>(L0 + o) * ld<R>(v0 + c);
-        a1 += ld<R>(L1 + o) * ld<R>(vr0 + c);
-        a2 += ld<R>(L2 + o) * ld<R>(vr1 + c);
-        a3 += ld<R>(L3 + o) * ld<R>(vl0 + c);
-        a4 += ld<R>(L4 + o) * ld<R>(vl1 + c);
+    // U columns of the five blocks in flight per round trip (fewer in FP64)
+    constexpr int U = sizeof(R) == 4 ? 8 : 4;
+    for (int c0 = 0; c0 < D; c0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + u;
+            if (c < D) {
+                const int o = c * D;
+                a0 += ld<R>(L0 + o) * ld<R>(v0 + c);
+                a1 += ld<R>(L1 + o) * ld<R>(vr0 + c);
+                a2 += ld<R>(L2 + o) * ld<R>(vr1 + c);
+                a3 += ld<R>(L3 + o) * ld<R>(vl0 + c);
+                a4 += ld<R>(L4 + o) * ld<R>(vl1 + c);
+            }
+        }
     }
     acc -= a0;
     acc -= f1 * a1 + f3 * a3;
@@ -312,9 +319,59 @@ __device__ __forceinline__ void w_dlocal_l(const CArgs<R>& A, int l, int blk, LP
     }
     wave_sync();
 }
+// 2x2-site Schwarz blocks on a square level larger than 2x2 (the reference's
+// blocking of the coarse grids): every site has exactly one in-block
+// neighbour in t and one in x and no block wraps the time boundary, so a row
+// walks three link blocks (A, the t hop, the x hop) instead of five; the two
+// dropped hops are the exact zeros of the general form.  Local index
+// m = 2 mx + mt; the t neighbour is m ^ 1, the x neighbour m ^ 2.
+template <typename R, typename LP>
+__device__ __forceinline__ void w_dlocal_2x2(const CArgs<R>& A, int l, int blk, LP links, const cplx<R>* in,
+                                             cplx<R>* out) {
+    const Geo& g = A.lv[l].g;
+    const int D = g.dof, nb = 4 * D, lane = threadIdx.x & 63;
+    const int msz = D * D, ksz = g.ns * msz;
+    for (int e = lane; e < nb; e += 64) {
+        const int m = e / D, r = e - m * D;
+        const int n = sap_site(g, blk, m);
+        const LP L0 = links + n * msz + r;
+        const LP Lt = L0 + ((m & 1) ? 3 : 1) * ksz;  // C_0 (backward) from mt = 1, B_0 (forward) from mt = 0
+        const LP Lx = L0 + ((m & 2) ? 4 : 2) * ksz;  // C_1 from mx = 1, B_1 from mx = 0
+        const cplx<R>* v0 = in + m * D;
+        const cplx<R>* vt = in + (m ^ 1) * D;
+        const cplx<R>* vx = in + (m ^ 2) * D;
+        cplx<R> a0 = czero<R>(), at = czero<R>(), ax = czero<R>();
+        // U columns of the three blocks in flight per round trip (fewer in FP64,
+        // whose complex values take twice the registers)
+        constexpr int U = sizeof(R) == 4 ? 10 : 5;
+        for (int c0 = 0; c0 < D; c0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = c0 + u;
+                if (c < D) {
+                    const int o = c * D;
+                    a0 += ld<R>(L0 + o) * ld<R>(v0 + c);
+                    at += ld<R>(Lt + o) * ld<R>(vt + c);
+                    ax += ld<R>(Lx + o) * ld<R>(vx + c);
+                }
+            }
+        }
+        cplx<R> acc = A.mm * in[e];
+        acc -= a0;
+        acc -= at;
+        acc -= ax;
+        out[e] = acc;
+    }
+    wave_sync();
+}
+
 template <typename R>
 __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
-    w_dlocal_l<R>(A, l, blk, as_global(A.lv[l].links), in, out);
+    const Geo& g = A.lv[l].g;
+    if (g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2)
+        w_dlocal_2x2<R>(A, l, blk, as_global(A.lv[l].links), in, out);
+    else
+        w_dlocal_l<R>(A, l, blk, as_global(A.lv[l].links), in, out);
 }
 
 // Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
