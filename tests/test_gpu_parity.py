@@ -369,3 +369,31 @@ def test_graph_replay_matches_eager(prec):
         assert np.array_equal(xe, xe1)
         s.set_graph(True)
     assert rre1 < 1e-10
+
+
+def test_full_size_256_three_levels():
+    """The 256x256, 3-level workload (aggregation 8x8 then 4x4 blocks, 10 test
+    vectors, 4x4 SAP blocks; beta = 2 Metropolis links from 500 sweeps,
+    m0 = -0.18, one Gaussian right-hand side) at full size: the plain FP64 GPU
+    solve against the oracle (iterations +-1, solution within 1e-9), and both
+    mixed-precision forms to the FP64 tolerance within 1e-8 of it."""
+    n = 256
+    rows = [(0, 8, 8, 10, 4, 4), (1, 4, 4, 10, 4, 4)]
+    U, _ = amgs.gauge_metropolis(n, n, 2.0, 500, 20260511)
+    b = amgs.gaussian(n * n * 2, 2001)
+    s = amgs.Solver(amgs.default_params(n, n, rows, m0=-0.18), U)
+    s.setup()
+    x, it, rr, _ = s.solve(b)
+    o = O.Oracle(O.default_params(n, n, rows, m0=-0.18, threads=16), U)
+    o.setup(draw_order=1)
+    xo, ito, _, rro = o.solve(b)
+    print(f"256^2 3-level: GPU FP64 {it} iterations relres {rr:.2e}; oracle {ito} iterations relres {rro:.2e}")
+    assert abs(it - ito) <= 1, (it, ito)
+    assert rr < 1e-10 and rel(x, xo) < 1e-9
+    for prec in (amgs.PREC_MIXED, amgs.PREC_MIXED16):
+        sm = amgs.Solver(amgs.default_params(n, n, rows, m0=-0.18, precision=prec), U)
+        sm.setup()
+        xm, itm, rrm, _ = sm.solve(b)
+        print(f"  precision {prec}: {itm} iterations, relres {rrm:.2e}, error vs oracle {rel(xm, xo):.2e}")
+        assert rrm < 1e-10 and rel(xm, xo) < 1e-8 and itm <= ito + 3
+        sm.close()
