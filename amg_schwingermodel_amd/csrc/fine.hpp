@@ -540,28 +540,34 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
     __syncthreads();
     if (!L.active) return;
     cplx<R>* w = slot(a, H ? 1 : j + 1, t.rhs);
+    // the update runs in the engine's precision (the Gram coefficients were
+    // formed in FP64 in the prologue)
+    cplx<R> eta[SAP_MAXM + 1];
+#pragma unroll
+    for (int i = 0; i <= SAP_MAXM; i++) eta[i] = i <= j ? cvt<R>(L.eta[i]) : czero<R>();
     double nn = 0;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
-        c64 w0 = cvt<double>(w[2 * n]), w1 = cvt<double>(w[2 * n + 1]);
-        for (int i = 0; i <= j; i++) {
-            cplx<R> v0, v1;
-            ld_slot<R, H>(a, i, t.rhs, n, v0, v1);
-            const c64 c = L.eta[i];
-            w0 -= c * cvt<double>(v0);
-            w1 -= c * cvt<double>(v1);
+        cplx<R> w0 = w[2 * n], w1 = w[2 * n + 1];
+#pragma unroll
+        for (int i = 0; i <= SAP_MAXM; i++) {
+            if (i <= j) {
+                cplx<R> v0, v1;
+                ld_slot<R, H>(a, i, t.rhs, n, v0, v1);
+                w0 -= eta[i] * v0;
+                w1 -= eta[i] * v1;
+            }
         }
         if constexpr (H) {
-            const double sg = a.sig;
-            const f16x4 h = to_h16(c32{float(sg * w0.re), float(sg * w0.im)}, c32{float(sg * w1.re), float(sg * w1.im)});
+            const float sg = a.sig;
+            const f16x4 h = to_h16(c32{sg * w0.re, sg * w0.im}, c32{sg * w1.re, sg * w1.im});
             hslot(a, j + 1, t.rhs)[n] = h;
             nn += double(h.x) * double(h.x) + double(h.y) * double(h.y) + double(h.z) * double(h.z) +
                   double(h.w) * double(h.w);
         } else {
-            const cplx<R> r0 = cvt<R>(w0), r1 = cvt<R>(w1);
-            w[2 * n] = r0;
-            w[2 * n + 1] = r1;
-            nn += double(norm2(r0)) + double(norm2(r1));
+            w[2 * n] = w0;
+            w[2 * n + 1] = w1;
+            nn += double(norm2(w0)) + double(norm2(w1));
         }
     }
     c64 q[1] = {{nn, 0.0}};
@@ -603,22 +609,28 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
     if (!L.active && !xacc) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
     const int mi = L.maxIt;
+    // x_B += sum_j eta_j Vt_j in the engine's precision
+    cplx<R> eta[SAP_MAXM];
+#pragma unroll
+    for (int i = 0; i < SAP_MAXM; i++) eta[i] = i < mi ? cvt<R>(L.eta[i]) : czero<R>();
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
-        c64 x0 = cvt<double>(a.xl[off + 2 * n]), x1 = cvt<double>(a.xl[off + 2 * n + 1]);
-        for (int jj = 0; jj < mi; jj++) {
-            cplx<R> v0, v1;
-            ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
-            const c64 c = L.eta[jj];
-            x0 += c * cvt<double>(v0);
-            x1 += c * cvt<double>(v1);
+        cplx<R> x0 = a.xl[off + 2 * n], x1 = a.xl[off + 2 * n + 1];
+#pragma unroll
+        for (int jj = 0; jj < SAP_MAXM; jj++) {
+            if (jj < mi) {
+                cplx<R> v0, v1;
+                ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
+                x0 += eta[jj] * v0;
+                x1 += eta[jj] * v1;
+            }
         }
         if (xacc) {
-            xacc[off + 2 * n] += cvt<R>(x0);
-            xacc[off + 2 * n + 1] += cvt<R>(x1);
+            xacc[off + 2 * n] += x0;
+            xacc[off + 2 * n + 1] += x1;
         } else {
-            a.xl[off + 2 * n] = cvt<R>(x0);
-            a.xl[off + 2 * n + 1] = cvt<R>(x1);
+            a.xl[off + 2 * n] = x0;
+            a.xl[off + 2 * n + 1] = x1;
         }
     }
 }
