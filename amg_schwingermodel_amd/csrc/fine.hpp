@@ -63,19 +63,26 @@ __device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in
     cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
     if (wr) f0 = -f0;
     if (wl) b0 = -b0;
-    cplx<R> A = czero<R>(), B = czero<R>(), C = czero<R>(), E = czero<R>();
-    cplx<R> p0, p1;
-    // forward/backward neighbour reads
+    // neighbour reads without branches: a masked-out neighbour reads the
+    // site's own (cached) entries and its term is replaced by zero, so the four
+    // loads are in flight together
+    cplx<R> q00, q01, q10, q11, q20, q21, q30, q31;
+    in(mr0 ? r0 : n, q00, q01);
+    in(mr1 ? r1 : n, q10, q11);
+    in(ml0 ? l0 : n, q20, q21);
+    in(ml1 ? l1 : n, q30, q31);
+    const cplx<R> z = czero<R>();
+    cplx<R> A, B, C, E;
     if (!DAG) {
-        if (mr0) { in(r0, p0, p1); A = f0 * (p0 - p1); }
-        if (mr1) { in(r1, p0, p1); B = f1 * (p0 + mul_i(p1)); }
-        if (ml0) { in(l0, p0, p1); C = b0 * (p0 + p1); }
-        if (ml1) { in(l1, p0, p1); E = b1 * (p0 - mul_i(p1)); }
+        A = mr0 ? f0 * (q00 - q01) : z;
+        B = mr1 ? f1 * (q10 + mul_i(q11)) : z;
+        C = ml0 ? b0 * (q20 + q21) : z;
+        E = ml1 ? b1 * (q30 - mul_i(q31)) : z;
     } else {
-        if (ml0) { in(l0, p0, p1); A = b0 * (p0 - p1); }
-        if (ml1) { in(l1, p0, p1); B = b1 * (p0 + mul_i(p1)); }
-        if (mr0) { in(r0, p0, p1); C = f0 * (p0 + p1); }
-        if (mr1) { in(r1, p0, p1); E = f1 * (p0 - mul_i(p1)); }
+        A = ml0 ? b0 * (q20 - q21) : z;
+        B = ml1 ? b1 * (q30 + mul_i(q31)) : z;
+        C = mr0 ? f0 * (q00 + q01) : z;
+        E = mr1 ? f1 * (q10 - mul_i(q11)) : z;
     }
     const R h = R(0.5);
     o0 = mm * c0 - h * (A + B + C + E);
