@@ -617,20 +617,15 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
     const int mi = L.maxIt;
     // x_B += sum_j eta_j Vt_j in the engine's precision
-    cplx<R> eta[SAP_MAXM];
-#pragma unroll
-    for (int i = 0; i < SAP_MAXM; i++) eta[i] = i < mi ? cvt<R>(L.eta[i]) : czero<R>();
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
         cplx<R> x0 = a.xl[off + 2 * n], x1 = a.xl[off + 2 * n + 1];
-#pragma unroll
-        for (int jj = 0; jj < SAP_MAXM; jj++) {
-            if (jj < mi) {
-                cplx<R> v0, v1;
-                ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
-                x0 += eta[jj] * v0;
-                x1 += eta[jj] * v1;
-            }
+        for (int jj = 0; jj < mi; jj++) {
+            cplx<R> v0, v1;
+            ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
+            const cplx<R> c = cvt<R>(L.eta[jj]);
+            x0 += c * v0;
+            x1 += c * v1;
         }
         if (xacc) {
             xacc[off + 2 * n] += x0;
