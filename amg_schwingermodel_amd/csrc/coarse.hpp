@@ -365,13 +365,72 @@ __device__ __forceinline__ void w_dlocal_2x2(const CArgs<R>& A, int l, int blk, 
     wave_sync();
 }
 
-template <typename R>
-__device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out) {
+// The same 2x2-block stencil with its 3 x 4D (row, link block) partial sums
+// spread evenly over the lanes (at most 4 per lane, all their columns in
+// flight together: two round trips instead of four for D = 20), combined per
+// row through a wave-private LDS area in the same order as w_dlocal_2x2.
+template <typename R, typename LP>
+__device__ __forceinline__ void w_dlocal_2x2s(const CArgs<R>& A, int l, int blk, LP links, const cplx<R>* in,
+                                              cplx<R>* out, cplx<R>* part) {
     const Geo& g = A.lv[l].g;
-    if (g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2)
+    const int D = g.dof, nb = 4 * D, nu = 3 * nb, lane = threadIdx.x & 63;
+    const int msz = D * D, ksz = g.ns * msz;
+    constexpr int S = 4;  // units per lane (nu <= 4 * 64)
+    LP Lp[S];
+    const cplx<R>* vp[S];
+    cplx<R> acc[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u0 = lane + 64 * s, u = u0 < nu ? u0 : 0;
+        const int q = u / nb, e = u - q * nb;  // link block (0 A, 1 t hop, 2 x hop) and row
+        const int m = e / D, r = e - m * D;
+        const int n = sap_site(g, blk, m);
+        const int k = q == 0 ? 0 : q == 1 ? ((m & 1) ? 3 : 1) : ((m & 2) ? 4 : 2);
+        Lp[s] = links + k * ksz + n * msz + r;
+        vp[s] = in + (q == 0 ? m : q == 1 ? (m ^ 1) : (m ^ 2)) * D;
+        acc[s] = czero<R>();
+    }
+    constexpr int U = 10;
+    for (int c0 = 0; c0 < D; c0 += U) {
+#pragma unroll
+        for (int uu = 0; uu < U; uu++) {
+            const int c = c0 + uu;
+            if (c < D) {
+#pragma unroll
+                for (int s = 0; s < S; s++) acc[s] += ld<R>(Lp[s] + c * D) * ld<R>(vp[s] + c);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++)
+        if (lane + 64 * s < nu) part[lane + 64 * s] = acc[s];
+    wave_sync();
+    for (int e = lane; e < nb; e += 64) {
+        cplx<R> o = A.mm * in[e];
+        o -= part[e];
+        o -= part[nb + e];
+        o -= part[2 * nb + e];
+        out[e] = o;
+    }
+    wave_sync();
+}
+
+// part: wave-private LDS scratch of >= 12 D complex entries (FP32 engine), or null
+template <typename R>
+__device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out,
+                                         cplx<R>* part = nullptr) {
+    const Geo& g = A.lv[l].g;
+    if (g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2) {
+        if constexpr (std::is_same<R, float>::value) {
+            if (part && 12 * g.dof <= 4 * 64) {
+                w_dlocal_2x2s<R>(A, l, blk, as_global(A.lv[l].links), in, out, part);
+                return;
+            }
+        }
         w_dlocal_2x2<R>(A, l, blk, as_global(A.lv[l].links), in, out);
-    else
+    } else {
         w_dlocal_l<R>(A, l, blk, as_global(A.lv[l].links), in, out);
+    }
 }
 
 // Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
@@ -405,6 +464,10 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
     auto V = [&](int j) { return base + (size_t)j * nb; };
     cplx<R>* xb = base + (size_t)(m + 1) * nb;
     cplx<R>* bb = base + (size_t)(m + 2) * nb;
+    // partial sums of the block stencil: this wave's slice of the stencil
+    // staging buffer (idle while the colour's blocks are solved)
+    constexpr int PART = VIN_BYTES / CW / (int)sizeof(cplx<R>);
+    cplx<R>* part = 12 * D <= PART ? (cplx<R>*)K.S.vin + wv * PART : nullptr;
     // gather b_B; x_B = b_B
     for (int e = lane; e < nb; e += 64) {
         const int mm = e / D, d = e - mm * D;
@@ -413,7 +476,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
         xb[e] = v;
     }
     wave_sync();
-    w_dlocal(A, l, blk, xb, V(1));
+    w_dlocal(A, l, blk, xb, V(1), part);
     for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
     wave_sync();
     const double normb = sqrt(w_dot(bb, bb, nb).re);
@@ -429,7 +492,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
         if (lane == 0) S.g[0] = g0;
         wave_sync();
         for (int j = 0; j < m; j++) {
-            w_dlocal(A, l, blk, V(j), V(j + 1));
+            w_dlocal(A, l, blk, V(j), V(j + 1), part);
             cplx<R>* w = V(j + 1);
             for (int i = 0; i <= j; i++) {
                 const c64 h = w_dot(w, V(i), nb);
@@ -477,7 +540,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
             xb[e] = cvt<R>(s);
         }
         wave_sync();
-        w_dlocal(A, l, blk, xb, V(1));
+        w_dlocal(A, l, blk, xb, V(1), part);
         for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
         wave_sync();
         err = sqrt(w_dot(V(0), V(0), nb).re);
