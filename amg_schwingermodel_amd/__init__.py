@@ -106,6 +106,7 @@ SIGNATURES = {
     "amgs_force": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_set_graph": ([_V, ctypes.c_int], ctypes.c_int),
+    "amgs_autotune": ([_V, ctypes.c_int, _I, _P], ctypes.c_int),
     "amgs_solve_multi": ([ctypes.c_int, ctypes.POINTER(_V), _I, ctypes.POINTER(_V), ctypes.POINTER(_V), _I, _P, _I],
                          ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -241,6 +242,14 @@ class Solver:
 
     def setup(self):
         self._chk(lib().amgs_setup(self.h), "amgs_setup")
+
+    def autotune(self, nrhs=1):
+        """Pick the level-0 SAP launch shape for batches of nrhs by timing; returns
+        (workgroup target, sweep ms)."""
+        best = ctypes.c_int(0)
+        ms = ctypes.c_double(0)
+        self._chk(lib().amgs_autotune(self.h, nrhs, ctypes.byref(best), ctypes.byref(ms)), "amgs_autotune")
+        return best.value, ms.value
 
     def set_graph(self, enable=True):
         """Replay the level-0 preconditioner from a captured HIP graph."""

@@ -221,14 +221,24 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
 
 // Chunks per SAP item (block x rhs): enough workgroups to fill the chip
 // (about AMGS_SAP_WGS, default 2048), each chunk a multiple of FT sites.
-static int sap_chunks(const Geo& g, int nrhs) {
+static int sap_default_wgs() {
     static const int target = [] {
         const char* e = std::getenv("AMGS_SAP_WGS");
         const int v = e ? std::atoi(e) : 0;
         return v > 0 ? v : 2048;
     }();
+    return target;
+}
+static int sap_chunks(const Geo& g, int nrhs, int target) {
     const int items = g.ncol * nrhs, maxc = (int)cdiv_u(g.sspb, FT);
     return std::max(1, std::min(maxc, (int)cdiv_u(target, items)));
+}
+// workgroup target of the level-0 SAP launches for this batch size: the
+// autotuned value (amgs_autotune) if there is one, else the default
+template <typename R> int Engine<R>::sap_wgs(int nrhs) const {
+    for (const auto& t : sap_tuned)
+        if (t.first == nrhs) return t.second;
+    return sap_default_wgs();
 }
 
 // FP16 Krylov storage exists for the FP32 engine only
@@ -258,7 +268,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                        (double*)nullptr, skip.as<int>());
-    const int nchunk = sap_chunks(g, nrhs), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
+    const int nchunk = sap_chunks(g, nrhs, sap_wgs(nrhs)), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
     a.mm = mm;
@@ -913,6 +923,73 @@ extern "C" int amgs_solve_dev(amgs_ctx* ctx, int nrhs, const void* b_dev, void* 
     HIP_CHECK(hipMemcpyAsync(x_dev, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToDevice, c.stream));
     c.sync();
     return rc;
+    API_END
+}
+
+// ---- autotuner of the level-0 SAP launch shape ----
+// Times one SAP sweep (all kernels of both colours) for a set of workgroup
+// targets, i.e. chunks per (block, rhs) item, and keeps the fastest for this
+// batch size.  Deterministic afterwards: the choice only fixes the chunking.
+__global__ void k_fill_pattern(size_t n, c32* out32, c64* out64) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = 0.5 - (double)((i * 2654435761u) % 1000003u) / 1000003.0;
+    const double b = 0.5 - (double)((i * 40503u + 7u) % 999983u) / 999983.0;
+    if (out32) out32[i] = c32{(float)a, (float)b};
+    if (out64) out64[i] = c64{a, b};
+}
+
+template <typename R> static int autotune_sap(Ctx& c, Engine<R>& e, int nrhs, double* best_ms) {
+    const Geo& g = c.geo[0];
+    const size_t tot = (size_t)nrhs * c.nvec(0);
+    DevBuf v, x;
+    v.alloc(sizeof(cplx<R>) * tot);
+    x.alloc(sizeof(cplx<R>) * tot);
+    const bool f32 = std::is_same<R, float>::value;
+    hipLaunchKernelGGL(k_fill_pattern, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                       f32 ? (c32*)v.p : nullptr, f32 ? nullptr : (c64*)v.p);
+    const int maxc = (int)cdiv_u(g.sspb, FT);
+    std::vector<int> cand;
+    for (int t : {512, 1024, 1536, 2048, 2560, 3072, 4096}) {
+        const int nch = sap_chunks(g, nrhs, t);
+        bool dup = false;
+        for (int u : cand) dup = dup || sap_chunks(g, nrhs, u) == nch;
+        if (!dup && nch <= maxc) cand.push_back(t);
+    }
+    int best = sap_default_wgs();
+    double bms = 1e30;
+    for (int t : cand) {
+        e.sap_tuned.assign(1, {nrhs, t});
+        float ms = 0, acc = 0;
+        for (int rep = 0; rep < 3; rep++) {
+            hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, x.as<cplx<R>>());
+            HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+            e.sap0(nrhs, v.as<cplx<R>>(), x.as<cplx<R>>(), 1);
+            HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+            HIP_CHECK(hipEventSynchronize(c.ev1));
+            HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+            if (rep > 0) acc += ms;  // the first run warms up
+        }
+        if (acc < bms) {
+            bms = acc;
+            best = t;
+        }
+    }
+    e.sap_tuned.assign(1, {nrhs, best});
+    if (best_ms) *best_ms = bms / 2;
+    return best;
+}
+
+extern "C" int amgs_autotune(amgs_ctx* ctx, int nrhs, int* best_wgs, double* sweep_ms) {
+    if (!ctx) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
+    c.clear_graphs();
+    const int best = c.mixed() ? autotune_sap(c, c.e32, nrhs, sweep_ms) : autotune_sap(c, c.e64, nrhs, sweep_ms);
+    if (best_wgs) *best_wgs = best;
+    return AMGS_OK;
     API_END
 }
 

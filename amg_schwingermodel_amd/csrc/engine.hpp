@@ -143,6 +143,9 @@ template <typename R> struct Engine {
     void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
                 int nu = 0);
     void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
+    // level-0 SAP launch shape chosen by the autotuner, per batch size
+    std::vector<std::pair<int, int>> sap_tuned;
+    int sap_wgs(int nrhs) const;
 };
 
 struct Ctx {

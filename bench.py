@@ -126,6 +126,7 @@ def main():
         t = time.perf_counter()
         sp.setup()
         setup_plain = (time.perf_counter() - t) * 1e3
+        sp.autotune(1)
         sp.set_graph(bool(a.graph))
         ms_plain, it_plain, rr_plain = timed(sp, 1, a.steps, a.warmup, "plain fp64 1 rhs")
         log(f"plain FP64: setup {setup_plain:.0f} ms, solve {ms_plain:.1f} ms, iterations {it_plain[0]}, "
@@ -141,6 +142,7 @@ def main():
         alt = "mixed" if a.precision == "mixed16" else "mixed16"
         sa = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=a.rhs, precision=precs[alt]), U)
         sa.setup()
+        sa.autotune(a.rhs)
         sa.set_graph(bool(a.graph))
         ms_alt, its_alt, rr_alt = timed(sa, a.rhs, a.steps, a.warmup, f"{alt} {a.rhs} rhs")
         out[f"{alt}_batched_alt"] = {"ms_per_rhs": round(ms_alt / a.rhs, 3), "iterations_min": int(its_alt.min()),
@@ -150,6 +152,8 @@ def main():
     t = time.perf_counter()
     sm.setup()
     setup_mixed = (time.perf_counter() - t) * 1e3
+    tuned_wgs, tuned_ms = sm.autotune(a.rhs)
+    log(f"autotuner: SAP launches of {tuned_wgs} workgroups ({tuned_ms:.2f} ms per sweep)")
     sm.set_graph(bool(a.graph))
     ms_step, its, rrs = timed(sm, a.rhs, a.steps, a.warmup, f"mixed {a.rhs} rhs")
     ms_rhs = ms_step / a.rhs
@@ -161,7 +165,8 @@ def main():
     sm.kernel_timing(0, False)
     log(f"mixed batched: setup {setup_mixed:.0f} ms, solve {ms_step:.1f} ms for {a.rhs} rhs -> {ms_rhs:.2f} ms/rhs, "
         f"iterations {its.min()}..{its.max()}, max relres {rrs.max():.2e}")
-    mixed = {"precision": a.precision, "graph": bool(a.graph), "setup_ms": round(setup_mixed, 1),
+    mixed = {"precision": a.precision, "graph": bool(a.graph), "sap_workgroups_autotuned": tuned_wgs,
+             "setup_ms": round(setup_mixed, 1),
              "iterations_min": int(its.min()), "iterations_max": int(its.max()), "relres_max": float(rrs.max()),
              "eager_timed_solve_ms_per_rhs": round(ms_roof / a.rhs, 3)}
     if not a.no_plain:

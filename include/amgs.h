@@ -120,6 +120,15 @@ int amgs_solve_dev(amgs_ctx* c, int nrhs, const void* b_dev, void* x_dev, int* i
  * solves with kernel timing enabled (amgs_kernel_timing) launch eagerly. */
 int amgs_set_graph(amgs_ctx* c, int enable);
 
+/* Autotuner (new): times one level-0 SAP sweep for a set of launch shapes
+ * (workgroups per launch, i.e. lattice chunks per Schwarz block and rhs) and
+ * keeps the fastest for batches of nrhs right-hand sides in this context.
+ * Results stay deterministic: the choice fixes the chunking (and with it the
+ * order of the per-chunk partial sums) for all later solves of that size.
+ * best_wgs (may be NULL) receives the workgroup target, sweep_ms (may be
+ * NULL) the sweep time it measured.  Captured graphs are dropped. */
+int amgs_autotune(amgs_ctx* c, int nrhs, int* best_wgs, double* sweep_ms);
+
 /* Uneven multi-configuration batch (new; the reference solves one gauge
  * configuration per process): the solves of nctx independent contexts - each
  * with its own lattice size, blocking, mass and gauge field, set up

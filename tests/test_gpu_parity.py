@@ -397,3 +397,18 @@ def test_full_size_256_three_levels():
         print(f"  precision {prec}: {itm} iterations, relres {rrm:.2e}, error vs oracle {rel(xm, xo):.2e}")
         assert rrm < 1e-10 and rel(xm, xo) < 1e-8 and itm <= ito + 3
         sm.close()
+
+
+def test_autotune_keeps_results_deterministic():
+    """The SAP launch-shape autotuner picks one of its candidate workgroup
+    targets; afterwards solves converge as before and repeat bit for bit."""
+    s, o, _ = make("64_2lvl", precision=amgs.PREC_MIXED16, max_rhs=2)
+    B = np.stack([cvec(s.size(0), 500 + i) for i in range(2)])
+    x0, it0, rr0, _ = s.solve(B)
+    best, ms = s.autotune(2)
+    assert best in (512, 1024, 1536, 2048, 2560, 3072, 4096) and ms > 0
+    x1, it1, rr1, _ = s.solve(B)
+    x2, it2, rr2, _ = s.solve(B)
+    assert np.array_equal(x1, x2) and np.array_equal(it1, it2)
+    assert np.all(rr1 < 1e-10) and np.all(np.abs(np.asarray(it1) - np.asarray(it0)) <= 1)
+    assert rel(x1, x0) < 1e-8
