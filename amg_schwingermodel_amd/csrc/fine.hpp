@@ -421,55 +421,58 @@ template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(
     }
 }
 
-// Arnoldi step J, part A: prologue finalises step J-1 (or starts cycle k),
-// body w = s_J D_B Vt_J -> Vt_{J+1} (slot 1 with FP16 storage H);
-// partials <w, Vt_i> (i <= J), <Vt_J, Vt_i> (i < J).
+// Arnoldi step J, part A: W = D_B Vt_J (unscaled; step B applies s_J) into
+// Vt_{J+1} (slot 1 with FP16 storage H), partials <W, Vt_i> (i <= J) and
+// <Vt_J, Vt_i> (i < J).  The stream does not depend on the item's scalar
+// state: only chunk 0 of an item first finalises step J-1 (or starts cycle k)
+// and stores the result, which every later kernel reads; the other chunks
+// start streaming at once, skipping only items whose end an earlier kernel
+// recorded.  An item that this kernel's chunk 0 finds finished has streamed
+// once for nothing; step B and the update skip it.
 template <typename R, int J, bool H, bool SQ>
 __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
     if (a.skip && a.skip[t.rhs]) return;
-    load_item(L, a.st, t.item, M);
-    if (J == 0) {
-        reduce_chunks<double>(a.st.qr + (size_t)t.item * a.nchunk * 2, 2, 2, a.nchunk, L.tmp, L.dr);
-    } else {
-        reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
-    }
-    if (threadIdx.x == 0) {
-        const SapState& st = a.st;
-        const bool store = t.chunk == 0;
-        int active = !L.done;
-        if (active && J == 0) {
-            // cycle start (src/fgmres.cpp:8-17, 61-65)
-            const double err = sqrt(L.dr[0]);
-            const double nb = k == 0 ? sqrt(L.dr[1]) : L.normb;
-            const bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
-            if (done) {
-                active = 0;
-                if (store) st.done[t.item] = 1;
-            } else {
-                L.s[0] = 1.0 / err;
-                if (store) {
+    if (t.chunk == 0) {
+        load_item(L, a.st, t.item, M);
+        if (J == 0) {
+            reduce_chunks<double>(a.st.qr + (size_t)t.item * a.nchunk * 2, 2, 2, a.nchunk, L.tmp, L.dr);
+        } else {
+            reduce_chunks<double>(a.st.qb + (size_t)t.item * a.nchunk, 1, 1, a.nchunk, L.tmp, L.dr);
+        }
+        if (threadIdx.x == 0) {
+            const SapState& st = a.st;
+            int active = !L.done;
+            if (active && J == 0) {
+                // cycle start (src/fgmres.cpp:8-17, 61-65)
+                const double err = sqrt(L.dr[0]);
+                const double nb = k == 0 ? sqrt(L.dr[1]) : L.normb;
+                const bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
+                if (done) {
+                    active = 0;
+                    st.done[t.item] = 1;
+                } else {
                     if (k == 0) st.normb[t.item] = nb;
-                    st.s[(size_t)t.item * (M + 1)] = L.s[0];
+                    st.s[(size_t)t.item * (M + 1)] = 1.0 / err;
                     st.gpre[(size_t)t.item * (M + 1)] = c64{err, 0.0};
                     st.jbreak[t.item] = -1;
                 }
+            } else if (active) {
+                if (L.jbreak >= 0) active = 0;
+                else {
+                    sap_finalize_step(st, t.item, L.dr[0], M, J - 1, a.tol, true, L, H ? a.sig : 1.0);
+                    if (L.broke) active = 0;
+                }
             }
-        } else if (active) {
-            if (L.jbreak >= 0) active = 0;
-            else {
-                sap_finalize_step(st, t.item, L.dr[0], M, J - 1, a.tol, store, L, H ? a.sig : 1.0);
-                if (L.broke) active = 0;
-            }
+            if (act && active) atomicAdd(act, 1);
         }
-        L.active = active;
-        if (act && active && t.chunk == 0) atomicAdd(act, 1);
+        __syncthreads();  // L.tmp is reused below
+    } else if (a.st.done[t.item] || (J > 0 && a.st.jbreak[t.item] >= 0)) {
+        // finished in an earlier cycle, or broke down at step J-2 or before
+        return;
     }
-    __syncthreads();
-    if (!L.active) return;
-    const R sj = R(L.s[J]);
     cplx<R>* w = slot(a, H ? 1 : J + 1, t.rhs);
     // per-thread partial sums in the working precision, combined in FP64
     cplx<R> pw[J + 1], pv[J + 1];
@@ -484,8 +487,6 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
             block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1);
         else
             block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1);
-        o0 = sj * o0;
-        o1 = sj * o1;
         w[2 * n] = o0;
         w[2 * n + 1] = o1;
         cplx<R> a0, a1;
@@ -528,8 +529,9 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
             const c64* d = L.dq;
             // Gram row j: G_jk = s_j s_k <Vt_j, Vt_k>, k < j
             for (int kk = 0; kk < j; kk++) L.G[j * (M + 1) + kk] = (L.s[j] * L.s[kk]) * d[j + 1 + kk];
+            // <w, V_i> with w = s_j W (step A stores W = D_B Vt_j unscaled)
             for (int i = 0; i <= j; i++) {
-                c64 h = L.s[i] * d[i];
+                c64 h = (L.s[i] * L.s[j]) * d[i];
                 for (int kk = 0; kk < i; kk++) h -= L.Hcol[kk] * conj(L.G[i * (M + 1) + kk]);
                 L.Hcol[i] = h;
             }
@@ -552,10 +554,11 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
     cplx<R> eta[SAP_MAXM + 1];
 #pragma unroll
     for (int i = 0; i <= SAP_MAXM; i++) eta[i] = i <= j ? cvt<R>(L.eta[i]) : czero<R>();
+    const R sj = R(L.s[j]);
     double nn = 0;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
-        cplx<R> w0 = w[2 * n], w1 = w[2 * n + 1];
+        cplx<R> w0 = sj * w[2 * n], w1 = sj * w[2 * n + 1];
 #pragma unroll
         for (int i = 0; i <= SAP_MAXM; i++) {
             if (i <= j) {
