@@ -374,6 +374,16 @@ template <typename R> static void bind_engine(Ctx& c, Engine<R>& e, DevBuf& U, D
     }
 }
 
+// column-major links [5][ns][D][D] (entry (c, r) at c*D + r) to column-pair
+// order [5][ns][D/2][D][2] (entry (c, r) at (c/2)*2D + 2r + c%2), D even
+__global__ void k_links_pairs(size_t n, int D, const c32* in, c32* out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t msz = (size_t)D * D, site = i / msz;
+    const int w = (int)(i - site * msz), c = w / D, r = w - c * D;
+    out[site * msz + (size_t)(c / 2) * 2 * D + 2 * r + (c & 1)] = in[i];
+}
+
 static void convert64to32(Ctx& c, DevBuf& src, DevBuf& dst, size_t ncomplex) {
     dst.alloc(sizeof(c32) * ncomplex);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(ncomplex, 256)), dim3(256), 0, c.stream, ncomplex,
@@ -454,6 +464,17 @@ static void setup_common(Ctx& c, bool smooth) {
             if (l >= 1) convert64to32(c, c.links64[l], c.links32[l], (size_t)5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
         }
         bind_engine(c, c.e32, c.U32, c.tv32, c.links32);
+        // levels with SAP smoothing: column-pair copies for the block solves
+        for (int l = 1; l < c.L - 1; l++) {
+            const Geo& g = c.geo[l];
+            if (g.dof % 2) continue;
+            const size_t nl = (size_t)5 * g.ns * g.dof * g.dof;
+            c.links2[l].alloc(sizeof(c32) * nl);
+            hipLaunchKernelGGL(k_links_pairs, dim3(cdiv_u(nl, 256)), dim3(256), 0, c.stream, nl, g.dof,
+                               (const c32*)c.links32[l].as<c32>(), c.links2[l].as<c32>());
+            HIP_CHECK(hipGetLastError());
+            c.e32.links2[l] = c.links2[l].as<c32>();
+        }
     }
     c.sync();
     c.setup_done = true;

@@ -29,6 +29,7 @@ enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLON
 template <typename R> struct CLevel {
     Geo g;
     const cplx<R>* links;  // [5][ns][dof][dof], column-major per site
+    const cplx<R>* links2; // FP32 engine, SAP levels: the same with column pairs interleaved per row, or null
     const cplx<R>* w;      // test vectors [ntest][ns*dof]  (levels < L-1)
     const int* red;        // SAP colour lists [ncol]
     const int* black;
@@ -123,6 +124,26 @@ struct PhaseTimer {
     }
     __device__ __forceinline__ void count(int id, unsigned long long v = 1) {
         if (p && threadIdx.x == 0) p[id] += v;
+    }
+};
+
+// per-wavefront phase timers (lane 0 of each wavefront), summed over the
+// wavefronts of the workgroup, only when a buffer is given
+struct WaveTimer {
+    unsigned long long* p;
+    unsigned long long t;
+    __device__ __forceinline__ WaveTimer(unsigned long long* prof) : p(prof), t(0) {
+        if (p && (threadIdx.x & 63) == 0) t = wall_clock64();
+    }
+    __device__ __forceinline__ void lap(int id) {
+        if (p && (threadIdx.x & 63) == 0) {
+            const unsigned long long n = wall_clock64();
+            atomicAdd(p + id, n - t);
+            t = n;
+        }
+    }
+    __device__ __forceinline__ void count(int id) {
+        if (p && (threadIdx.x & 63) == 0) atomicAdd(p + id, 1ull);
     }
 };
 
@@ -415,6 +436,64 @@ __device__ __forceinline__ void w_dlocal_2x2s(const CArgs<R>& A, int l, int blk,
     wave_sync();
 }
 
+// w_dlocal_2x2s with the links in column-pair order [5][ns][dof/2][dof][2]:
+// each lane reads columns c, c+1 of its row, and of its input, with one
+// 16-byte load each, half the vector-memory instructions of the column-major
+// form, which bound the stencil (a workgroup's eight wavefronts share the
+// CU's memory pipeline).  Same products in the same order, so the result is
+// that of w_dlocal_2x2s.  Needs an even dof and 16-byte aligned block vectors.
+template <typename R>
+__device__ __forceinline__ void w_dlocal_2x2p(const CArgs<R>& A, int l, int blk, gptr<cplx<R>> links2,
+                                              const cplx<R>* in, cplx<R>* out, cplx<R>* part) {
+    static_assert(sizeof(R) == 4, "FP32 engine only");
+    using V4 = float __attribute__((ext_vector_type(4)));
+    const Geo& g = A.lv[l].g;
+    const int D = g.dof, nb = 4 * D, nu = 3 * nb, lane = threadIdx.x & 63;
+    const int msz = D * D, ksz = g.ns * msz, np = D / 2;
+    constexpr int S = 4;  // units per lane (nu <= 4 * 64)
+    const __attribute__((address_space(1))) V4* Lp[S];
+    const V4* vp[S];
+    cplx<R> acc[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u0 = lane + 64 * s, u = u0 < nu ? u0 : 0;
+        const int q = u / nb, e = u - q * nb;  // link block (0 A, 1 t hop, 2 x hop) and row
+        const int m = e / D, r = e - m * D;
+        const int n = sap_site(g, blk, m);
+        const int k = q == 0 ? 0 : q == 1 ? ((m & 1) ? 3 : 1) : ((m & 2) ? 4 : 2);
+        Lp[s] = (const __attribute__((address_space(1))) V4*)(links2 + k * ksz + n * msz + 2 * r);
+        vp[s] = (const V4*)(in + (q == 0 ? m : q == 1 ? (m ^ 1) : (m ^ 2)) * D);
+        acc[s] = czero<R>();
+    }
+    constexpr int U = 5;  // column pairs per round
+    for (int p0 = 0; p0 < np; p0 += U) {
+#pragma unroll
+        for (int uu = 0; uu < U; uu++) {
+            const int p = p0 + uu;
+            if (p < np) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const V4 lv = Lp[s][p * D], v = vp[s][p];
+                    acc[s] += cplx<R>{lv.x, lv.y} * cplx<R>{v.x, v.y};
+                    acc[s] += cplx<R>{lv.z, lv.w} * cplx<R>{v.z, v.w};
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++)
+        if (lane + 64 * s < nu) part[lane + 64 * s] = acc[s];
+    wave_sync();
+    for (int e = lane; e < nb; e += 64) {
+        cplx<R> o = A.mm * in[e];
+        o -= part[e];
+        o -= part[nb + e];
+        o -= part[2 * nb + e];
+        out[e] = o;
+    }
+    wave_sync();
+}
+
 // part: wave-private LDS scratch of >= 12 D complex entries (FP32 engine), or null
 template <typename R>
 __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out,
@@ -423,7 +502,8 @@ __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, cons
     if (g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2) {
         if constexpr (std::is_same<R, float>::value) {
             if (part && 12 * g.dof <= 4 * 64) {
-                w_dlocal_2x2s<R>(A, l, blk, as_global(A.lv[l].links), in, out, part);
+                if (A.lv[l].links2) w_dlocal_2x2p<R>(A, l, blk, as_global(A.lv[l].links2), in, out, part);
+                else w_dlocal_2x2s<R>(A, l, blk, as_global(A.lv[l].links), in, out, part);
                 return;
             }
         }
@@ -479,8 +559,10 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
     w_dlocal(A, l, blk, xb, V(1), part);
     for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
     wave_sync();
+    WaveTimer T(K.prof);
     const double normb = sqrt(w_dot(bb, bb, nb).re);
     double err = sqrt(w_dot(V(0), V(0), nb).re);
+    T.lap(24);
     if (normb == 0.0) return;  // zero block right-hand side: x_B = 0
     int maxIt = m;
     for (int k = 0; k < A.sap_restarts; k++) {
@@ -493,6 +575,8 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
         wave_sync();
         for (int j = 0; j < m; j++) {
             w_dlocal(A, l, blk, V(j), V(j + 1), part);
+            T.lap(25);
+            T.count(19);
             cplx<R>* w = V(j + 1);
             for (int i = 0; i <= j; i++) {
                 const c64 h = w_dot(w, V(i), nb);
@@ -502,11 +586,13 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
                 for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
                 wave_sync();
             }
+            T.lap(26);
             const double hn = sqrt(w_dot(w, w, nb).re);
             if (hn > 0) {
                 const R ih = R(1.0 / hn);
                 for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
             }
+            T.lap(27);
             if (lane == 0) {
                 S.col[j + 1] = c64{hn, 0.0};
                 c64 cnj, snj;
@@ -521,6 +607,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
                 S.brk = sqrt(norm2(gn)) < A.sap_gtol * normb;
             }
             wave_sync();
+            T.lap(28);
             if (S.brk) {
                 maxIt = j + 1;
                 break;
@@ -534,6 +621,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
             }
         }
         wave_sync();
+        T.lap(29);
         for (int e = lane; e < nb; e += 64) {
             c64 s = cvt<double>(xb[e]);
             for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(V(j)[e]);
@@ -544,6 +632,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
         for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
         wave_sync();
         err = sqrt(w_dot(V(0), V(0), nb).re);
+        T.lap(30);
         if (err < A.sap_gtol * normb) break;
     }
     for (int e = lane; e < nb; e += 64) {
@@ -847,7 +936,7 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
     (void)Aval;
     const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ CLds S;
-    extern __shared__ char dyn_lds[];
+    extern __shared__ __align__(16) char dyn_lds[];
     const int rhs = blockIdx.x;
     CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
             A.prof ? A.prof + (size_t)rhs * 32 : nullptr};

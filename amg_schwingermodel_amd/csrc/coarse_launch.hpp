@@ -15,6 +15,7 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     a.mm = R(c->P.m0 + 2.0);
     for (int l = 1; l < c->L; l++) {
         a.lv[l].links = links[l];
+        a.lv[l].links2 = links2[l];
         a.lv[l].w = l < c->L - 1 ? tv[l] : nullptr;
     }
     a.mode = mode;

@@ -127,6 +127,7 @@ template <typename R> struct Engine {
     const cplx<R>* U = nullptr;
     const cplx<R>* tv[MAXL] = {};
     const cplx<R>* links[MAXL] = {};
+    const cplx<R>* links2[MAXL] = {};  // column-pair copies for the coarse SAP block stencil (FP32 engine)
     // fine SAP workspace
     DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero;
     SapState st{};
@@ -158,7 +159,7 @@ struct Ctx {
     std::string err;
     bool gauge_set = false, setup_done = false;
     // FP64 hierarchy (always), FP32 copy in mixed mode
-    DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], redl[MAXL], blackl[MAXL];
+    DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], links2[MAXL], redl[MAXL], blackl[MAXL];
     Engine<double> e64;
     Engine<float> e32;
     // outer FGMRES
