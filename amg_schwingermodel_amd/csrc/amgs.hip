@@ -302,9 +302,9 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                 for (int j = 0; j < a.m; j++) {
                     // algorithmic bytes: links of the colour's sites once; per active
                     // item Vt_j read, w written, Vt_i (i < j) read for the dots.
-                    // With FP16 slots: w and slot 0 complex FP32, slots >= 1 FP16.
+                    // With FP16 slots: slot 0 complex FP32, w and slots >= 1 FP16.
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
-                    const double pa = h16 ? 32.0 + 8.0 * j : 2 * S * (2 + j);
+                    const double pa = h16 ? 24.0 + 8.0 * j : 2 * S * (2 + j);
                     int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * pa);
                     if (h16) {
                         if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act);
@@ -315,7 +315,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     }
                     if (act) c->kt[KT_SAP_A].end(s);
                     // w and Vt_i (i <= j) read, w written
-                    const double pb = h16 ? 40.0 + 8.0 * j : 2 * S * (3 + j);
+                    const double pb = h16 ? 32.0 + 8.0 * j : 2 * S * (3 + j);
                     act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * pb);
                     if (h16) hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
                     else hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);

@@ -487,8 +487,18 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
             block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1);
         else
             block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1);
-        w[2 * n] = o0;
-        w[2 * n + 1] = o1;
+        if constexpr (H) {
+            // FP16 storage: W in FP16 too (in the first half of slot 1); the
+            // dot products use the stored values, so that the Hessenberg
+            // column belongs to the vector that step B reads
+            const f16x4 hw = to_h16(o0, o1);
+            reinterpret_cast<f16x4*>(w)[n] = hw;
+            o0 = cplx<R>{R(hw.x), R(hw.y)};
+            o1 = cplx<R>{R(hw.z), R(hw.w)};
+        } else {
+            w[2 * n] = o0;
+            w[2 * n + 1] = o1;
+        }
         cplx<R> a0, a1;
         ld_slot<R, H>(a, J, t.rhs, n, a0, a1);
 #pragma unroll
@@ -558,7 +568,15 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
     double nn = 0;
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
-        cplx<R> w0 = sj * w[2 * n], w1 = sj * w[2 * n + 1];
+        cplx<R> w0, w1;
+        if constexpr (H) {
+            const f16x4 hw = reinterpret_cast<const f16x4*>(w)[n];
+            w0 = sj * cplx<R>{R(hw.x), R(hw.y)};
+            w1 = sj * cplx<R>{R(hw.z), R(hw.w)};
+        } else {
+            w0 = sj * w[2 * n];
+            w1 = sj * w[2 * n + 1];
+        }
 #pragma unroll
         for (int i = 0; i <= SAP_MAXM; i++) {
             if (i <= j) {
