@@ -104,7 +104,7 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     const int m = c->P.sap_m;
     const bool h16 = std::is_same<R, float>::value && c->h16();
     Vt.alloc(sizeof(cplx<R>) * (h16 ? 2 : m + 1) * cap * n0);
-    if (h16) Vh.alloc(sizeof(f16x4) * m * cap * g0.ns);
+    if (h16) Vh.alloc(sizeof(f16x4) * (m + 1) * cap * g0.ns);
     xl.alloc(sizeof(cplx<R>) * cap * n0);
     r.alloc(sizeof(cplx<R>) * cap * n0);
     res0.alloc(sizeof(cplx<R>) * cap * n0);
@@ -125,6 +125,7 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     const size_t o_cn = take(16 * items * M), o_sn = take(16 * items * M);
     const size_t o_qa = take(16 * items * nchunk_max * 2 * M), o_qb = take(8 * items * nchunk_max);
     const size_t o_qr = take(16 * items * nchunk_max);
+    const size_t o_sig0 = take(8 * items);
     st_mem.alloc(off);
     char* b = st_mem.as<char>();
     st.maxIt = (int*)(b + o_maxIt);
@@ -142,10 +143,12 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
     st.qa = (c64*)(b + o_qa);
     st.qb = (double*)(b + o_qb);
     st.qr = (double*)(b + o_qr);
+    st.sig0 = (double*)(b + o_sig0);
     skip.alloc(sizeof(int) * cap);
     const unsigned ntile = cdiv_u(g0.ns, FT);
     part_r.alloc(sizeof(double) * cap * ntile);
     part_v.alloc(sizeof(double) * cap * ntile);
+    rsig.alloc(sizeof(double) * cap);
     // K-cycle(0) and coarse workspace
     const size_t n1 = c->nvec(1);
     eta1.alloc(sizeof(cplx<R>) * cap * n1);
@@ -286,6 +289,10 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     a.st = st;
     const bool h16 = std::is_same<R, float>::value && c->h16();
     a.Vh = h16 ? Vh.as<f16x4>() : nullptr;
+    a.rsig = rsig.as<double>();
+    if (h16)
+        hipLaunchKernelGGL(k_sap_rscale, dim3(nrhs), dim3(FT), 0, s, (const double*)part_v.as<double>(), (int)ntile,
+                           2.0 * g.ns, rsig.as<double>());
     // power-of-two scale of the FP16 slots: a unit-norm block vector has
     // entries of order 1 / sqrt(2 sspb), stored as entries of order 1
     a.sig = h16 ? std::ldexp(1.0f, (int)std::lround(0.5 * std::log2(2.0 * g.sspb))) : 1.0f;
@@ -296,15 +303,20 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
-            if (sq) hipLaunchKernelGGL((k_sap_init<R, true>), grid, dim3(FT), 0, s, a);
-            else hipLaunchKernelGGL((k_sap_init<R, false>), grid, dim3(FT), 0, s, a);
+            if (h16) {
+                if (sq) hipLaunchKernelGGL((k_sap_init<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
+                else hipLaunchKernelGGL((k_sap_init<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
+            } else {
+                if (sq) hipLaunchKernelGGL((k_sap_init<R, false, true>), grid, dim3(FT), 0, s, a);
+                else hipLaunchKernelGGL((k_sap_init<R, false, false>), grid, dim3(FT), 0, s, a);
+            }
             for (int k = 0; k < c->P.sap_restarts; k++) {
                 for (int j = 0; j < a.m; j++) {
                     // algorithmic bytes: links of the colour's sites once; per active
                     // item Vt_j read, w written, Vt_i (i < j) read for the dots.
-                    // With FP16 slots: slot 0 complex FP32, w and slots >= 1 FP16.
+                    // With FP16 slots: w and every slot FP16.
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
-                    const double pa = h16 ? 24.0 + 8.0 * j : 2 * S * (2 + j);
+                    const double pa = h16 ? 16.0 + 8.0 * j : 2 * S * (2 + j);
                     int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * pa);
                     if (h16) {
                         if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act);
@@ -315,7 +327,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     }
                     if (act) c->kt[KT_SAP_A].end(s);
                     // w and Vt_i (i <= j) read, w written
-                    const double pb = h16 ? 32.0 + 8.0 * j : 2 * S * (3 + j);
+                    const double pb = h16 ? 24.0 + 8.0 * j : 2 * S * (3 + j);
                     act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * pb);
                     if (h16) hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
                     else hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
@@ -326,8 +338,13 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                 if (h16) hipLaunchKernelGGL((k_sap_update<R, kH16<R>>), grid, dim3(FT), 0, s, a, xacc);
                 else hipLaunchKernelGGL((k_sap_update<R, false>), grid, dim3(FT), 0, s, a, xacc);
                 if (!last) {
-                    if (sq) hipLaunchKernelGGL((k_sap_resid<R, true>), grid, dim3(FT), 0, s, a);
-                    else hipLaunchKernelGGL((k_sap_resid<R, false>), grid, dim3(FT), 0, s, a);
+                    if (h16) {
+                        if (sq) hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
+                        else hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
+                    } else {
+                        if (sq) hipLaunchKernelGGL((k_sap_resid<R, false, true>), grid, dim3(FT), 0, s, a);
+                        else hipLaunchKernelGGL((k_sap_resid<R, false, false>), grid, dim3(FT), 0, s, a);
+                    }
                 }
             }
             hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),

@@ -129,7 +129,7 @@ template <typename R> struct Engine {
     const cplx<R>* links[MAXL] = {};
     const cplx<R>* links2[MAXL] = {};  // column-pair copies for the coarse SAP block stencil (FP32 engine)
     // fine SAP workspace
-    DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero;
+    DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero, rsig;
     SapState st{};
     int nchunk_max = 0;
     // K-cycle(0) temporaries

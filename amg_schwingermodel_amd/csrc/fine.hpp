@@ -187,7 +187,8 @@ template <typename R> struct SapArgs {
     double tol;
     const cplx<R>* rglob;  // b of each item = global residual restricted to the block
     cplx<R>* Vt;           // (m+1) slots of [nrhs][ns][2]; with FP16 storage only slot 0 (V_0) and slot 1 (w)
-    f16x4* Vh;             // FP16 storage: slots 1..m of [nrhs][ns] (both spins of a site in 8 bytes)
+    f16x4* Vh;             // FP16 storage: slots 0..m of [nrhs][ns] (both spins of a site in 8 bytes)
+    const double* rsig;    // FP16 storage: power-of-two scale of Vt_0 at the start [rhs]
     float sig;             // FP16 storage: power-of-two scale of the stored slots (1 otherwise)
     float rcp_stel;        // 1 / stel (block-local index -> coordinates)
     cplx<R>* xl;           // [nrhs][ns][2] block solutions
@@ -256,20 +257,35 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
     return a.Vt + ((size_t)j * a.nrhs + rhs) * a.g.ns * 2;
 }
-// FP16 slot j >= 1
+// FP16 slot j
 template <typename R> __device__ __forceinline__ f16x4* hslot(const SapArgs<R>& a, int j, int rhs) {
-    return a.Vh + ((size_t)(j - 1) * a.nrhs + rhs) * a.g.ns;
+    return a.Vh + ((size_t)j * a.nrhs + rhs) * a.g.ns;
 }
-// both spins of Krylov slot i at site n: FP16 slots 1..m when H, else complex R
+// both spins of Krylov slot i at site n: FP16 slots when H, else complex R
 template <typename R, bool H>
 __device__ __forceinline__ void ld_slot(const SapArgs<R>& a, int i, int rhs, int n, cplx<R>& b0, cplx<R>& b1) {
     if constexpr (H) {
-        if (i > 0) {
-            SpinH16{hslot(a, i, rhs)}(n, b0, b1);
-            return;
-        }
+        SpinH16{hslot(a, i, rhs)}(n, b0, b1);
+        return;
     }
     SpinPtr<R>{slot(a, i, rhs)}(n, b0, b1);
+}
+// Vt_0 = r at site n; returns |stored|^2.  With FP16 storage the stored
+// values are sg r rounded (sg a power of two that brings the entries to order 1).
+template <typename R, bool H>
+__device__ __forceinline__ double st_slot0(const SapArgs<R>& a, int rhs, int n, cplx<R> r0, cplx<R> r1, double sg) {
+    if constexpr (H) {
+        const float f = float(sg);
+        const f16x4 h = to_h16(c32{f * r0.re, f * r0.im}, c32{f * r1.re, f * r1.im});
+        hslot(a, 0, rhs)[n] = h;
+        return double(h.x) * double(h.x) + double(h.y) * double(h.y) + double(h.z) * double(h.z) +
+               double(h.w) * double(h.w);
+    } else {
+        cplx<R>* v0 = slot(a, 0, rhs);
+        v0[2 * n] = r0;
+        v0[2 * n + 1] = r1;
+        return double(norm2(r0)) + double(norm2(r1));
+    }
 }
 
 // Per-item scalar state held in LDS during a kernel (matrices with row
@@ -388,12 +404,12 @@ __device__ __forceinline__ void park(c64 v, int q, c64* tmp, int nqmax) {
 }
 
 // init: x_B = b_B (x0 = b, src/sap.cpp:61), Vt_0 = b_B - D_B b_B; partials |r|^2, |b|^2
-template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R> a) {
+template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(SapArgs<R> a) {
     __shared__ c64 red[8];
     const ItemIdx t = item_of(a);
     if (a.skip && a.skip[t.rhs]) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
-    cplx<R>* v0 = slot(a, 0, t.rhs);
+    const double sg = H ? a.rsig[t.rhs] : 1.0;
     c64 q[2] = {{0, 0}, {0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const BSite b = bsite(a, t, m);
@@ -404,9 +420,7 @@ template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(
         a.xl[off + 2 * n] = b0;
         a.xl[off + 2 * n + 1] = b1;
         const cplx<R> r0 = b0 - o0, r1 = b1 - o1;
-        v0[2 * n] = r0;
-        v0[2 * n + 1] = r1;
-        q[0].re += double(norm2(r0)) + double(norm2(r1));
+        q[0].re += st_slot0<R, H>(a, t.rhs, n, r0, r1, sg);
         q[1].re += double(norm2(b0)) + double(norm2(b1));
     }
     block_sum<2>(q, red);
@@ -417,6 +431,7 @@ template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_init(
             a.st.maxIt[t.item] = a.m;
             a.st.done[t.item] = 0;
             a.st.jbreak[t.item] = -1;
+            if (H) a.st.sig0[t.item] = sg;
         }
     }
 }
@@ -446,8 +461,9 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
             const SapState& st = a.st;
             int active = !L.done;
             if (active && J == 0) {
-                // cycle start (src/fgmres.cpp:8-17, 61-65)
-                const double err = sqrt(L.dr[0]);
+                // cycle start (src/fgmres.cpp:8-17, 61-65); Vt_0 is stored
+                // scaled by sig0 (1 without FP16 storage)
+                const double est = sqrt(L.dr[0]), err = H ? est / st.sig0[t.item] : est;
                 const double nb = k == 0 ? sqrt(L.dr[1]) : L.normb;
                 const bool done = (k > 0 && err < a.tol * nb) || nb == 0.0;
                 if (done) {
@@ -455,7 +471,7 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
                     st.done[t.item] = 1;
                 } else {
                     if (k == 0) st.normb[t.item] = nb;
-                    st.s[(size_t)t.item * (M + 1)] = 1.0 / err;
+                    st.s[(size_t)t.item * (M + 1)] = 1.0 / est;
                     st.gpre[(size_t)t.item * (M + 1)] = c64{err, 0.0};
                     st.jbreak[t.item] = -1;
                 }
@@ -483,7 +499,7 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
         const BSite b = bsite(a, t, m);
         const int n = b.n;
         cplx<R> o0, o1;
-        if constexpr (H && J > 0)
+        if constexpr (H)
             block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1);
         else
             block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1);
@@ -629,6 +645,13 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
                 L.eta[i] = cdiv(e, L.R[i * M + i]);
             }
             for (int i = 0; i < maxIt; i++) L.eta[i] = L.s[i] * L.eta[i];  // coefficients on the Vt_i
+            if (H && t.chunk == 0) {
+                // scale of the next cycle's stored Vt_0: the residual estimate
+                // spread over the block's 2 sspb entries, brought to order 1
+                const double est = sqrt(norm2(L.gpre[maxIt]));
+                const double want = sqrt(2.0 * a.g.sspb) / est;
+                if (est > 0 && want < 1e30) a.st.sig0[t.item] = exp2(rint(log2(want)));
+            }
         }
         L.maxIt = maxIt;
         L.active = active;
@@ -658,17 +681,20 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update
     }
 }
 
-// True block residual after a cycle: Vt_0 = b_B - D_B x_B, partial |r|^2.
-template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<R> a) {
+// True block residual after a cycle: Vt_0 = b_B - D_B x_B, partial |stored|^2.
+template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_sap_resid(SapArgs<R> a) {
     __shared__ c64 red[8];
     __shared__ int act;
+    __shared__ double sg;
     const ItemIdx t = item_of(a);
     if (a.skip && a.skip[t.rhs]) return;
-    if (threadIdx.x == 0) act = !a.st.done[t.item];
+    if (threadIdx.x == 0) {
+        act = !a.st.done[t.item];
+        sg = H ? a.st.sig0[t.item] : 1.0;
+    }
     __syncthreads();
     if (!act) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
-    cplx<R>* v0 = slot(a, 0, t.rhs);
     c64 q[1] = {{0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const BSite b = bsite(a, t, m);
@@ -676,9 +702,7 @@ template <typename R, bool SQ> __global__ void __launch_bounds__(FT) k_sap_resid
         cplx<R> o0, o1;
         block_stencil<SQ>(a, t, b, SpinPtr<R>{a.xl + off}, o0, o1);
         const cplx<R> r0 = a.rglob[off + 2 * n] - o0, r1 = a.rglob[off + 2 * n + 1] - o1;
-        v0[2 * n] = r0;
-        v0[2 * n + 1] = r1;
-        q[0].re += double(norm2(r0)) + double(norm2(r1));
+        q[0].re += st_slot0<R, H>(a, t.rhs, n, r0, r1, sg);
     }
     block_sum<1>(q, red);
     if (threadIdx.x == 0) a.st.qr[((size_t)t.item * a.nchunk + t.chunk) * 2] = q[0].re;
@@ -698,6 +722,21 @@ __global__ void __launch_bounds__(FT) k_sap_check(const double* part_r, const do
     }
     block_sum<2>(q, red);
     if (threadIdx.x == 0 && sqrt(q[0].re) < tol * sqrt(q[1].re)) skip[rhs] = 1;
+}
+
+// FP16 storage: power-of-two scale rsig[rhs] of the first cycle's stored Vt_0,
+// from |v|^2 partials of the smoother's input: its nent entries brought to
+// order 1 (the residuals the blocks start from are no larger than v)
+__global__ void __launch_bounds__(FT) k_sap_rscale(const double* part_v, int ntile, double nent, double* rsig) {
+    __shared__ c64 red[8];
+    const int rhs = blockIdx.x;
+    c64 q[1] = {{0, 0}};
+    for (int i = threadIdx.x; i < ntile; i += FT) q[0].re += part_v[(size_t)rhs * ntile + i];
+    block_sum<1>(q, red);
+    if (threadIdx.x == 0) {
+        const double want = q[0].re > 0 ? sqrt(nent / q[0].re) : 1.0;
+        rsig[rhs] = want < 1e30 ? exp2(rint(log2(want))) : 1.0;
+    }
 }
 
 // |v|^2 per tile (for SAP's ||v||)

@@ -15,6 +15,7 @@ constexpr int OUT_MAXM = 32;  // max restart length of the outer FGMRES
 struct SapState {
     int *maxIt, *jbreak, *done;
     double *normb, *s;          // s: [item][M+1]
+    double* sig0;               // FP16 storage: power-of-two scale of the stored Vt_0 [item]
     c64 *Hraw, *Hrot;           // [item][(M+1)*M]  (row i, column j at i*M+j)
     c64 *G;                     // [item][(M+1)*(M+1)]  G[i][k] = <V_i, V_k>, k < i
     c64 *gpre, *gpost, *cn, *sn;

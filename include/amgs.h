@@ -225,8 +225,9 @@ int amgs_write_residuals(const char* path, const double* hist, int n);
 /* ---- measurement ---- */
 /* Average device time (ms) of `reps` back-to-back launches of one kernel on
  * the context's stream, bracketed by HIP events on that stream.
- * kernel: 0 = fine D (FP64), 1 = fine D (FP32), 2 = SAP block GMRES sweep,
- * 3 = coarse solve.  Inputs are whatever the context holds. */
+ * kernel: 0 = fine D (FP64), 1 = fine D (FP32, needs a mixed-precision
+ * setup); other ids return AMGS_ERR_ARG (the level-0 SAP sweep is timed by
+ * amgs_autotune).  Inputs are whatever the context holds. */
 int amgs_time_kernel(amgs_ctx* c, int kernel, int nrhs, int reps, double* avg_ms);
 /* Phase timers of the last solve (ms): [0] total, [1] setup, [2] solve. */
 int amgs_last_timings(amgs_ctx* c, double* ms3);
