@@ -445,7 +445,7 @@ template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_s
 // recorded.  An item that this kernel's chunk 0 finds finished has streamed
 // once for nothing; step B and the update skip it.
 template <typename R, int J, bool H, bool SQ>
-__global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
+__global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi_a(SapArgs<R> a, int k, int* act) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(FT) k_sap_arnoldi_a(SapArgs<R> a, int k, int* 
     cplx<R> pw[J + 1], pv[J + 1];
 #pragma unroll
     for (int i = 0; i <= J; i++) pw[i] = pv[i] = czero<R>();
-#pragma unroll 2
+#pragma unroll (J >= 4 ? 1 : 2)
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const BSite b = bsite(a, t, m);
         const int n = b.n;
