@@ -623,7 +623,7 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
 // (src/fgmres.cpp:52-59, 107-115) and update x_B += sum_j eta_j V_j.  With
 // xacc set (last cycle) the block solution is added into the global x
 // instead (src/sap.cpp:117-125), also for items that converged earlier.
-template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_update(SapArgs<R> a, cplx<R>* xacc) {
+template <typename R, bool H> __global__ void __launch_bounds__(FT, (H ? 8 : 1)) k_sap_update(SapArgs<R> a, cplx<R>* xacc) {
     __shared__ ItemLds L;
     const ItemIdx t = item_of(a);
     const int M = a.m;
