@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <type_traits>
 
 #define AMGS_HD __host__ __device__ __forceinline__
 
@@ -29,6 +30,33 @@ template <typename R> AMGS_HD cplx<R> conj(cplx<R> a) { return {a.re, -a.im}; }
 // a * conj(b): the summand of the reference dot product (include/lin_alg_op.h:32-40)
 template <typename R> AMGS_HD cplx<R> mulc(cplx<R> a, cplx<R> b) {
     return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// acc += a b, acc += a conj(b) and acc -= a b.  In FP32 as four fused
+// multiply-adds (contraction does not reach across the separate statements of
+// an accumulation); FP64 keeps the plain expressions and their rounding.
+template <typename R> AMGS_HD void cmad(cplx<R>& acc, cplx<R> a, cplx<R> b) {
+    if constexpr (std::is_same<R, float>::value) {
+        acc.re = fmaf(a.re, b.re, fmaf(-a.im, b.im, acc.re));
+        acc.im = fmaf(a.re, b.im, fmaf(a.im, b.re, acc.im));
+    } else {
+        acc += a * b;
+    }
+}
+template <typename R> AMGS_HD void cmadc(cplx<R>& acc, cplx<R> a, cplx<R> b) {
+    if constexpr (std::is_same<R, float>::value) {
+        acc.re = fmaf(a.re, b.re, fmaf(a.im, b.im, acc.re));
+        acc.im = fmaf(a.im, b.re, fmaf(-a.re, b.im, acc.im));
+    } else {
+        acc += mulc(a, b);
+    }
+}
+template <typename R> AMGS_HD void cmsb(cplx<R>& acc, cplx<R> a, cplx<R> b) {
+    if constexpr (std::is_same<R, float>::value) {
+        acc.re = fmaf(-a.re, b.re, fmaf(a.im, b.im, acc.re));
+        acc.im = fmaf(-a.re, b.im, fmaf(-a.im, b.re, acc.im));
+    } else {
+        acc -= a * b;
+    }
 }
 // i * a and -i * a
 template <typename R> AMGS_HD cplx<R> mul_i(cplx<R> a) { return {-a.im, a.re}; }

@@ -28,12 +28,16 @@ __device__ __forceinline__ int xcd_tile(int wg, int nwg) {
 //   (for D^dagger the same with the forward/backward hops exchanged).
 // MASK(n') says whether neighbour n' contributes (zero Dirichlet for SAP blocks).
 // ---------------------------------------------------------------------------
-// Site readers: the two spin components of site n of one vector.
+// Site readers: the two spin components of site n of one vector.  Site
+// indices are non-negative and below 2^28, so element offsets are formed as
+// unsigned 32-bit values: with a uniform base the loads then take the
+// scalar-base + 32-bit-offset form instead of 64-bit address arithmetic.
 template <typename R> struct SpinPtr {  // complex FP32/FP64 storage [ns][2]
     const cplx<R>* p;
     __device__ __forceinline__ void operator()(int n, cplx<R>& a, cplx<R>& b) const {
-        a = p[2 * n];
-        b = p[2 * n + 1];
+        const unsigned u = 2u * (unsigned)n;
+        a = p[u];
+        b = p[u + 1u];
     }
 };
 // FP16 storage: (re0, im0, re1, im1) of a site in 8 bytes; values are read as
@@ -42,7 +46,7 @@ typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 struct SpinH16 {
     const f16x4* p;
     __device__ __forceinline__ void operator()(int n, c32& a, c32& b) const {
-        const f16x4 v = p[n];
+        const f16x4 v = p[(unsigned)n];
         a = c32{float(v.x), float(v.y)};
         b = c32{float(v.z), float(v.w)};
     }
@@ -59,8 +63,9 @@ __device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in
                                             cplx<R>& o0, cplx<R>& o1) {
     cplx<R> c0, c1;
     in(n, c0, c1);
-    cplx<R> f0 = U[2 * n], f1 = U[2 * n + 1];
-    cplx<R> b0 = conj(U[2 * l0]), b1 = conj(U[2 * l1 + 1]);
+    const unsigned un = 2u * (unsigned)n;
+    cplx<R> f0 = U[un], f1 = U[un + 1u];
+    cplx<R> b0 = conj(U[2u * (unsigned)l0]), b1 = conj(U[2u * (unsigned)l1 + 1u]);
     if (wr) f0 = -f0;
     if (wl) b0 = -b0;
     // neighbour reads without branches: a masked-out neighbour reads the
@@ -277,13 +282,13 @@ __device__ __forceinline__ double st_slot0(const SapArgs<R>& a, int rhs, int n, 
     if constexpr (H) {
         const float f = float(sg);
         const f16x4 h = to_h16(c32{f * r0.re, f * r0.im}, c32{f * r1.re, f * r1.im});
-        hslot(a, 0, rhs)[n] = h;
+        hslot(a, 0, rhs)[(unsigned)n] = h;
         return double(h.x) * double(h.x) + double(h.y) * double(h.y) + double(h.z) * double(h.z) +
                double(h.w) * double(h.w);
     } else {
         cplx<R>* v0 = slot(a, 0, rhs);
-        v0[2 * n] = r0;
-        v0[2 * n + 1] = r1;
+        v0[2u * (unsigned)n] = r0;
+        v0[2u * (unsigned)n + 1u] = r1;
         return double(norm2(r0)) + double(norm2(r1));
     }
 }
@@ -409,6 +414,8 @@ template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_s
     const ItemIdx t = item_of(a);
     if (a.skip && a.skip[t.rhs]) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    const cplx<R>* rg = a.rglob + off;
+    cplx<R>* xl = a.xl + off;
     const double sg = H ? a.rsig[t.rhs] : 1.0;
     c64 q[2] = {{0, 0}, {0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
@@ -416,9 +423,10 @@ template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_s
         const int n = b.n;
         cplx<R> o0, o1;
         block_stencil<SQ>(a, t, b, SpinPtr<R>{a.rglob + off}, o0, o1);
-        const cplx<R> b0 = a.rglob[off + 2 * n], b1 = a.rglob[off + 2 * n + 1];
-        a.xl[off + 2 * n] = b0;
-        a.xl[off + 2 * n + 1] = b1;
+        const unsigned u = 2u * (unsigned)n;
+        const cplx<R> b0 = rg[u], b1 = rg[u + 1u];
+        xl[u] = b0;
+        xl[u + 1u] = b1;
         const cplx<R> r0 = b0 - o0, r1 = b1 - o1;
         q[0].re += st_slot0<R, H>(a, t.rhs, n, r0, r1, sg);
         q[1].re += double(norm2(b0)) + double(norm2(b1));
@@ -508,12 +516,12 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
             // dot products use the stored values, so that the Hessenberg
             // column belongs to the vector that step B reads
             const f16x4 hw = to_h16(o0, o1);
-            reinterpret_cast<f16x4*>(w)[n] = hw;
+            reinterpret_cast<f16x4*>(w)[(unsigned)n] = hw;
             o0 = cplx<R>{R(hw.x), R(hw.y)};
             o1 = cplx<R>{R(hw.z), R(hw.w)};
         } else {
-            w[2 * n] = o0;
-            w[2 * n + 1] = o1;
+            w[2u * (unsigned)n] = o0;
+            w[2u * (unsigned)n + 1u] = o1;
         }
         cplx<R> a0, a1;
         ld_slot<R, H>(a, J, t.rhs, n, a0, a1);
@@ -521,8 +529,12 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
         for (int i = 0; i <= J; i++) {
             cplx<R> b0, b1;
             ld_slot<R, H>(a, i, t.rhs, n, b0, b1);
-            pw[i] += mulc(o0, b0) + mulc(o1, b1);
-            if (i < J) pv[i] += mulc(a0, b0) + mulc(a1, b1);
+            cmadc(pw[i], o0, b0);
+            cmadc(pw[i], o1, b1);
+            if (i < J) {
+                cmadc(pv[i], a0, b0);
+                cmadc(pv[i], a1, b1);
+            }
         }
     }
 #pragma unroll
@@ -586,31 +598,31 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
         const int n = bsite(a, t, m).n;
         cplx<R> w0, w1;
         if constexpr (H) {
-            const f16x4 hw = reinterpret_cast<const f16x4*>(w)[n];
+            const f16x4 hw = reinterpret_cast<const f16x4*>(w)[(unsigned)n];
             w0 = sj * cplx<R>{R(hw.x), R(hw.y)};
             w1 = sj * cplx<R>{R(hw.z), R(hw.w)};
         } else {
-            w0 = sj * w[2 * n];
-            w1 = sj * w[2 * n + 1];
+            w0 = sj * w[2u * (unsigned)n];
+            w1 = sj * w[2u * (unsigned)n + 1u];
         }
 #pragma unroll
         for (int i = 0; i <= SAP_MAXM; i++) {
             if (i <= j) {
                 cplx<R> v0, v1;
                 ld_slot<R, H>(a, i, t.rhs, n, v0, v1);
-                w0 -= eta[i] * v0;
-                w1 -= eta[i] * v1;
+                cmsb(w0, eta[i], v0);
+                cmsb(w1, eta[i], v1);
             }
         }
         if constexpr (H) {
             const float sg = a.sig;
             const f16x4 h = to_h16(c32{sg * w0.re, sg * w0.im}, c32{sg * w1.re, sg * w1.im});
-            hslot(a, j + 1, t.rhs)[n] = h;
+            hslot(a, j + 1, t.rhs)[(unsigned)n] = h;
             nn += double(h.x) * double(h.x) + double(h.y) * double(h.y) + double(h.z) * double(h.z) +
                   double(h.w) * double(h.w);
         } else {
-            w[2 * n] = w0;
-            w[2 * n + 1] = w1;
+            w[2u * (unsigned)n] = w0;
+            w[2u * (unsigned)n + 1u] = w1;
             nn += double(norm2(w0)) + double(norm2(w1));
         }
     }
@@ -659,24 +671,26 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT, (H ? 8 : 1))
     __syncthreads();
     if (!L.active && !xacc) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    cplx<R>* xl = a.xl + off;
     const int mi = L.maxIt;
     // x_B += sum_j eta_j Vt_j in the engine's precision
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
-        cplx<R> x0 = a.xl[off + 2 * n], x1 = a.xl[off + 2 * n + 1];
+        const unsigned u = 2u * (unsigned)n;
+        cplx<R> x0 = xl[u], x1 = xl[u + 1u];
         for (int jj = 0; jj < mi; jj++) {
             cplx<R> v0, v1;
             ld_slot<R, H>(a, jj, t.rhs, n, v0, v1);
             const cplx<R> c = cvt<R>(L.eta[jj]);
-            x0 += c * v0;
-            x1 += c * v1;
+            cmad(x0, c, v0);
+            cmad(x1, c, v1);
         }
         if (xacc) {
-            xacc[off + 2 * n] += x0;
-            xacc[off + 2 * n + 1] += x1;
+            xacc[off + u] += x0;
+            xacc[off + u + 1u] += x1;
         } else {
-            a.xl[off + 2 * n] = x0;
-            a.xl[off + 2 * n + 1] = x1;
+            xl[u] = x0;
+            xl[u + 1u] = x1;
         }
     }
 }
@@ -695,13 +709,15 @@ template <typename R, bool H, bool SQ> __global__ void __launch_bounds__(FT) k_s
     __syncthreads();
     if (!act) return;
     const size_t off = (size_t)t.rhs * a.g.ns * 2;
+    const cplx<R>* rg = a.rglob + off;
     c64 q[1] = {{0, 0}};
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const BSite b = bsite(a, t, m);
         const int n = b.n;
         cplx<R> o0, o1;
         block_stencil<SQ>(a, t, b, SpinPtr<R>{a.xl + off}, o0, o1);
-        const cplx<R> r0 = a.rglob[off + 2 * n] - o0, r1 = a.rglob[off + 2 * n + 1] - o1;
+        const unsigned u = 2u * (unsigned)n;
+        const cplx<R> r0 = rg[u] - o0, r1 = rg[u + 1u] - o1;
         q[0].re += st_slot0<R, H>(a, t.rhs, n, r0, r1, sg);
     }
     block_sum<1>(q, red);
