@@ -45,6 +45,7 @@ template <typename R> struct SpinPtr {  // complex FP32/FP64 storage [ns][2]
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 struct SpinH16 {
     const f16x4* p;
+    __device__ __forceinline__ f16x4 raw(int n) const { return p[(unsigned)n]; }
     __device__ __forceinline__ void operator()(int n, c32& a, c32& b) const {
         const f16x4 v = p[(unsigned)n];
         a = c32{float(v.x), float(v.y)};
@@ -56,12 +57,12 @@ __device__ __forceinline__ f16x4 to_h16(c32 a, c32 b) {
 }
 
 // Stencil core on given neighbours r0 = n+t, r1 = n+x, l0 = n-t, l1 = n-x,
-// their masks and the antiperiodic wraps of the time hops.
+// their masks and the antiperiodic wraps of the time hops; c0, c1 return the
+// site's own input values.
 template <typename R, bool DAG, typename In>
 __device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in, int n, int r0, int r1, int l0, int l1,
                                             bool mr0, bool mr1, bool ml0, bool ml1, bool wr, bool wl, R mm,
-                                            cplx<R>& o0, cplx<R>& o1) {
-    cplx<R> c0, c1;
+                                            cplx<R>& o0, cplx<R>& o1, cplx<R>& c0, cplx<R>& c1) {
     in(n, c0, c1);
     const unsigned un = 2u * (unsigned)n;
     cplx<R> f0 = U[un], f1 = U[un + 1u];
@@ -71,27 +72,70 @@ __device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in
     // neighbour reads without branches: a masked-out neighbour reads the
     // site's own (cached) entries and its term is replaced by zero, so the four
     // loads are in flight together
-    cplx<R> q00, q01, q10, q11, q20, q21, q30, q31;
-    in(mr0 ? r0 : n, q00, q01);
-    in(mr1 ? r1 : n, q10, q11);
-    in(ml0 ? l0 : n, q20, q21);
-    in(ml1 ? l1 : n, q30, q31);
+    // spin projections of the four neighbours (right t, right x, left t,
+    // left x): D takes p0 - p1, p0 + i p1, p0 + p1, p0 - i p1; D^dagger the
+    // opposite signs
+    cplx<R> dr0, dr1, dl0, dl1;
+    if constexpr (std::is_same<In, SpinH16>::value) {
+        // straight from the FP16 entries (x, y) = p0, (z, w) = p1: b + s a as
+        // one mixed-precision FMA, the exact sum rounded once, as in FP32
+        const f16x4 h0 = in.raw(mr0 ? r0 : n), h1 = in.raw(mr1 ? r1 : n);
+        const f16x4 h2 = in.raw(ml0 ? l0 : n), h3 = in.raw(ml1 ? l1 : n);
+        // (the +-1 pass through an empty asm: folded to an add, the FMA would
+        // become two conversions and an add)
+        float p1 = 1.f, m1 = -1.f;
+        __asm__("" : "+s"(p1), "+s"(m1));
+        auto pm = [](_Float16 a, float sg, _Float16 b) { return fmaf((float)a, sg, (float)b); };
+        auto sum = [&](f16x4 v, bool neg) {  // p0 +- p1
+            return cplx<R>{pm(v.z, neg ? m1 : p1, v.x), pm(v.w, neg ? m1 : p1, v.y)};
+        };
+        auto sumi = [&](f16x4 v, bool neg) {  // p0 +- i p1
+            return cplx<R>{pm(v.w, neg ? p1 : m1, v.x), pm(v.z, neg ? m1 : p1, v.y)};
+        };
+        dr0 = sum(h0, !DAG);
+        dr1 = sumi(h1, DAG);
+        dl0 = sum(h2, DAG);
+        dl1 = sumi(h3, !DAG);
+    } else {
+        cplx<R> q00, q01, q10, q11, q20, q21, q30, q31;
+        in(mr0 ? r0 : n, q00, q01);
+        in(mr1 ? r1 : n, q10, q11);
+        in(ml0 ? l0 : n, q20, q21);
+        in(ml1 ? l1 : n, q30, q31);
+        if (!DAG) {
+            dr0 = q00 - q01;
+            dr1 = q10 + mul_i(q11);
+            dl0 = q20 + q21;
+            dl1 = q30 - mul_i(q31);
+        } else {
+            dr0 = q00 + q01;
+            dr1 = q10 - mul_i(q11);
+            dl0 = q20 - q21;
+            dl1 = q30 + mul_i(q31);
+        }
+    }
     const cplx<R> z = czero<R>();
     cplx<R> A, B, C, E;
     if (!DAG) {
-        A = mr0 ? f0 * (q00 - q01) : z;
-        B = mr1 ? f1 * (q10 + mul_i(q11)) : z;
-        C = ml0 ? b0 * (q20 + q21) : z;
-        E = ml1 ? b1 * (q30 - mul_i(q31)) : z;
+        A = mr0 ? f0 * dr0 : z;
+        B = mr1 ? f1 * dr1 : z;
+        C = ml0 ? b0 * dl0 : z;
+        E = ml1 ? b1 * dl1 : z;
     } else {
-        A = ml0 ? b0 * (q20 - q21) : z;
-        B = ml1 ? b1 * (q30 + mul_i(q31)) : z;
-        C = mr0 ? f0 * (q00 + q01) : z;
-        E = mr1 ? f1 * (q10 - mul_i(q11)) : z;
+        A = ml0 ? b0 * dl0 : z;
+        B = ml1 ? b1 * dl1 : z;
+        C = mr0 ? f0 * dr0 : z;
+        E = mr1 ? f1 * dr1 : z;
     }
     const R h = R(0.5);
-    o0 = mm * c0 - h * (A + B + C + E);
-    o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+    if constexpr (std::is_same<R, float>::value) {
+        const cplx<R> s0 = (A + B) + (C + E), s1 = (C - A) + mul_i(E - B);
+        o0 = cplx<R>{fmaf(-h, s0.re, mm * c0.re), fmaf(-h, s0.im, mm * c0.im)};
+        o1 = cplx<R>{fmaf(-h, s1.re, mm * c1.re), fmaf(-h, s1.im, mm * c1.im)};
+    } else {
+        o0 = mm * c0 - h * (A + B + C + E);
+        o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
+    }
 }
 
 template <typename R, bool DAG, typename Mask, typename In>
@@ -99,8 +143,9 @@ __device__ __forceinline__ void wilson_site_in(const cplx<R>* __restrict__ U, In
                                                Mask mask, cplx<R>& o0, cplx<R>& o1) {
     const int r0 = nb_right(n, 0, nx, nt), r1 = nb_right(n, 1, nx, nt);
     const int l0 = nb_left(n, 0, nx, nt), l1 = nb_left(n, 1, nx, nt);
+    cplx<R> c0, c1;
     wilson_core<R, DAG>(U, in, n, r0, r1, l0, l1, mask(r0), mask(r1), mask(l0), mask(l1), wraps_right(n, 0, nt),
-                        wraps_left(n, 0, nt), mm, o0, o1);
+                        wraps_left(n, 0, nt), mm, o0, o1, c0, c1);
 }
 
 template <typename R, bool DAG, typename Mask>
@@ -241,9 +286,10 @@ template <typename R> __device__ __forceinline__ BSite bsite(const SapArgs<R>& a
 // (src/level.cpp:304-368).  On square lattices a neighbour's membership
 // follows from the local coordinates; otherwise it is the reference's
 // getMandBlock test with its x = n / Nx (include/level.h:48-59).
+// c0, c1 return the site's own input values.
 template <bool SQ, typename R, typename In>
 __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx& t, const BSite& b, In in, cplx<R>& o0,
-                                              cplx<R>& o1) {
+                                              cplx<R>& o1, cplx<R>& c0, cplx<R>& c1) {
     const Geo& g = a.g;
     if constexpr (SQ) {
         const bool tr = b.t + 1 == g.nt, tl = b.t == 0;
@@ -253,10 +299,17 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
         const int l1 = b.x == 0 ? b.n + (g.nx - 1) * g.nt : b.n - g.nt;
         const bool wt = g.stel == g.nt, wx = g.sxel == g.nx;  // block spans the extent
         wilson_core<R, false>(a.U, in, b.n, r0, r1, l0, l1, wt || b.mt + 1 < g.stel, wx || b.mx + 1 < g.sxel,
-                              wt || b.mt > 0, wx || b.mx > 0, tr, tl, a.mm, o0, o1);
+                              wt || b.mt > 0, wx || b.mx > 0, tr, tl, a.mm, o0, o1, c0, c1);
     } else {
         wilson_site_in<R, false>(a.U, in, b.n, g.nx, g.nt, a.mm, InBlock{g, t.blk}, o0, o1);
+        in(b.n, c0, c1);
     }
+}
+template <bool SQ, typename R, typename In>
+__device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx& t, const BSite& b, In in, cplx<R>& o0,
+                                              cplx<R>& o1) {
+    cplx<R> c0, c1;
+    block_stencil<SQ>(a, t, b, in, o0, o1, c0, c1);
 }
 
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
@@ -506,11 +559,12 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const BSite b = bsite(a, t, m);
         const int n = b.n;
-        cplx<R> o0, o1;
+        // a0, a1: Vt_J at the site, read once by the stencil
+        cplx<R> o0, o1, a0, a1;
         if constexpr (H)
-            block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1);
+            block_stencil<SQ>(a, t, b, SpinH16{hslot(a, J, t.rhs)}, o0, o1, a0, a1);
         else
-            block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1);
+            block_stencil<SQ>(a, t, b, SpinPtr<R>{slot(a, J, t.rhs)}, o0, o1, a0, a1);
         if constexpr (H) {
             // FP16 storage: W in FP16 too (in the first half of slot 1); the
             // dot products use the stored values, so that the Hessenberg
@@ -523,12 +577,10 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
             w[2u * (unsigned)n] = o0;
             w[2u * (unsigned)n + 1u] = o1;
         }
-        cplx<R> a0, a1;
-        ld_slot<R, H>(a, J, t.rhs, n, a0, a1);
 #pragma unroll
         for (int i = 0; i <= J; i++) {
-            cplx<R> b0, b1;
-            ld_slot<R, H>(a, i, t.rhs, n, b0, b1);
+            cplx<R> b0 = a0, b1 = a1;
+            if (i < J) ld_slot<R, H>(a, i, t.rhs, n, b0, b1);
             cmadc(pw[i], o0, b0);
             cmadc(pw[i], o1, b1);
             if (i < J) {
