@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--no-alt", action="store_true", help="skip timing the other inner precision beside it")
     ap.add_argument("--graph", type=int, default=1,
                     help="1: replay the level-0 preconditioner from a captured HIP graph in the timed solves")
+    ap.add_argument("--split", type=int, default=2,
+                    help="also time the batch as this many concurrent equal sub-batches (amgs_solve_multi, one "
+                         "context and stream each); value is the faster of the two forms (1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -172,8 +175,43 @@ def main():
     if not a.no_plain:
         x0 = Xd[0].cpu().numpy().view(np.complex128)
         mixed["x_relerr_vs_fp64_rhs0"] = float(np.linalg.norm(x0 - xs_plain) / np.linalg.norm(xs_plain))
-        out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] / ms_rhs, 2)
     out["mixed_batched"] = mixed
+    ms_best, concurrent = ms_step, 1
+
+    # ---- the same rhs as concurrent sub-batches, one context and stream each:
+    # one sub-batch's VALU-bound SAP step A overlaps another's HBM-bound step B
+    if a.split > 1 and a.rhs % a.split == 0:
+        per = a.rhs // a.split
+        ss = []
+        for i in range(a.split):
+            s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=per, precision=precs[a.precision]), U)
+            s.setup()
+            s.autotune(per)
+            s.set_graph(bool(a.graph))
+            ss.append(s)
+        bp = [Bd[i * per].data_ptr() for i in range(a.split)]
+        xp = [Xd[i * per].data_ptr() for i in range(a.split)]
+        for w in range(a.warmup):
+            amgs.solve_multi(ss, bp, xp, [per] * a.split)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(a.steps):
+            st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, [per] * a.split)
+            log(f"{a.split} x {per} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
+        torch.cuda.synchronize()
+        ms_split = (time.perf_counter() - t) * 1e3 / a.steps
+        its_s, rr_s = np.concatenate(its_s), np.concatenate(rr_s)
+        log(f"{a.split} x {per} rhs concurrent: {ms_split:.1f} ms -> {ms_split / a.rhs:.2f} ms/rhs, iterations "
+            f"{its_s.min()}..{its_s.max()}, max relres {rr_s.max():.2e}, status {st.tolist()}")
+        out["mixed_batched_concurrent"] = {"sub_batches": a.split, "ms_per_rhs": round(ms_split / a.rhs, 3),
+                                           "iterations_min": int(its_s.min()), "iterations_max": int(its_s.max()),
+                                           "relres_max": float(rr_s.max()), "status": st.tolist()}
+        if ms_split < ms_best and rr_s.max() < 1e-10 and all(v == 0 for v in st.tolist()):
+            ms_best, concurrent = ms_split, a.split
+        for s in ss:
+            s.close()
+    if "plain_fp64" in out:
+        out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] * a.rhs / ms_best, 2)
 
     # ---- roofline: SAP Arnoldi step A over the kernel-timing solve ----------
     avg_us = k_ms * 1e3 / max(n_launch, 1)
@@ -233,15 +271,16 @@ def main():
                                          f"({t_setup:.1f}s, not in the value), then {what}",
                                "iterations": int(it_cpu)}
 
-    res = {"metric": METRIC, "value": round(ms_rhs, 3), "unit": "ms/RHS", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": False, "scaling": "weak",
+    res = {"metric": METRIC, "value": round(ms_best / a.rhs, 3), "unit": "ms/RHS", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_best, 2), "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None,
            "dtype": "f32+f64" if a.precision == "mixed" else "f16+f32+f64", "data": "synthetic",
            "config": {"workload": f"{nx}x{nt} 3-level K-cycle FGMRES(20,50) to 1e-10, {a.rhs} Gaussian rhs "
                                   f"batched, beta={BETA} Metropolis ({SWEEPS} sweeps), m0={a.m0} "
                                   f"(named m0={M0_NAMED} stagnates; sampled in named_mass_sample), "
                                   f"rows 0 8 8 10 4 4 / 1 4 4 10 4 4",
-                      "nrhs_batch": a.rhs, "levels": 3, "m0": a.m0, "parallelism": "single GPU"}}
+                      "nrhs_batch": a.rhs, "concurrent_sub_batches": concurrent, "levels": 3, "m0": a.m0,
+                      "parallelism": "single GPU"}}
     res.update(out)
     print(json.dumps(res), flush=True)
 
