@@ -186,7 +186,8 @@ def main():
         for i in range(a.split):
             s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=per, precision=precs[a.precision]), U)
             s.setup()
-            s.autotune(per)
+            wgs_i, ms_i = s.autotune(per)
+            log(f"sub-batch {i}: autotuner: SAP launches of {wgs_i} workgroups ({ms_i:.2f} ms per sweep)")
             s.set_graph(bool(a.graph))
             ss.append(s)
         bp = [Bd[i * per].data_ptr() for i in range(a.split)]
