@@ -297,6 +297,10 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     // entries of order 1 / sqrt(2 sspb), stored as entries of order 1
     a.sig = h16 ? std::ldexp(1.0f, (int)std::lround(0.5 * std::log2(2.0 * g.sspb))) : 1.0f;
     a.rcp_stel = 1.0f / (float)g.stel;
+    // sites in pairs along t in step A: FP16 storage, square lattice, blocks of
+    // even t extent that span neither direction, chunks of even length
+    a.pairs = h16 && g.nx == g.nt && g.stel % 2 == 0 && g.stel < g.nt && g.sxel < g.nx && a.cs % 2 == 0 &&
+              g.sspb % 2 == 0 && g.nt % 2 == 0;
     // square lattice: block membership of a neighbour from local coordinates
     const bool sq = g.nx == g.nt;
     const dim3 grid(g.ncol * a.nchunk, nrhs);

@@ -56,64 +56,31 @@ __device__ __forceinline__ f16x4 to_h16(c32 a, c32 b) {
     return f16x4{(_Float16)a.re, (_Float16)a.im, (_Float16)b.re, (_Float16)b.im};
 }
 
-// Stencil core on given neighbours r0 = n+t, r1 = n+x, l0 = n-t, l1 = n-x,
-// their masks and the antiperiodic wraps of the time hops; c0, c1 return the
-// site's own input values.
-template <typename R, bool DAG, typename In>
-__device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in, int n, int r0, int r1, int l0, int l1,
-                                            bool mr0, bool mr1, bool ml0, bool ml1, bool wr, bool wl, R mm,
-                                            cplx<R>& o0, cplx<R>& o1, cplx<R>& c0, cplx<R>& c1) {
-    in(n, c0, c1);
-    const unsigned un = 2u * (unsigned)n;
-    cplx<R> f0 = U[un], f1 = U[un + 1u];
-    cplx<R> b0 = conj(U[2u * (unsigned)l0]), b1 = conj(U[2u * (unsigned)l1 + 1u]);
-    if (wr) f0 = -f0;
-    if (wl) b0 = -b0;
-    // neighbour reads without branches: a masked-out neighbour reads the
-    // site's own (cached) entries and its term is replaced by zero, so the four
-    // loads are in flight together
-    // spin projections of the four neighbours (right t, right x, left t,
-    // left x): D takes p0 - p1, p0 + i p1, p0 + p1, p0 - i p1; D^dagger the
-    // opposite signs
-    cplx<R> dr0, dr1, dl0, dl1;
-    if constexpr (std::is_same<In, SpinH16>::value) {
-        // straight from the FP16 entries (x, y) = p0, (z, w) = p1: b + s a as
-        // one mixed-precision FMA, the exact sum rounded once, as in FP32
-        const f16x4 h0 = in.raw(mr0 ? r0 : n), h1 = in.raw(mr1 ? r1 : n);
-        const f16x4 h2 = in.raw(ml0 ? l0 : n), h3 = in.raw(ml1 ? l1 : n);
-        // (the +-1 pass through an empty asm: folded to an add, the FMA would
-        // become two conversions and an add)
-        float p1 = 1.f, m1 = -1.f;
-        __asm__("" : "+s"(p1), "+s"(m1));
-        auto pm = [](_Float16 a, float sg, _Float16 b) { return fmaf((float)a, sg, (float)b); };
-        auto sum = [&](f16x4 v, bool neg) {  // p0 +- p1
-            return cplx<R>{pm(v.z, neg ? m1 : p1, v.x), pm(v.w, neg ? m1 : p1, v.y)};
-        };
-        auto sumi = [&](f16x4 v, bool neg) {  // p0 +- i p1
-            return cplx<R>{pm(v.w, neg ? p1 : m1, v.x), pm(v.z, neg ? m1 : p1, v.y)};
-        };
-        dr0 = sum(h0, !DAG);
-        dr1 = sumi(h1, DAG);
-        dl0 = sum(h2, DAG);
-        dl1 = sumi(h3, !DAG);
-    } else {
-        cplx<R> q00, q01, q10, q11, q20, q21, q30, q31;
-        in(mr0 ? r0 : n, q00, q01);
-        in(mr1 ? r1 : n, q10, q11);
-        in(ml0 ? l0 : n, q20, q21);
-        in(ml1 ? l1 : n, q30, q31);
-        if (!DAG) {
-            dr0 = q00 - q01;
-            dr1 = q10 + mul_i(q11);
-            dl0 = q20 + q21;
-            dl1 = q30 - mul_i(q31);
-        } else {
-            dr0 = q00 + q01;
-            dr1 = q10 - mul_i(q11);
-            dl0 = q20 - q21;
-            dl1 = q30 + mul_i(q31);
-        }
+// Spin projections of FP16 site entries (x, y) = p0, (z, w) = p1 straight
+// from the halves: b + s a as one mixed-precision FMA, the exact sum rounded
+// once, as in FP32.  The +-1 pass through an empty asm: folded to an add, the
+// FMA would become two conversions and an add.
+struct H16Proj {
+    float p1, m1;
+    __device__ __forceinline__ H16Proj() : p1(1.f), m1(-1.f) { __asm__("" : "+s"(p1), "+s"(m1)); }
+    __device__ __forceinline__ static float pm(_Float16 a, float sg, _Float16 b) { return fmaf((float)a, sg, (float)b); }
+    __device__ __forceinline__ c32 sum(f16x4 v, bool neg) const {  // p0 +- p1
+        return c32{pm(v.z, neg ? m1 : p1, v.x), pm(v.w, neg ? m1 : p1, v.y)};
     }
+    __device__ __forceinline__ c32 sumi(f16x4 v, bool neg) const {  // p0 +- i p1
+        return c32{pm(v.w, neg ? p1 : m1, v.x), pm(v.z, neg ? m1 : p1, v.y)};
+    }
+};
+
+// The hop terms of a site from its neighbours' spin projections (right t,
+// right x, left t, left x: D takes p0 - p1, p0 + i p1, p0 + p1, p0 - i p1,
+// D^dagger the opposite signs), its links (with the antiperiodic signs
+// applied) and the neighbour masks:
+//   out0 = mm c0 - (A+B+C+E)/2,  out1 = mm c1 - (-A-iB+C+iE)/2.
+template <typename R, bool DAG>
+__device__ __forceinline__ void wilson_combine(cplx<R> c0, cplx<R> c1, cplx<R> dr0, cplx<R> dr1, cplx<R> dl0,
+                                               cplx<R> dl1, cplx<R> f0, cplx<R> f1, cplx<R> b0, cplx<R> b1, bool mr0,
+                                               bool mr1, bool ml0, bool ml1, R mm, cplx<R>& o0, cplx<R>& o1) {
     const cplx<R> z = czero<R>();
     cplx<R> A, B, C, E;
     if (!DAG) {
@@ -136,6 +103,55 @@ __device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in
         o0 = mm * c0 - h * (A + B + C + E);
         o1 = mm * c1 - h * ((C - A) + mul_i(E - B));
     }
+}
+
+// Stencil core on given neighbours r0 = n+t, r1 = n+x, l0 = n-t, l1 = n-x,
+// their masks and the antiperiodic wraps of the time hops; c0, c1 return the
+// site's own input values.
+template <typename R, bool DAG, typename In>
+__device__ __forceinline__ void wilson_core(const cplx<R>* __restrict__ U, In in, int n, int r0, int r1, int l0, int l1,
+                                            bool mr0, bool mr1, bool ml0, bool ml1, bool wr, bool wl, R mm,
+                                            cplx<R>& o0, cplx<R>& o1, cplx<R>& c0, cplx<R>& c1) {
+    in(n, c0, c1);
+    const unsigned un = 2u * (unsigned)n;
+    cplx<R> f0 = U[un], f1 = U[un + 1u];
+    cplx<R> b0 = conj(U[2u * (unsigned)l0]), b1 = conj(U[2u * (unsigned)l1 + 1u]);
+    if (wr) f0 = -f0;
+    if (wl) b0 = -b0;
+    // neighbour reads without branches: a masked-out neighbour reads the
+    // site's own (cached) entries and its term is replaced by zero, so the four
+    // loads are in flight together
+    // spin projections of the four neighbours (right t, right x, left t,
+    // left x): D takes p0 - p1, p0 + i p1, p0 + p1, p0 - i p1; D^dagger the
+    // opposite signs
+    cplx<R> dr0, dr1, dl0, dl1;
+    if constexpr (std::is_same<In, SpinH16>::value) {
+        const f16x4 h0 = in.raw(mr0 ? r0 : n), h1 = in.raw(mr1 ? r1 : n);
+        const f16x4 h2 = in.raw(ml0 ? l0 : n), h3 = in.raw(ml1 ? l1 : n);
+        const H16Proj P;
+        dr0 = P.sum(h0, !DAG);
+        dr1 = P.sumi(h1, DAG);
+        dl0 = P.sum(h2, DAG);
+        dl1 = P.sumi(h3, !DAG);
+    } else {
+        cplx<R> q00, q01, q10, q11, q20, q21, q30, q31;
+        in(mr0 ? r0 : n, q00, q01);
+        in(mr1 ? r1 : n, q10, q11);
+        in(ml0 ? l0 : n, q20, q21);
+        in(ml1 ? l1 : n, q30, q31);
+        if (!DAG) {
+            dr0 = q00 - q01;
+            dr1 = q10 + mul_i(q11);
+            dl0 = q20 + q21;
+            dl1 = q30 - mul_i(q31);
+        } else {
+            dr0 = q00 + q01;
+            dr1 = q10 - mul_i(q11);
+            dl0 = q20 - q21;
+            dl1 = q30 + mul_i(q31);
+        }
+    }
+    wilson_combine<R, DAG>(c0, c1, dr0, dr1, dl0, dl1, f0, f1, b0, b1, mr0, mr1, ml0, ml1, mm, o0, o1);
 }
 
 template <typename R, bool DAG, typename Mask, typename In>
@@ -240,6 +256,7 @@ template <typename R> struct SapArgs {
     f16x4* Vh;             // FP16 storage: slots 0..m of [nrhs][ns] (both spins of a site in 8 bytes)
     const double* rsig;    // FP16 storage: power-of-two scale of Vt_0 at the start [rhs]
     float sig;             // FP16 storage: power-of-two scale of the stored slots (1 otherwise)
+    int pairs;             // step A may take sites in pairs along t (block_stencil_pair's conditions)
     float rcp_stel;        // 1 / stel (block-local index -> coordinates)
     cplx<R>* xl;           // [nrhs][ns][2] block solutions
     const int* skip;       // per-rhs: SAP already converged
@@ -310,6 +327,53 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
                                               cplx<R>& o1) {
     cplx<R> c0, c1;
     block_stencil<SQ>(a, t, b, in, o0, o1, c0, c1);
+}
+
+// D_B at the two sites n, n+1 of a pair along t (the block-local t of site n
+// even; square lattice, FP16 input, blocks of even t extent that do not span
+// the lattice).  Each site is the other's t-neighbour, so the pair reads its
+// own entries, its x-neighbour pairs and its links with 16-byte loads and only
+// the two outer t-neighbours singly: about half the loads and index arithmetic
+// of two single-site stencils, with the same arithmetic per site.
+// o[s][spin] are the outputs, c[s][spin] the pair's own input values.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ void block_stencil_pair(const SapArgs<float>& a, const BSite& b,
+                                                   const f16x4* __restrict__ in, c32 (&o)[2][2], c32 (&c)[2][2]) {
+    const Geo& g = a.g;
+    const int n = b.n;
+    const bool tl = b.t == 0, tr1 = b.t + 2 == g.nt;  // wraps: site n to the left, site n+1 to the right
+    const bool ml0 = b.mt > 0, mr1t = b.mt + 2 < g.stel;         // outer t-neighbours in the block
+    const bool mx_r = b.mx + 1 < g.sxel, mx_l = b.mx > 0;        // x-neighbours in the block
+    const int l0 = tl ? n + g.nt - 1 : n - 1;
+    const int r0b = tr1 ? n + 1 - (b.t + 1) : n + 2;
+    const int xr = b.x + 1 == g.nx ? b.t : n + g.nt;
+    const int xl = b.x == 0 ? n + (g.nx - 1) * g.nt : n - g.nt;
+    // loads: a masked-out neighbour reads the pair's own entries (its term is zero)
+    const f16x8 cp = *reinterpret_cast<const f16x8*>(in + (unsigned)n);
+    const f16x8 rp = *reinterpret_cast<const f16x8*>(in + (unsigned)(mx_r ? xr : n));
+    const f16x8 lp = *reinterpret_cast<const f16x8*>(in + (unsigned)(mx_l ? xl : n));
+    const f16x4 tL = in[(unsigned)(ml0 ? l0 : n)], tR = in[(unsigned)(mr1t ? r0b : n)];
+    using F4 = float __attribute__((ext_vector_type(4)));
+    const F4* U4 = reinterpret_cast<const F4*>(a.U);
+    const F4 ua = U4[(unsigned)n], ub = U4[(unsigned)n + 1u];  // f0, f1 of n; f0, f1 of n+1
+    const c32 ul0 = a.U[2u * (unsigned)l0];
+    const c32 ux0 = a.U[2u * (unsigned)xl + 1u], ux1 = a.U[2u * (unsigned)xl + 3u];
+    const f16x4 h0 = cp.lo, h1 = cp.hi, r0v = rp.lo, r1v = rp.hi, l0v = lp.lo, l1v = lp.hi;
+    c[0][0] = c32{float(h0.x), float(h0.y)};
+    c[0][1] = c32{float(h0.z), float(h0.w)};
+    c[1][0] = c32{float(h1.x), float(h1.y)};
+    c[1][1] = c32{float(h1.z), float(h1.w)};
+    c32 f00{ua.x, ua.y}, f01{ua.z, ua.w}, f10{ub.x, ub.y}, f11{ub.z, ub.w};
+    c32 b00 = conj(ul0), b10 = conj(f00);  // left t links of n and n+1
+    if (tl) b00 = -b00;
+    if (tr1) f10 = -f10;
+    const c32 b01 = conj(ux0), b11 = conj(ux1);  // left x links
+    const H16Proj P;
+    // site n: right t = n+1, left t = tL; site n+1: right t = tR, left t = n
+    wilson_combine<float, false>(c[0][0], c[0][1], P.sum(h1, true), P.sumi(r0v, false), P.sum(tL, false),
+                                 P.sumi(l0v, true), f00, f01, b00, b01, true, mx_r, ml0, mx_l, a.mm, o[0][0], o[0][1]);
+    wilson_combine<float, false>(c[1][0], c[1][1], P.sum(tR, true), P.sumi(r1v, false), P.sum(h0, false),
+                                 P.sumi(l1v, true), f10, f11, b10, b11, mr1t, mx_r, true, mx_l, a.mm, o[1][0], o[1][1]);
 }
 
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
@@ -555,8 +619,41 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
     cplx<R> pw[J + 1], pv[J + 1];
 #pragma unroll
     for (int i = 0; i <= J; i++) pw[i] = pv[i] = czero<R>();
+    if constexpr (H && SQ) {
+        if (a.pairs) {
+            // two sites along t per thread (the host checked the geometry)
+            const f16x4* vin = hslot(a, J, t.rhs);
+#pragma unroll 1
+            for (int m = t.m_lo + 2 * threadIdx.x; m < t.m_hi; m += 2 * FT) {
+                const BSite b = bsite(a, t, m);
+                const unsigned n = (unsigned)b.n;
+                c32 o[2][2], cc[2][2];
+                block_stencil_pair(a, b, vin, o, cc);
+                const f16x4 w0 = to_h16(o[0][0], o[0][1]), w1 = to_h16(o[1][0], o[1][1]);
+                *reinterpret_cast<f16x8*>(reinterpret_cast<f16x4*>(w) + n) =
+                    f16x8{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+                const c32 s00{float(w0.x), float(w0.y)}, s01{float(w0.z), float(w0.w)};
+                const c32 s10{float(w1.x), float(w1.y)}, s11{float(w1.z), float(w1.w)};
+#pragma unroll
+                for (int i = 0; i <= J; i++) {
+                    c32 v00 = cc[0][0], v01 = cc[0][1], v10 = cc[1][0], v11 = cc[1][1];
+                    if (i < J) {
+                        const f16x8 v = *reinterpret_cast<const f16x8*>(hslot(a, i, t.rhs) + n);
+                        v00 = c32{float(v.s0), float(v.s1)};
+                        v01 = c32{float(v.s2), float(v.s3)};
+                        v10 = c32{float(v.s4), float(v.s5)};
+                        v11 = c32{float(v.s6), float(v.s7)};
+                    }
+                    cmadc(pw[i], s00, v00);
+                    cmadc(pw[i], s01, v01);
+                    cmadc(pw[i], s10, v10);
+                    cmadc(pw[i], s11, v11);
+                }
+            }
+        }
+    }
 #pragma unroll (J >= 4 ? 1 : 2)
-    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi && !(H && SQ && a.pairs); m += FT) {
         const BSite b = bsite(a, t, m);
         const int n = b.n;
         // a0, a1: Vt_J at the site, read once by the stencil
