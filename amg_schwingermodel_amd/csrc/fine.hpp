@@ -743,7 +743,38 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
     for (int i = 0; i <= SAP_MAXM; i++) eta[i] = i <= j ? cvt<R>(L.eta[i]) : czero<R>();
     const R sj = R(L.s[j]);
     double nn = 0;
-    for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
+    if constexpr (H) {
+        if (a.pairs) {
+            // two sites along t per thread, 16-byte loads and stores
+            const float sg = a.sig;
+            for (int m = t.m_lo + 2 * threadIdx.x; m < t.m_hi; m += 2 * FT) {
+                const unsigned n = (unsigned)bsite(a, t, m).n;
+                const f16x8 hw = *reinterpret_cast<const f16x8*>(reinterpret_cast<const f16x4*>(w) + n);
+                c32 u[4] = {sj * c32{float(hw.s0), float(hw.s1)}, sj * c32{float(hw.s2), float(hw.s3)},
+                            sj * c32{float(hw.s4), float(hw.s5)}, sj * c32{float(hw.s6), float(hw.s7)}};
+#pragma unroll
+                for (int i = 0; i <= SAP_MAXM; i++) {
+                    if (i <= j) {
+                        const f16x8 v = *reinterpret_cast<const f16x8*>(hslot(a, i, t.rhs) + n);
+                        const c32 e = cvt<float>(eta[i]);
+                        cmsb(u[0], e, c32{float(v.s0), float(v.s1)});
+                        cmsb(u[1], e, c32{float(v.s2), float(v.s3)});
+                        cmsb(u[2], e, c32{float(v.s4), float(v.s5)});
+                        cmsb(u[3], e, c32{float(v.s6), float(v.s7)});
+                    }
+                }
+                const f16x4 h0 = to_h16(c32{sg * u[0].re, sg * u[0].im}, c32{sg * u[1].re, sg * u[1].im});
+                const f16x4 h1 = to_h16(c32{sg * u[2].re, sg * u[2].im}, c32{sg * u[3].re, sg * u[3].im});
+                *reinterpret_cast<f16x8*>(hslot(a, j + 1, t.rhs) + n) =
+                    f16x8{h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+                nn += double(h0.x) * double(h0.x) + double(h0.y) * double(h0.y) + double(h0.z) * double(h0.z) +
+                      double(h0.w) * double(h0.w);
+                nn += double(h1.x) * double(h1.x) + double(h1.y) * double(h1.y) + double(h1.z) * double(h1.z) +
+                      double(h1.w) * double(h1.w);
+            }
+        }
+    }
+    for (int m = t.m_lo + threadIdx.x; m < t.m_hi && !(H && a.pairs); m += FT) {
         const int n = bsite(a, t, m).n;
         cplx<R> w0, w1;
         if constexpr (H) {
@@ -823,6 +854,36 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT, (H ? 8 : 1))
     cplx<R>* xl = a.xl + off;
     const int mi = L.maxIt;
     // x_B += sum_j eta_j Vt_j in the engine's precision
+    if constexpr (H) {
+        if (a.pairs) {
+            // two sites along t per thread, 16-byte loads and stores
+            using F4 = float __attribute__((ext_vector_type(4)));
+            for (int m = t.m_lo + 2 * threadIdx.x; m < t.m_hi; m += 2 * FT) {
+                const unsigned n = (unsigned)bsite(a, t, m).n;
+                F4* xp = reinterpret_cast<F4*>(xl) + n;  // sites n, n+1: 2 x 16 bytes
+                F4 xa = xp[0], xb = xp[1];
+                c32 x[4] = {c32{xa.x, xa.y}, c32{xa.z, xa.w}, c32{xb.x, xb.y}, c32{xb.z, xb.w}};
+                for (int jj = 0; jj < mi; jj++) {
+                    const f16x8 v = *reinterpret_cast<const f16x8*>(hslot(a, jj, t.rhs) + n);
+                    const c32 c = cvt<float>(L.eta[jj]);
+                    cmad(x[0], c, c32{float(v.s0), float(v.s1)});
+                    cmad(x[1], c, c32{float(v.s2), float(v.s3)});
+                    cmad(x[2], c, c32{float(v.s4), float(v.s5)});
+                    cmad(x[3], c, c32{float(v.s6), float(v.s7)});
+                }
+                if (xacc) {
+                    F4* gp = reinterpret_cast<F4*>(xacc + off) + n;
+                    F4 ga = gp[0], gb = gp[1];
+                    gp[0] = F4{ga.x + x[0].re, ga.y + x[0].im, ga.z + x[1].re, ga.w + x[1].im};
+                    gp[1] = F4{gb.x + x[2].re, gb.y + x[2].im, gb.z + x[3].re, gb.w + x[3].im};
+                } else {
+                    xp[0] = F4{x[0].re, x[0].im, x[1].re, x[1].im};
+                    xp[1] = F4{x[2].re, x[2].im, x[3].re, x[3].im};
+                }
+            }
+            return;
+        }
+    }
     for (int m = t.m_lo + threadIdx.x; m < t.m_hi; m += FT) {
         const int n = bsite(a, t, m).n;
         const unsigned u = 2u * (unsigned)n;
