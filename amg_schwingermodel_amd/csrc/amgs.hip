@@ -10,6 +10,7 @@
 //   - every right-hand side of a batch follows its own control flow exactly as
 //     a single solve would; finished right-hand sides are skipped.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -248,10 +249,16 @@ template <typename R> int Engine<R>::sap_wgs(int nrhs) const {
 template <typename R> constexpr bool kH16 = std::is_same<R, float>::value;
 
 template <typename R, bool H, bool SQ>
-static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act) {
+static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act, KTimer& kt) {
     switch (j) {
-#define AMGS_SAP_A(J) \
-    case J: hipLaunchKernelGGL((k_sap_arnoldi_a<R, J, H, SQ>), grid, dim3(FT), 0, s, a, k, act); break;
+#define AMGS_SAP_A(J)                                                                                          \
+    case J:                                                                                                    \
+        if (act)                                                                                               \
+            hipExtLaunchKernelGGL((k_sap_arnoldi_a<R, J, H, SQ>), grid, dim3(FT), 0, s, kt.start(), kt.stop(), 0, \
+                                  a, k, act);                                                                  \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_sap_arnoldi_a<R, J, H, SQ>), grid, dim3(FT), 0, s, a, k, act);               \
+        break;
         AMGS_SAP_A(0) AMGS_SAP_A(1) AMGS_SAP_A(2) AMGS_SAP_A(3)
         AMGS_SAP_A(4) AMGS_SAP_A(5) AMGS_SAP_A(6) AMGS_SAP_A(7)
 #undef AMGS_SAP_A
@@ -321,21 +328,31 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     // With FP16 slots: w and every slot FP16.
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
                     const double pa = h16 ? 16.0 + 8.0 * j : 2 * S * (2 + j);
-                    int* act = c->kt[KT_SAP_A].begin(s, sites * 2 * S, g.sspb * pa);
+                    KTimer& ka = c->kt[KT_SAP_A];
+                    int* act = ka.begin(s, sites * 2 * S, g.sspb * pa);
                     if (h16) {
-                        if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act);
-                        else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act);
+                        if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act, ka);
+                        else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act, ka);
                     } else {
-                        if (sq) sap_arnoldi_a<R, false, true>(grid, s, a, j, k, act);
-                        else sap_arnoldi_a<R, false, false>(grid, s, a, j, k, act);
+                        if (sq) sap_arnoldi_a<R, false, true>(grid, s, a, j, k, act, ka);
+                        else sap_arnoldi_a<R, false, false>(grid, s, a, j, k, act, ka);
                     }
-                    if (act) c->kt[KT_SAP_A].end(s);
                     // w and Vt_i (i <= j) read, w written
                     const double pb = h16 ? 24.0 + 8.0 * j : 2 * S * (3 + j);
-                    act = c->kt[KT_SAP_B].begin(s, 0.0, g.sspb * pb);
-                    if (h16) hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
-                    else hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
-                    if (act) c->kt[KT_SAP_B].end(s);
+                    KTimer& kb = c->kt[KT_SAP_B];
+                    act = kb.begin(s, 0.0, g.sspb * pb);
+                    if (act) {
+                        if (h16)
+                            hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, kb.start(),
+                                                  kb.stop(), 0, a, j, act);
+                        else
+                            hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, kb.start(),
+                                                  kb.stop(), 0, a, j, act);
+                    } else if (h16) {
+                        hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
+                    } else {
+                        hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
+                    }
                 }
                 const bool last = k + 1 == c->P.sap_restarts;
                 cplx<R>* xacc = last ? x : (cplx<R>*)nullptr;

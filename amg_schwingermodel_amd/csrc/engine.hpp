@@ -57,8 +57,9 @@ struct Ctx;
 
 // ---------------------------------------------------------------------------
 // Per-launch timing of one kernel class inside solves: a HIP event pair on the
-// library stream around every launch, and a device counter of the items that
-// did work in that launch (for the algorithmic byte count).
+// library stream recorded by every launch's own dispatch (hipExtLaunchKernel:
+// the pair brackets the kernel, not the gaps around it), and a device counter
+// of the items that did work in that launch (for the algorithmic byte count).
 // ---------------------------------------------------------------------------
 struct KTimer {
     static constexpr int CAP = 8192;  // launches recorded between two resolves
@@ -91,12 +92,13 @@ struct KTimer {
             HIP_CHECK(hipEventCreate(&e));
             ev.push_back(e);
         }
-        HIP_CHECK(hipEventRecord(ev[2 * i], s));
         fixed.push_back(fx);
         per_item.push_back(pi);
         return actlog.as<int>() + i;
     }
-    void end(hipStream_t s) { HIP_CHECK(hipEventRecord(ev[2 * (fixed.size() - 1) + 1], s)); }
+    // the start and stop events of the launch that the last begin() opened
+    hipEvent_t start() const { return ev[2 * (fixed.size() - 1)]; }
+    hipEvent_t stop() const { return ev[2 * (fixed.size() - 1) + 1]; }
     // fold the pending records into the totals (synchronises the stream)
     void resolve(hipStream_t s) {
         if (fixed.empty()) return;
