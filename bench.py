@@ -69,8 +69,9 @@ def parse():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: replay the level-0 preconditioner from a captured HIP graph in the timed solves")
     ap.add_argument("--split", type=int, default=2,
-                    help="also time the batch as this many concurrent equal sub-batches (amgs_solve_multi, one "
-                         "context and stream each); value is the faster of the two forms (1: skip)")
+                    help="also time the batch as this many concurrent sub-batches of (nearly) equal size "
+                         "(amgs_solve_multi, one context and stream each); value is the faster of the two forms "
+                         "(1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -180,31 +181,34 @@ def main():
 
     # ---- the same rhs as concurrent sub-batches, one context and stream each:
     # one sub-batch's VALU-bound SAP step A overlaps another's HBM-bound step B
-    if a.split > 1 and a.rhs % a.split == 0:
-        per = a.rhs // a.split
+    if 1 < a.split <= a.rhs:
+        # sub-batch sizes as equal as possible
+        sizes = [a.rhs // a.split + (1 if i < a.rhs % a.split else 0) for i in range(a.split)]
+        first = [sum(sizes[:i]) for i in range(a.split)]
         ss = []
         for i in range(a.split):
-            s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=per, precision=precs[a.precision]), U)
+            s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=sizes[i],
+                                                precision=precs[a.precision]), U)
             s.setup()
-            wgs_i, ms_i = s.autotune(per)
+            wgs_i, ms_i = s.autotune(sizes[i])
             log(f"sub-batch {i}: autotuner: SAP launches of {wgs_i} workgroups ({ms_i:.2f} ms per sweep)")
             s.set_graph(bool(a.graph))
             ss.append(s)
-        bp = [Bd[i * per].data_ptr() for i in range(a.split)]
-        xp = [Xd[i * per].data_ptr() for i in range(a.split)]
+        bp = [Bd[first[i]].data_ptr() for i in range(a.split)]
+        xp = [Xd[first[i]].data_ptr() for i in range(a.split)]
         for w in range(a.warmup):
-            amgs.solve_multi(ss, bp, xp, [per] * a.split)
+            amgs.solve_multi(ss, bp, xp, sizes)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for k in range(a.steps):
-            st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, [per] * a.split)
-            log(f"{a.split} x {per} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
+            st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, sizes)
+            log(f"{sizes} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
         torch.cuda.synchronize()
         ms_split = (time.perf_counter() - t) * 1e3 / a.steps
         its_s, rr_s = np.concatenate(its_s), np.concatenate(rr_s)
-        log(f"{a.split} x {per} rhs concurrent: {ms_split:.1f} ms -> {ms_split / a.rhs:.2f} ms/rhs, iterations "
+        log(f"{sizes} rhs concurrent: {ms_split:.1f} ms -> {ms_split / a.rhs:.2f} ms/rhs, iterations "
             f"{its_s.min()}..{its_s.max()}, max relres {rr_s.max():.2e}, status {st.tolist()}")
-        out["mixed_batched_concurrent"] = {"sub_batches": a.split, "ms_per_rhs": round(ms_split / a.rhs, 3),
+        out["mixed_batched_concurrent"] = {"sub_batches": sizes, "ms_per_rhs": round(ms_split / a.rhs, 3),
                                            "iterations_min": int(its_s.min()), "iterations_max": int(its_s.max()),
                                            "relres_max": float(rr_s.max()), "status": st.tolist()}
         if ms_split < ms_best and rr_s.max() < 1e-10 and all(v == 0 for v in st.tolist()):
