@@ -214,6 +214,17 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
     const Geo& g1 = c->geo[1];
     const dim3 grid(g1.ns, nrhs);
     const int nv = g.ntest;
+    if constexpr (std::is_same<R, float>::value) {
+        if (nv <= 12) {
+            const dim3 g2(g1.ns, cdiv_u(nrhs, 2));
+            if (nv <= 8)
+                hipLaunchKernelGGL(k_restrict0_g<8>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs);
+            else
+                hipLaunchKernelGGL(k_restrict0_g<12>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs);
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
     if (nv <= 8)
         hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
     else if (nv <= 16)
@@ -395,8 +406,19 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
     restrict0(nrhs, tv[0], rr, eta1.as<cplx<R>>());
     // K-cycle: FGMRES_k_cycle[1] (src/amg.cpp:161); V-cycle: v_cycle(1) from zero (src/amg.cpp:110)
     coarse(c->P.cycle == 0 ? CM_CYCLE : CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
-    hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                       (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, 1);
+    if constexpr (std::is_same<R, float>::value) {
+        const dim3 gp(ntile, cdiv_u(nrhs, 8));
+        const int nv = g.ntest, dofc = c->geo[1].dof, nsc = c->geo[1].ns;
+        const c32* pc = (const c32*)psi1.as<c32>();
+        if (nv <= 8) hipLaunchKernelGGL(k_prolong0_g<8>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
+        else if (nv <= 12) hipLaunchKernelGGL(k_prolong0_g<12>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
+        else if (nv <= 16) hipLaunchKernelGGL(k_prolong0_g<16>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
+        else hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
+                                (const cplx<R>*)psi1.as<cplx<R>>(), psi, dofc, nsc, 1);
+    } else {
+        hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
+                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, 1);
+    }
     if (c->P.nu2 > 0) sap0(nrhs, eta, psi, c->P.nu2);
     HIP_CHECK(hipGetLastError());
 }

@@ -1045,6 +1045,93 @@ __global__ void __launch_bounds__(FT) k_prolong0(Geo g, int nv, const cplx<R>* _
     }
 }
 
+// The FP32 engine's interpolation with the test vectors read once per group
+// of right-hand sides (the kernels above read them once per rhs: 32 passes
+// over them per launch at 32 rhs).  Same sums in the same order, with fused
+// complex multiply-adds.
+// Restriction, two rhs per workgroup (grid: aggregates x rhs pairs).
+template <int NV>
+__global__ void __launch_bounds__(FT) k_restrict0_g(Geo g, int nv, const c32* __restrict__ w,
+                                                    const c32* __restrict__ v, c32* __restrict__ out, int dofc,
+                                                    int nsc, int nrhs) {
+    __shared__ c32 red[4 * 2 * 2 * NV];
+    const int X = blockIdx.x, rA = 2 * blockIdx.y, rB = rA + 1 < nrhs ? rA + 1 : rA;
+    const size_t nvec = (size_t)g.ns * 2;
+    c32 acc[2][2 * NV];
+#pragma unroll
+    for (int q = 0; q < 2 * NV; q++) acc[0][q] = acc[1][q] = czero<float>();
+    for (int m = threadIdx.x; m < g.spb; m += FT) {
+        const unsigned n = 2u * (unsigned)agg_site(g, X, m);
+        const c32 a0 = v[rA * nvec + n], a1 = v[rA * nvec + n + 1u];
+        const c32 b0 = v[rB * nvec + n], b1 = v[rB * nvec + n + 1u];
+#pragma unroll
+        for (int cc = 0; cc < NV; cc++) {
+            if (cc < nv) {
+                const c32 w0 = w[cc * nvec + n], w1 = w[cc * nvec + n + 1u];
+                cmadc(acc[0][2 * cc], a0, w0);
+                cmadc(acc[0][2 * cc + 1], a1, w1);
+                cmadc(acc[1][2 * cc], b0, w0);
+                cmadc(acc[1][2 * cc + 1], b1, w1);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int q = 0; q < 2 * NV; q++) {
+            if (q < 2 * nv) {
+                const c32 x = wave_sum(acc[r][q]);
+                if ((threadIdx.x & 63) == 0) red[((threadIdx.x >> 6) * 2 + r) * 2 * NV + q] = x;
+            }
+        }
+    __syncthreads();
+    const int r = threadIdx.x / (2 * NV), q = threadIdx.x - r * 2 * NV;
+    if (r < 2 && q < 2 * nv && (r == 0 || rB != rA)) {
+        c32 x = red[r * 2 * NV + q];
+        for (int wv = 1; wv < FT / 64; wv++) x += red[(wv * 2 + r) * 2 * NV + q];
+        out[(size_t)(r ? rB : rA) * nsc * dofc + (size_t)X * dofc + q] = x;
+    }
+}
+
+// Prolongation: a site's test-vector entries held in registers across up to
+// 8 rhs (grid: site tiles x rhs groups of 8).
+template <int NV>
+__global__ void __launch_bounds__(FT) k_prolong0_g(Geo g, int nv, const c32* __restrict__ w,
+                                                   const c32* __restrict__ vc, c32* __restrict__ out, int dofc,
+                                                   int nsc, int nrhs, int accumulate) {
+    const int n = blockIdx.x * FT + threadIdx.x;
+    if (n >= g.ns) return;
+    const size_t nvec = (size_t)g.ns * 2;
+    const unsigned u = 2u * (unsigned)n;
+    c32 w0[NV], w1[NV];
+#pragma unroll
+    for (int cc = 0; cc < NV; cc++) {
+        w0[cc] = cc < nv ? w[cc * nvec + u] : czero<float>();
+        w1[cc] = cc < nv ? w[cc * nvec + u + 1u] : czero<float>();
+    }
+    const size_t X = (size_t)agg_block_of(g, n);
+    const int r1 = min(nrhs, 8 * (int)blockIdx.y + 8);
+    for (int rhs = 8 * blockIdx.y; rhs < r1; rhs++) {
+        const c32* c = vc + (size_t)rhs * nsc * dofc + X * dofc;
+        c32 o0 = czero<float>(), o1 = czero<float>();
+#pragma unroll
+        for (int cc = 0; cc < NV; cc++) {
+            if (cc < nv) {
+                cmad(o0, w0[cc], c[2 * cc]);
+                cmad(o1, w1[cc], c[2 * cc + 1]);
+            }
+        }
+        c32* op = out + (size_t)rhs * nvec + u;
+        if (accumulate) {
+            op[0] += o0;
+            op[1] += o1;
+        } else {
+            op[0] = o0;
+            op[1] = o1;
+        }
+    }
+}
+
 // precision conversion / scaling of whole vectors
 template <typename To, typename From>
 __global__ void k_convert(size_t n, const cplx<From>* __restrict__ in, cplx<To>* __restrict__ out) {
