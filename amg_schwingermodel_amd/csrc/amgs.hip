@@ -623,6 +623,7 @@ static void outer_alloc(Ctx& c, int cap) {
     s.qr = (double*)(b + o_qr);
     s.M = M;
     s.ntile = ntile;
+    s.ntile_a = (int)cdiv_u(ntile, OUT_SPT);
     s.hcap = hcap;
 }
 
@@ -729,7 +730,7 @@ static void outer_solve(Ctx& c, int nrhs) {
         hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0);
         for (int j = 0; j < M; j++) {
             outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride);
-            hipLaunchKernelGGL(k_outer_a, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(),
+            hipLaunchKernelGGL(k_outer_a, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(),
                                (const c64*)(Z + (size_t)j * vstride), (const c64*)V, w, st, j, vstride);
             hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
             hipLaunchKernelGGL(k_outer_b, g2, dim3(FT), 0, s, g.ns, (const c64*)V, w, st, j, vstride);

@@ -9,6 +9,7 @@ namespace amgs {
 constexpr int FT = 256;       // threads per workgroup for fine-lattice kernels
 constexpr int SAP_MAXM = 8;   // max restart length of the SAP block GMRES
 constexpr int OUT_MAXM = 32;  // max restart length of the outer FGMRES
+constexpr int OUT_SPT = 4;    // tiles of FT sites per workgroup in the outer Arnoldi step A
 
 // Scalar state of every SAP item (one Schwarz block x one right-hand side),
 // in device memory between the launches of one SAP sweep.
@@ -29,10 +30,11 @@ struct OuterState {
     int *maxIt, *jbreak, *done, *active, *iters, *nhist;
     double *normb, *s, *hist;  // s [rhs][M+1]; hist [rhs][hcap]
     c64 *R, *G, *g, *cn, *sn, *eta;  // R [rhs][M][M], G [rhs][M+1][M+1]
-    c64* qa;     // [rhs][ntile][2M+1]
+    c64* qa;     // [rhs][ntile_a][2M+1]
     double* qb;  // [rhs][ntile]
     double* qr;  // [rhs][ntile][2]
     int M, ntile, hcap;
+    int ntile_a;  // workgroups per rhs of the Arnoldi step A (OUT_SPT tiles of FT sites each)
 };
 
 }  // namespace amgs

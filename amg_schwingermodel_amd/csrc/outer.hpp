@@ -94,27 +94,38 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
     __shared__ c64 red[(FT / 64) * NQ];
     const int rhs = blockIdx.y;
     if (!st.active[rhs]) return;
+    // OUT_SPT consecutive tiles of FT sites per workgroup: the 2j+1 block
+    // reductions are shared by OUT_SPT sites per thread
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int n = tile * FT + threadIdx.x;
     const size_t off = (size_t)rhs * g.ns * 2;
     const int nq = 2 * j + 1;
-    const bool live = n < g.ns;
-    c64 o0 = {0, 0}, o1 = {0, 0}, a0 = {0, 0}, a1 = {0, 0};
-    if (live) {
-        wilson_site<double, false>(U, Zj + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
-        w[off + 2 * n] = o0;
-        w[off + 2 * n + 1] = o1;
-        const c64* vj = V + (size_t)j * vstride + off;
-        a0 = vj[2 * n];
-        a1 = vj[2 * n + 1];
+    c64 o0[OUT_SPT], o1[OUT_SPT], a0[OUT_SPT], a1[OUT_SPT];
+    bool live[OUT_SPT];
+    const c64* vj = V + (size_t)j * vstride + off;
+#pragma unroll
+    for (int s = 0; s < OUT_SPT; s++) {
+        const int n = (tile * OUT_SPT + s) * FT + threadIdx.x;
+        live[s] = n < g.ns;
+        o0[s] = o1[s] = a0[s] = a1[s] = c64{0, 0};
+        if (live[s]) {
+            wilson_site<double, false>(U, Zj + off, n, g.nx, g.nt, mm, AllSites{}, o0[s], o1[s]);
+            w[off + 2 * n] = o0[s];
+            w[off + 2 * n + 1] = o1[s];
+            a0[s] = vj[2 * n];
+            a1[s] = vj[2 * n + 1];
+        }
     }
     for (int i = 0; i <= j; i++) {
         c64 pw = {0, 0}, pv = {0, 0};
-        if (live) {
-            const c64* vi = V + (size_t)i * vstride + off;
-            const c64 b0 = vi[2 * n], b1 = vi[2 * n + 1];
-            pw = mulc(o0, b0) + mulc(o1, b1);
-            pv = mulc(a0, b0) + mulc(a1, b1);
+        const c64* vi = V + (size_t)i * vstride + off;
+#pragma unroll
+        for (int s = 0; s < OUT_SPT; s++) {
+            if (live[s]) {
+                const int n = (tile * OUT_SPT + s) * FT + threadIdx.x;
+                const c64 b0 = vi[2 * n], b1 = vi[2 * n + 1];
+                pw += mulc(o0[s], b0) + mulc(o1[s], b1);
+                pv += mulc(a0[s], b0) + mulc(a1[s], b1);
+            }
         }
         park(pw, i, red, NQ);
         if (i < j) park(pv, j + 1 + i, red, NQ);
@@ -123,7 +134,7 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
     if (threadIdx.x < nq) {
         c64 r = red[threadIdx.x];
         for (int wv = 1; wv < FT / 64; wv++) r += red[wv * NQ + threadIdx.x];
-        st.qa[((size_t)rhs * st.ntile + tile) * (2 * st.M + 1) + threadIdx.x] = r;
+        st.qa[((size_t)rhs * st.ntile_a + tile) * (2 * st.M + 1) + threadIdx.x] = r;
     }
 }
 
@@ -153,7 +164,7 @@ __global__ void __launch_bounds__(FT) k_outer_hcol(OuterState st, int j) {
     if (!st.active[rhs]) return;
     const int nq = 2 * j + 1, M = st.M;
     for (int q = 0; q < nq; q++)
-        tile_sum(st.qa + (size_t)rhs * st.ntile * (2 * M + 1), 2 * M + 1, st.ntile, q, red, &d[q]);
+        tile_sum(st.qa + (size_t)rhs * st.ntile_a * (2 * M + 1), 2 * M + 1, st.ntile_a, q, red, &d[q]);
     if (threadIdx.x != 0) return;
     c64* G = st.G + (size_t)rhs * (M + 1) * (M + 1);
     c64* Rm = st.R + (size_t)rhs * M * M;
