@@ -695,6 +695,27 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
     }
 }
 
+// The outer solve's per-rhs active flags on the host.  The copy goes to
+// pinned memory and the host spins on an event behind it: this wait happens
+// once per outer Arnoldi step, and a blocking wait (or a copy to pageable
+// memory) adds the host's wake-up latency to every step.
+static void read_flags(Ctx& c, const int* dflags, int nrhs, std::vector<int>& out) {
+    if (!c.evf) HIP_CHECK(hipEventCreateWithFlags(&c.evf, hipEventDisableTiming));
+    if (nrhs > c.hflags_cap) {
+        if (c.hflags) HIP_CHECK(hipHostFree(c.hflags));
+        c.hflags = nullptr;
+        HIP_CHECK(hipHostMalloc((void**)&c.hflags, sizeof(int) * nrhs));
+        c.hflags_cap = nrhs;
+    }
+    HIP_CHECK(hipMemcpyAsync(c.hflags, dflags, sizeof(int) * nrhs, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipEventRecord(c.evf, c.stream));
+    hipError_t e;
+    while ((e = hipEventQuery(c.evf)) == hipErrorNotReady) {
+    }
+    HIP_CHECK(e);
+    std::copy(c.hflags, c.hflags + nrhs, out.begin());
+}
+
 // FGMRES_AMG::fgmres(b, 0, x) on device buffers ob -> ox (src/fgmres.cpp:5-88)
 static void outer_solve(Ctx& c, int nrhs) {
     const Geo& g = c.geo[0];
@@ -724,8 +745,7 @@ static void outer_solve(Ctx& c, int nrhs) {
                        (const c64*)x, r, st);
     for (int k = 0; k < c.P.out_restarts; k++) {
         hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k, tol, M);
-        HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
-        c.sync();
+        read_flags(c, st.active, nrhs, flags);
         if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
         hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0);
         for (int j = 0; j < M; j++) {
@@ -738,8 +758,7 @@ static void outer_solve(Ctx& c, int nrhs) {
             if (j + 1 < M)
                 hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)w, V + (size_t)(j + 1) * vstride,
                                    st, j + 1);
-            HIP_CHECK(hipMemcpyAsync(flags.data(), st.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, s));
-            c.sync();
+            read_flags(c, st.active, nrhs, flags);
             for (KTimer& kt : c.kt) kt.resolve(s);
             step++;
             if (verbose) {
@@ -896,11 +915,14 @@ extern "C" void amgs_destroy(amgs_ctx* ctx) {
     if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
     Ctx& c = ctx->c;
     hipStream_t s = c.stream;
-    hipEvent_t a = c.ev0, b = c.ev1;
+    hipEvent_t a = c.ev0, b = c.ev1, f = c.evf;
+    int* hf = c.hflags;
     c.clear_graphs();
     delete ctx;  // frees device buffers
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
+    if (f) (void)hipEventDestroy(f);
+    if (hf) (void)hipHostFree(hf);
     if (s) (void)hipStreamDestroy(s);
 }
 

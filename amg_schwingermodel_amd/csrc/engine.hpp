@@ -175,6 +175,11 @@ struct Ctx {
     int outer_prec = 0;  // outer FGMRES preconditioner: 0 multigrid cycle, 1 one SAP sweep, 2 none
     double last_ms[3] = {0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pinned copy of the outer solve's per-rhs active flags and the event the
+    // host spins on for it (read once per outer Arnoldi step)
+    int* hflags = nullptr;
+    int hflags_cap = 0;
+    hipEvent_t evf = nullptr;
     KTimer kt[KT_CLASSES];
     // HIP-graph replay of the level-0 preconditioner, one executable graph per
     // batch size; FP64 inputs/outputs are staged through fixed buffers
