@@ -371,6 +371,32 @@ def test_graph_replay_matches_eager(prec):
     assert rre1 < 1e-10
 
 
+def test_kernel_timing_records_sap_steps():
+    """Per-launch timing of SAP steps A and B (amgs_kernel_timing, the event
+    pair recorded by each launch's own dispatch): launches, device time and
+    algorithmic bytes are recorded, the bytes per launch of step A lie within
+    what its sites can move (FP16 basis: at most (16 + 8 J) bytes per block
+    site and rhs for J <= m - 1, plus the colour's links), and the timed solve
+    gives the untimed solve's result bit for bit."""
+    s, o, _ = make("32_3lvl", precision=amgs.PREC_MIXED16, max_rhs=2)
+    B = np.stack([cvec(s.size(0), 500 + i) for i in range(2)])
+    x0, it0, _, _ = s.solve(B)
+    s.kernel_timing(0, True)
+    s.kernel_timing(1, True)
+    x1, it1, rr1, _ = s.solve(B)
+    na, msa, bya = s.kernel_stats(0)
+    nb, msb, byb = s.kernel_stats(1)
+    s.kernel_timing(0, False)
+    s.kernel_timing(1, False)
+    assert np.array_equal(it1, it0) and np.array_equal(x1, x0)
+    assert na > 0 and nb > 0 and msa > 0 and msb > 0 and bya > 0 and byb > 0
+    half = s.size(0) // 2 // 2          # sites of one colour (size(0) counts 2 spins per site)
+    m = 5                               # SAP restart length of make()'s parameters
+    bound = (16 + 8 * (m - 1)) * half * 2 + 16 * half
+    assert bya / na <= bound, (bya / na, bound)
+    assert rr1.max() < 1e-10
+
+
 def test_full_size_256_three_levels():
     """The 256x256, 3-level workload (aggregation 8x8 then 4x4 blocks, 10 test
     vectors, 4x4 SAP blocks; beta = 2 Metropolis links from 500 sweeps,
