@@ -657,15 +657,17 @@ static hipGraphExec_t precond_graph(Ctx& c, int nrhs) {
     return exec;
 }
 
-static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
+// eta32_ready: the FP32 staging input already holds vj (written by the
+// kernel that produced vj)
+static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_ready) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
     // graph replay (kernel timers record eager launches only)
     if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on) {
-        if (c.mixed())
+        if (!c.mixed())
+            HIP_CHECK(hipMemcpyAsync(c.oeta64.p, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        else if (!eta32_ready)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                                c.oeta32.as<c32>());
-        else
-            HIP_CHECK(hipMemcpyAsync(c.oeta64.p, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
         HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs), c.stream));
         if (c.mixed())
             hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
@@ -685,8 +687,9 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj) {
         return;
     }
     if (c.mixed()) {
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
-                           c.oeta32.as<c32>());
+        if (!eta32_ready)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
+                               c.oeta32.as<c32>());
         c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
         hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
                            (const c32*)c.opsi32.as<c32>(), zj);
@@ -731,6 +734,10 @@ static void outer_solve(Ctx& c, int nrhs) {
     c64* b = c.ob.as<c64>();
     c64* r = c.or_.as<c64>();
     c64* w = c.ow.as<c64>();
+    // the mixed multigrid preconditioner reads an FP32 copy of V_j: the
+    // scaling kernel that writes V_j writes it too
+    const bool eta32 = c.mixed() && c.outer_prec == 0;
+    c32* oeta32 = eta32 ? c.oeta32.as<c32>() : nullptr;
     HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nrhs * n0, s));
     HIP_CHECK(hipMemsetAsync(st.done, 0, sizeof(int) * nrhs, s));
     HIP_CHECK(hipMemsetAsync(st.nhist, 0, sizeof(int) * nrhs, s));
@@ -747,9 +754,9 @@ static void outer_solve(Ctx& c, int nrhs) {
         hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k, tol, M);
         read_flags(c, st.active, nrhs, flags);
         if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
-        hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0);
+        hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0, eta32 ? oeta32 : nullptr);
         for (int j = 0; j < M; j++) {
-            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride);
+            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride, eta32);
             hipLaunchKernelGGL(k_outer_a, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(),
                                (const c64*)(Z + (size_t)j * vstride), (const c64*)V, w, st, j, vstride);
             hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
@@ -757,7 +764,7 @@ static void outer_solve(Ctx& c, int nrhs) {
             hipLaunchKernelGGL(k_outer_fin, dim3(nrhs), dim3(FT), 0, s, st, j, tol);
             if (j + 1 < M)
                 hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)w, V + (size_t)(j + 1) * vstride,
-                                   st, j + 1);
+                                   st, j + 1, eta32 ? oeta32 : nullptr);
             read_flags(c, st.active, nrhs, flags);
             for (KTimer& kt : c.kt) kt.resolve(s);
             step++;

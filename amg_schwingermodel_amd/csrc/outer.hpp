@@ -74,16 +74,23 @@ __global__ void __launch_bounds__(FT) k_outer_start(OuterState st, int k, double
 }
 
 // V_j = s_j * src (V_0 from r, V_{j+1} from w)
+// dst = s_j src; with dst32 also its FP32 copy (the mixed-precision
+// preconditioner's input, so that no separate conversion pass reads dst)
 __global__ void __launch_bounds__(FT) k_outer_scale(int ns, const c64* __restrict__ src, c64* __restrict__ dst,
-                                                    OuterState st, int j) {
+                                                    OuterState st, int j, c32* __restrict__ dst32) {
     const int rhs = blockIdx.y;
     if (!st.active[rhs]) return;
     const int n = blockIdx.x * FT + threadIdx.x;
     if (n >= ns) return;
     const double s = st.s[(size_t)rhs * (st.M + 1) + j];
     const size_t off = (size_t)rhs * ns * 2;
-    dst[off + 2 * n] = s * src[off + 2 * n];
-    dst[off + 2 * n + 1] = s * src[off + 2 * n + 1];
+    const c64 a = s * src[off + 2 * n], b = s * src[off + 2 * n + 1];
+    dst[off + 2 * n] = a;
+    dst[off + 2 * n + 1] = b;
+    if (dst32) {
+        dst32[off + 2 * n] = cvt<float>(a);
+        dst32[off + 2 * n + 1] = cvt<float>(b);
+    }
 }
 
 // w = D Z_j; partials <w, V_i> (i <= j), <V_j, V_k> (k < j)
