@@ -287,8 +287,12 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     hipLaunchKernelGGL(k_zero_int, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, nrhs, skip.as<int>());
     if (nu <= 0) return;
     hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
+    // r is read only on the blocks of the colour about to be smoothed: the
+    // residual before the red blocks is formed on the red blocks only
+    const unsigned char* mred = c->colmask.as<unsigned char>();
+    const unsigned char* mblack = mred + g.nsb;
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
-                       (double*)nullptr, skip.as<int>());
+                       (double*)nullptr, skip.as<int>(), mred);
     const int nchunk = sap_chunks(g, nrhs, sap_wgs(nrhs)), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
@@ -379,11 +383,19 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     }
                 }
             }
-            hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
-                               colour == 1 ? part_r.as<double>() : (double*)nullptr, skip.as<int>());
+            // after red: r on the black blocks; after black: all of r and its
+            // norm for the tolerance check, which decides nothing after the
+            // last sweep and is left out there
+            if (colour == 0)
+                hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
+                                   (double*)nullptr, skip.as<int>(), mblack);
+            else if (it + 1 < nu)
+                hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
+                                   part_r.as<double>(), skip.as<int>(), (const unsigned char*)nullptr);
         }
-        hipLaunchKernelGGL(k_sap_check, dim3(nrhs), dim3(FT), 0, s, (const double*)part_r.as<double>(),
-                           (const double*)part_v.as<double>(), (int)ntile, c->P.sap_tol, skip.as<int>());
+        if (it + 1 < nu)
+            hipLaunchKernelGGL(k_sap_check, dim3(nrhs), dim3(FT), 0, s, (const double*)part_r.as<double>(),
+                               (const double*)part_v.as<double>(), (int)ntile, c->P.sap_tol, skip.as<int>());
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -400,7 +412,7 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
     if (c->P.nu1 > 0 || !from_zero) {
         if (c->P.nu1 > 0) sap0(nrhs, eta, psi, c->P.nu1);
         hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, R(c->P.m0 + 2.0), U, eta, psi,
-                           res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr);
+                           res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr, nullptr);
         rr = res0.as<cplx<R>>();
     }
     restrict0(nrhs, tv[0], rr, eta1.as<cplx<R>>());
@@ -494,7 +506,7 @@ static void bootstrap(Ctx& c) {
             if (l == 0) {
                 hipLaunchKernelGGL(k_resid<double>, dim3(cdiv_u(g.ns, FT), nv), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
                                    (const c64*)c.U64.as<c64>(), (const c64*)w, (const c64*)w, rhs.as<c64>(),
-                                   (double*)nullptr, (const int*)nullptr);
+                                   (double*)nullptr, (const int*)nullptr, nullptr);
                 e.kcycle0(nv, rhs.as<c64>(), tvn.as<c64>());
             } else {
                 e.coarse(CM_DOP, l, nv, w, n, rhs.as<c64>(), n);
@@ -816,7 +828,7 @@ static void true_relres(Ctx& c, int nrhs, double* out) {
     pb.alloc(sizeof(double) * nrhs * ntile);
     hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
                        (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), (const c64*)c.ox.as<c64>(),
-                       c.ow.as<c64>(), pr.as<double>(), (const int*)nullptr);
+                       c.ow.as<c64>(), pr.as<double>(), (const int*)nullptr, nullptr);
     hipLaunchKernelGGL(k_norm_part<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g.ns,
                        (const c64*)c.ob.as<c64>(), pb.as<double>());
     std::vector<double> hr((size_t)nrhs * ntile), hb((size_t)nrhs * ntile);
@@ -887,6 +899,15 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
         HIP_CHECK(hipEventCreate(&c.ev1));
         const size_t n0 = c.nvec(0);
         c.U64.alloc(sizeof(c64) * n0);
+        {
+            // level-0 Schwarz block flags: [0][blk] red, [1][blk] black
+            const Geo& g0 = c.geo[0];
+            std::vector<unsigned char> m(2 * (size_t)g0.nsb, 0);
+            for (int b : c.red[0]) m[b] = 1;
+            for (int b : c.black[0]) m[g0.nsb + b] = 1;
+            c.colmask.alloc(m.size());
+            HIP_CHECK(hipMemcpy(c.colmask.p, m.data(), m.size(), hipMemcpyHostToDevice));
+        }
         for (int l = 0; l < c.L; l++) {
             c.redl[l].alloc(sizeof(int) * c.red[l].size());
             c.blackl[l].alloc(sizeof(int) * c.black[l].size());
@@ -1619,7 +1640,7 @@ extern "C" int amgs_multigrid(amgs_ctx* ctx, int nrhs, const double* b, double* 
     for (int it = 0; it < max_cycles && left > 0; it++) {
         c.e64.kcycle0(nrhs, db, dx, false);
         hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
-                           (const c64*)c.U64.as<c64>(), (const c64*)db, (const c64*)dx, dr, dpr, (const int*)nullptr);
+                           (const c64*)c.U64.as<c64>(), (const c64*)db, (const c64*)dx, dr, dpr, (const int*)nullptr, nullptr);
         HIP_CHECK(hipMemcpyAsync(pr.data(), dpr, sizeof(double) * nrhs * ntile, hipMemcpyDeviceToHost, c.stream));
         c.sync();
         for (int r = 0; r < nrhs; r++) {

@@ -162,6 +162,7 @@ struct Ctx {
     bool gauge_set = false, setup_done = false;
     // FP64 hierarchy (always), FP32 copy in mixed mode
     DevBuf U64, U32, tv64[MAXL], tv32[MAXL], links64[MAXL], links32[MAXL], links2[MAXL], redl[MAXL], blackl[MAXL];
+    DevBuf colmask;  // level-0 Schwarz block flags [2][nsb]: red, black
     Engine<double> e64;
     Engine<float> e32;
     // outer FGMRES

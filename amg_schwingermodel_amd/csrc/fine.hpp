@@ -194,12 +194,15 @@ __global__ void __launch_bounds__(FT) k_wilson(Geo g, R mm, const cplx<R>* __res
 }
 
 // r = v - D x; optional per-workgroup partial sums of |r|^2 (for SAP's
-// convergence check) into part[rhs][tile].  skip[rhs] != 0 leaves r unchanged.
+// convergence check) into part[rhs][tile].  skip[rhs] != 0 leaves r unchanged;
+// with want (one flag per Schwarz block) only the flagged blocks' sites are
+// written, the others keep their values.
 template <typename R>
 __global__ void __launch_bounds__(FT) k_resid(Geo g, R mm, const cplx<R>* __restrict__ U,
                                               const cplx<R>* __restrict__ v, const cplx<R>* __restrict__ x,
                                               cplx<R>* __restrict__ r, double* __restrict__ part,
-                                              const int* __restrict__ skip) {
+                                              const int* __restrict__ skip,
+                                              const unsigned char* __restrict__ want) {
     __shared__ c64 red[4];
     const int rhs = blockIdx.y;
     if (skip && skip[rhs]) return;
@@ -207,7 +210,13 @@ __global__ void __launch_bounds__(FT) k_resid(Geo g, R mm, const cplx<R>* __rest
     const int n = tile * FT + threadIdx.x;
     const size_t off = (size_t)rhs * g.ns * 2;
     double nn = 0;
-    if (n < g.ns) {
+    bool on = n < g.ns;
+    if (on && want) {
+        // only the sites of the Schwarz blocks flagged in want (true coordinates)
+        const int x = n / g.nt, t = n - x * g.nt;
+        on = want[(x / g.sxel) * g.sbt + t / g.stel] != 0;
+    }
+    if (on) {
         cplx<R> o0, o1;
         wilson_site<R, false>(U, x + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
         cplx<R> a = v[off + 2 * n] - o0, b = v[off + 2 * n + 1] - o1;
