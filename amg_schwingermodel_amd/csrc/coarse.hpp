@@ -658,6 +658,9 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
             for (int bi = threadIdx.x >> 6; bi < L.g.ncol; bi += CW) w_block_gmres(A, K, l, list[bi], r, x);
             __syncthreads();
             T.lap(20);
+            // the residual after the last colour of the last sweep and its
+            // tolerance check decide nothing: x is final
+            if (colour == 1 && it + 1 == nu) return;
             c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
             for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
             T.lap(21);
@@ -852,15 +855,15 @@ __device__ __forceinline__ void fg_run(const CArgs<R>& A, CK<R>& K, int l, FgSca
                 for (int q = 0; q < maxIt; q++) s += S.eta[q] * cvt<double>(Z(q)[e]);
                 x[e] = cvt<R>(s);
             }
+            // after the last cycle the true residual decides nothing (only x
+            // leaves the solve)
+            if (k + 1 == restarts) break;
             c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
             for (int e = threadIdx.x; e < n; e += CT) r[e] = b[e] - Dx[e];
             err = sqrt(c_dot(K.S, r, r, n).re);
             if (err < tol * nb) break;
-            if (++k < restarts) {
-                pc = 1;
-                continue;
-            }
-            break;
+            ++k;
+            pc = 1;
         }
     }
     if (threadIdx.x == 0) frame_pop(K.S);
