@@ -628,6 +628,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
             xb[e] = cvt<R>(s);
         }
         wave_sync();
+        if (k + 1 == A.sap_restarts) break;  // the last cycle's residual decides nothing
         w_dlocal(A, l, blk, xb, V(1), part);
         for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
         wave_sync();
