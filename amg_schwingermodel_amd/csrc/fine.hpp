@@ -178,6 +178,21 @@ struct InBlock {
     int blk;
     __device__ bool operator()(int n) const { return sap_block_of(g, n) == blk; }
 };
+// Block-vector reads of the Schwarz-block stencil on non-square lattices: the
+// reference indexes the block vector by the local index m(n') that its block
+// test computes for a neighbour n' (getMandBlock, include/level.h:48-59, with
+// x = n / Nx; SAP_fine_level::D_B, src/sap.cpp:196-210), so an in-block
+// neighbour reads the entry of the block's site with local index m(n'),
+// sap_site(blk, m(n')), which differs from n' when Nx != Nt.  The centre site
+// (and a masked-out neighbour, which reads the centre) reads its own entry.
+template <typename R, typename In> struct BlockLocalIn {
+    In in;
+    Geo g;
+    int blk, n;
+    __device__ __forceinline__ void operator()(int i, cplx<R>& a, cplx<R>& b) const {
+        in(i == n ? n : sap_site(g, blk, sap_local_of(g, i)), a, b);
+    }
+};
 
 // out = D in  (or D^dagger), all sites, all right-hand sides
 template <typename R, bool DAG>
@@ -327,7 +342,8 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
         wilson_core<R, false>(a.U, in, b.n, r0, r1, l0, l1, wt || b.mt + 1 < g.stel, wx || b.mx + 1 < g.sxel,
                               wt || b.mt > 0, wx || b.mx > 0, tr, tl, a.mm, o0, o1, c0, c1);
     } else {
-        wilson_site_in<R, false>(a.U, in, b.n, g.nx, g.nt, a.mm, InBlock{g, t.blk}, o0, o1);
+        wilson_site_in<R, false>(a.U, BlockLocalIn<R, In>{in, g, t.blk, b.n}, b.n, g.nx, g.nt, a.mm,
+                                 InBlock{g, t.blk}, o0, o1);
         in(b.n, c0, c1);
     }
 }
