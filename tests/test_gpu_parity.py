@@ -327,6 +327,33 @@ def test_force(prec, form):
         assert abs(fd - F[0][n, mu]) < 1e-4 * np.abs(F[0]).max() + 1e-6 * abs(fd), (n, mu, fd, F[0][n, mu])
 
 
+@pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED16])
+@pytest.mark.parametrize("case", ["32_3lvl", "32x16_2lvl"])
+def test_restart_lengths_at_compiled_limits(case, prec):
+    """SAP GMRES(8, 2), K-cycle FGMRES(8, 1), coarsest GMRES(24, 5) and outer
+    FGMRES(32, 10): the largest restart lengths the kernels are compiled for
+    (every SAP Arnoldi step J = 0..7, nine FP16 basis slots), against the
+    oracle with the same parameters."""
+    common = dict(sap_m=8, sap_restarts=2, kc_m=8, kc_restarts=1, crs_m=24, crs_restarts=5, out_m=32,
+                  out_restarts=10)
+    s, o, rows = make(case, common=common, precision=prec, max_rhs=2)
+    for l in range(len(rows)):
+        v = cvec(s.size(l), 35 + l)
+        x0 = cvec(s.size(l), 45 + l)
+        assert rel(s.sap(l, v, x0, 2), o.sap(l, v, x0, 2)[0]) < 1e-9, l
+    B = np.stack([cvec(s.size(0), 95 + i) for i in range(2)])
+    X, its, rrs, _ = s.solve(B)
+    for i in range(2):
+        xo, ito, _, _ = o.solve(B[i])
+        assert rrs[i] < 1e-10
+        if prec == amgs.PREC_FP64:
+            assert abs(its[i] - ito) <= 1, (its[i], ito)
+            assert rel(X[i], xo) < 1e-9
+        else:
+            assert its[i] <= ito + 3, (its[i], ito)
+            assert rel(X[i], xo) < 1e-8
+
+
 def test_solve_multi_uneven_configurations():
     """Concurrent solves of independent configurations of different lattice
     sizes, blockings, masses and couplings (amgs_solve_multi): each equals its
