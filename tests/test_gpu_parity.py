@@ -354,6 +354,33 @@ def test_restart_lengths_at_compiled_limits(case, prec):
             assert rel(X[i], xo) < 1e-8
 
 
+@pytest.mark.parametrize("smooth", [dict(nu1=1, nu2=3, sap_tol=0.3), dict(nu1=2, nu2=2, sap_tol=0.05)])
+@pytest.mark.parametrize("case", ["32_3lvl", "32_4lvl_odd"])
+def test_smoothing_parameters(case, smooth):
+    """Pre-smoothing (nu1 > 0), three post-smoothing sweeps and a loose SAP
+    tolerance, so that sweeps end early on the tolerance test after a black
+    colour (src/sap.cpp:64-174): SAP of every level over three sweeps, one
+    K-cycle and the full solve against the oracle; the FP16-basis solve
+    converges to the FP64 tolerance."""
+    s, o, rows = make(case, common=dict(smooth))
+    for l in range(len(rows)):
+        v = cvec(s.size(l), 98 + l)
+        x0 = cvec(s.size(l), 108 + l)
+        assert rel(s.sap(l, v, x0, 3), o.sap(l, v, x0, 3)[0]) < 1e-9, l
+    eta = cvec(s.size(0), 118)
+    assert rel(s.kcycle(eta), o.kcycle0(eta)) < 1e-9
+    b = cvec(s.size(0), 97)
+    x, it, rr, _ = s.solve(b)
+    xo, ito, _, _ = o.solve(b)
+    assert abs(it - ito) <= 1, (it, ito)
+    assert rr < 1e-10
+    assert rel(x, xo) < 1e-9
+    s16, _, _ = make(case, setup=True, common=dict(smooth), precision=amgs.PREC_MIXED16)
+    x16, it16, rr16, _ = s16.solve(b)
+    assert rr16 < 1e-10 and it16 <= ito + 3, (it16, ito)
+    assert rel(x16, xo) < 1e-8
+
+
 def test_solve_multi_uneven_configurations():
     """Concurrent solves of independent configurations of different lattice
     sizes, blockings, masses and couplings (amgs_solve_multi): each equals its
