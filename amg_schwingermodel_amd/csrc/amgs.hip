@@ -407,7 +407,12 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
     const size_t n0 = c->nvec(0), n1 = c->nvec(1);
     hipStream_t s = c->stream;
     const unsigned ntile = cdiv_u(g.ns, FT);
-    if (from_zero) hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(nrhs * n0, 256)), dim3(256), 0, s, nrhs * n0, psi);
+    // from zero without pre-smoothing nothing reads psi before the
+    // prolongation, which then writes it instead of adding to zero
+    const bool direct = from_zero && c->P.nu1 <= 0;
+    const int acc = direct ? 0 : 1;
+    if (from_zero && !direct)
+        hipLaunchKernelGGL(k_zero<R>, dim3(cdiv_u(nrhs * n0, 256)), dim3(256), 0, s, nrhs * n0, psi);
     const cplx<R>* rr = eta;  // r = eta - D psi = eta for psi = 0
     if (c->P.nu1 > 0 || !from_zero) {
         if (c->P.nu1 > 0) sap0(nrhs, eta, psi, c->P.nu1);
@@ -422,14 +427,14 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
         const dim3 gp(ntile, cdiv_u(nrhs, 8));
         const int nv = g.ntest, dofc = c->geo[1].dof, nsc = c->geo[1].ns;
         const c32* pc = (const c32*)psi1.as<c32>();
-        if (nv <= 8) hipLaunchKernelGGL(k_prolong0_g<8>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
-        else if (nv <= 12) hipLaunchKernelGGL(k_prolong0_g<12>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
-        else if (nv <= 16) hipLaunchKernelGGL(k_prolong0_g<16>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, 1);
+        if (nv <= 8) hipLaunchKernelGGL(k_prolong0_g<8>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
+        else if (nv <= 12) hipLaunchKernelGGL(k_prolong0_g<12>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
+        else if (nv <= 16) hipLaunchKernelGGL(k_prolong0_g<16>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
         else hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                                (const cplx<R>*)psi1.as<cplx<R>>(), psi, dofc, nsc, 1);
+                                (const cplx<R>*)psi1.as<cplx<R>>(), psi, dofc, nsc, acc);
     } else {
         hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, 1);
+                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc);
     }
     if (c->P.nu2 > 0) sap0(nrhs, eta, psi, c->P.nu2);
     HIP_CHECK(hipGetLastError());
