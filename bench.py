@@ -30,7 +30,9 @@ Prints one JSON line with "roofline" (the dominant kernel, the SAP Arnoldi
 step A: block Wilson stencil + Gram dot products, timed per launch with HIP
 events on the library stream during the timed solves) and "cpu_baseline"
 (the oracle, a C restatement of the reference path, timed on this host on a
-bounded sample of the same workload).
+bounded sample of the same workload).  Beside the value, "large_lattice"
+times a 1024x1024 4-level solve of 16 right-hand sides (beta = 4, m0 = -0.06,
+12 test vectors per level; --no-large skips it).
 """
 import argparse
 import json
@@ -48,6 +50,9 @@ HBM_PEAK_GBS = 8000.0
 ROWS = [(0, 8, 8, 10, 4, 4), (1, 4, 4, 10, 4, 4)]
 BETA, SWEEPS, GAUGE_SEED = 3.0, 500, 20260512
 M0_NAMED = -0.20
+# the larger lattice measured beside the value (near its critical mass)
+ROWS_L = [(0, 8, 8, 12, 4, 4), (1, 4, 4, 12, 4, 4), (2, 2, 2, 12, 2, 2)]
+BETA_L, GAUGE_SEED_L = 4.0, 20260513
 
 
 def parse():
@@ -74,6 +79,7 @@ def parse():
                          "(1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024 4-level solve beside the value")
     return ap.parse_args()
 
 
@@ -101,24 +107,25 @@ def main():
     Bd = torch.from_numpy(B.view(np.float64).copy()).to(dev)
     Xd = torch.zeros_like(Bd)
 
-    def solve_dev(solver, nrhs):
+    def solve_dev(solver, nrhs, bx=None):
+        bd, xd = bx or (Bd, Xd)
         its = np.zeros(nrhs, np.int32)
         rr = np.zeros(nrhs)
-        rc = amgs.lib().amgs_solve_dev(solver.h, nrhs, Bd.data_ptr(), Xd.data_ptr(), its.ctypes.data_as(amgs._I),
+        rc = amgs.lib().amgs_solve_dev(solver.h, nrhs, bd.data_ptr(), xd.data_ptr(), its.ctypes.data_as(amgs._I),
                                        rr.ctypes.data_as(amgs._P))
         solver._chk(rc, "amgs_solve_dev", ok=(amgs.AMGS_OK, amgs.AMGS_NOT_CONVERGED))
         return its, rr
 
-    def timed(solver, nrhs, steps, warmup, tag, on_start=None):
+    def timed(solver, nrhs, steps, warmup, tag, on_start=None, bx=None):
         for w in range(warmup):
-            its, rr = solve_dev(solver, nrhs)
+            its, rr = solve_dev(solver, nrhs, bx)
             log(f"{tag}: warmup {w + 1}/{warmup}: iterations {its.min()}..{its.max()}, relres {rr.max():.2e}")
         if on_start:
             on_start()
         torch.cuda.synchronize()
         t = time.perf_counter()
         for k in range(steps):
-            its, rr = solve_dev(solver, nrhs)
+            its, rr = solve_dev(solver, nrhs, bx)
             log(f"{tag}: step {k + 1}/{steps} done ({(time.perf_counter() - t):.2f}s)")
         torch.cuda.synchronize()
         return (time.perf_counter() - t) * 1e3 / steps, its, rr
@@ -179,42 +186,48 @@ def main():
     out["mixed_batched"] = mixed
     ms_best, concurrent = ms_step, 1
 
-    # ---- the same rhs as concurrent sub-batches, one context and stream each:
-    # one sub-batch's VALU-bound SAP step A overlaps another's HBM-bound step B
-    if 1 < a.split <= a.rhs:
+    def concurrent_solve(nl, rows, Ul, m0, nrhs, split, bd, xd, tag):
+        """The nrhs right-hand sides as `split` concurrent sub-batches, one
+        context and stream each: one sub-batch's VALU-bound SAP step A
+        overlaps another's HBM-bound step B.  Returns (ms per step, record)."""
         # sub-batch sizes as equal as possible
-        sizes = [a.rhs // a.split + (1 if i < a.rhs % a.split else 0) for i in range(a.split)]
-        first = [sum(sizes[:i]) for i in range(a.split)]
+        sizes = [nrhs // split + (1 if i < nrhs % split else 0) for i in range(split)]
+        first = [sum(sizes[:i]) for i in range(split)]
         ss = []
-        for i in range(a.split):
-            s = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=sizes[i],
-                                                precision=precs[a.precision]), U)
+        for i in range(split):
+            s = amgs.Solver(amgs.default_params(nl, nl, rows, m0=m0, max_rhs=sizes[i],
+                                                precision=precs[a.precision]), Ul)
             s.setup()
             wgs_i, ms_i = s.autotune(sizes[i])
-            log(f"sub-batch {i}: autotuner: SAP launches of {wgs_i} workgroups ({ms_i:.2f} ms per sweep)")
+            log(f"{tag} sub-batch {i}: autotuner: SAP launches of {wgs_i} workgroups ({ms_i:.2f} ms per sweep)")
             s.set_graph(bool(a.graph))
             ss.append(s)
-        bp = [Bd[first[i]].data_ptr() for i in range(a.split)]
-        xp = [Xd[first[i]].data_ptr() for i in range(a.split)]
+        bp = [bd[first[i]].data_ptr() for i in range(split)]
+        xp = [xd[first[i]].data_ptr() for i in range(split)]
         for w in range(a.warmup):
             amgs.solve_multi(ss, bp, xp, sizes)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for k in range(a.steps):
             st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, sizes)
-            log(f"{sizes} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
+            log(f"{tag} {sizes} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
         torch.cuda.synchronize()
         ms_split = (time.perf_counter() - t) * 1e3 / a.steps
         its_s, rr_s = np.concatenate(its_s), np.concatenate(rr_s)
-        log(f"{sizes} rhs concurrent: {ms_split:.1f} ms -> {ms_split / a.rhs:.2f} ms/rhs, iterations "
+        log(f"{tag} {sizes} rhs concurrent: {ms_split:.1f} ms -> {ms_split / nrhs:.2f} ms/rhs, iterations "
             f"{its_s.min()}..{its_s.max()}, max relres {rr_s.max():.2e}, status {st.tolist()}")
-        out["mixed_batched_concurrent"] = {"sub_batches": sizes, "ms_per_rhs": round(ms_split / a.rhs, 3),
-                                           "iterations_min": int(its_s.min()), "iterations_max": int(its_s.max()),
-                                           "relres_max": float(rr_s.max()), "status": st.tolist()}
-        if ms_split < ms_best and rr_s.max() < 1e-10 and all(v == 0 for v in st.tolist()):
-            ms_best, concurrent = ms_split, a.split
         for s in ss:
             s.close()
+        ok = rr_s.max() < 1e-10 and all(v == 0 for v in st.tolist())
+        return ms_split, ok, {"sub_batches": sizes, "ms_per_rhs": round(ms_split / nrhs, 3),
+                              "iterations_min": int(its_s.min()), "iterations_max": int(its_s.max()),
+                              "relres_max": float(rr_s.max()), "status": st.tolist()}
+
+    if 1 < a.split <= a.rhs:
+        ms_split, ok, rec = concurrent_solve(nx, ROWS, U, a.m0, a.rhs, a.split, Bd, Xd, f"{nx}^2")
+        out["mixed_batched_concurrent"] = rec
+        if ms_split < ms_best and ok:
+            ms_best, concurrent = ms_split, a.split
     if "plain_fp64" in out:
         out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] * a.rhs / ms_best, 2)
 
@@ -247,6 +260,34 @@ def main():
                        "relres_after": float(rr_e.max())}
             s.close()
         out["named_mass_sample"] = ex
+
+    # ---- a larger lattice: 1024^2, 4 levels, 16 rhs (not the value) ---------
+    if not a.no_large:
+        nl, nrl, m0l = 1024, 16, -0.06
+        t0 = time.time()
+        Ul, plq = amgs.gauge_metropolis(nl, nl, BETA_L, SWEEPS, GAUGE_SEED_L)
+        log(f"gauge field {nl}x{nl} beta={BETA_L}: plaquette {plq:.5f} ({time.time() - t0:.1f}s)")
+        Bl = np.stack([amgs.gaussian(nl * nl * 2, 5000 + i) for i in range(nrl)])
+        Bld = torch.from_numpy(Bl.view(np.float64).copy()).to(dev)
+        Xld = torch.zeros_like(Bld)
+        sl = amgs.Solver(amgs.default_params(nl, nl, ROWS_L, m0=m0l, max_rhs=nrl, precision=precs[a.precision]), Ul)
+        t = time.perf_counter()
+        sl.setup()
+        setup_l = (time.perf_counter() - t) * 1e3
+        sl.autotune(nrl)
+        sl.set_graph(bool(a.graph))
+        ms_l, its_l, rr_l = timed(sl, nrl, a.steps, a.warmup, f"{nl}^2 {nrl} rhs", bx=(Bld, Xld))
+        sl.close()
+        large = {"workload": f"{nl}x{nl} 4-level K-cycle FGMRES(20,50) to 1e-10, {nrl} Gaussian rhs, "
+                             f"beta={BETA_L} Metropolis ({SWEEPS} sweeps), m0={m0l}, "
+                             f"rows 0 8 8 12 4 4 / 1 4 4 12 4 4 / 2 2 2 12 2 2",
+                 "setup_ms": round(setup_l, 1), "ms_per_rhs": round(ms_l / nrl, 3),
+                 "iterations_min": int(its_l.min()), "iterations_max": int(its_l.max()),
+                 "relres_max": float(rr_l.max())}
+        if 1 < a.split <= nrl:
+            _, _, large["concurrent"] = concurrent_solve(nl, ROWS_L, Ul, m0l, nrl, a.split, Bld, Xld, f"{nl}^2")
+        out["large_lattice"] = large
+        del Bld, Xld
 
     # ---- CPU baseline: the oracle on this host, one right-hand side ----------
     if not a.no_cpu:
