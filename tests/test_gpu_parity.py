@@ -383,17 +383,18 @@ def test_smoothing_parameters(case, smooth):
 
 def test_solve_multi_uneven_configurations():
     """Concurrent solves of independent configurations of different lattice
-    sizes, blockings, masses and couplings (amgs_solve_multi): each equals its
+    sizes and shapes, blockings, masses and couplings (amgs_solve_multi): each equals its
     own solve on the same context, bit for bit, and converges."""
     import torch
 
-    confs = [(32, [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)], -0.10, 2.0, amgs.PREC_FP64, 1),
-             (64, [(0, 4, 4, 10, 4, 4)], -0.15, 3.0, amgs.PREC_MIXED, 2),
-             (48, [(0, 4, 4, 6, 4, 4), (1, 2, 2, 6, 2, 2)], -0.05, 1.0, amgs.PREC_MIXED16, 1)]
+    confs = [(32, 32, [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)], -0.10, 2.0, amgs.PREC_FP64, 1),
+             (64, 64, [(0, 4, 4, 10, 4, 4)], -0.15, 3.0, amgs.PREC_MIXED, 2),
+             (48, 48, [(0, 4, 4, 6, 4, 4), (1, 2, 2, 6, 2, 2)], -0.05, 1.0, amgs.PREC_MIXED16, 1),
+             (16, 32, [(0, 4, 8, 4, 2, 4), (1, 2, 2, 2, 2, 2)], 0.0, 2.0, amgs.PREC_MIXED16, 2)]
     solvers, bs, xs, nr = [], [], [], []
-    for i, (n, rows, m0, beta, prec, k) in enumerate(confs):
-        U, _ = amgs.gauge_metropolis(n, n, beta, 30, 300 + i)
-        s = amgs.Solver(amgs.default_params(n, n, rows, m0=m0, precision=prec, max_rhs=k), U)
+    for i, (nx, nt, rows, m0, beta, prec, k) in enumerate(confs):
+        U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 300 + i)
+        s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec, max_rhs=k), U)
         s.setup()
         solvers.append(s)
         b = np.stack([cvec(s.size(0), 310 + 10 * i + r) for r in range(k)])
@@ -413,11 +414,12 @@ def test_solve_multi_uneven_configurations():
 
 
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
-def test_graph_replay_matches_eager(prec):
+@pytest.mark.parametrize("case", ["32_3lvl", "16x32_3lvl"])
+def test_graph_replay_matches_eager(case, prec):
     """The level-0 preconditioner replayed from a captured HIP graph gives the
     eager result bit for bit, over many replays (every outer iteration of
     several solves), with eager solves in between and for two batch sizes."""
-    s, o, _ = make("32_3lvl", precision=prec, max_rhs=2)
+    s, o, _ = make(case, precision=prec, max_rhs=2)
     B = np.stack([cvec(s.size(0), 400 + i) for i in range(2)])
     xe1, ite1, rre1, _ = s.solve(B[0])
     xe2, ite2, _, _ = s.solve(B)
