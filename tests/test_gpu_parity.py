@@ -29,6 +29,9 @@ CASES = {
     "16x32_3lvl": (16, 32, [(0, 4, 8, 4, 2, 4), (1, 2, 2, 2, 2, 2)], 0.0, 2.0),
     # one level-0 Schwarz block: it is both the red and the black block
     "32_1blk_3lvl": (32, 32, [(0, 4, 4, 4, 1, 1), (1, 2, 2, 4, 2, 2)], -0.10, 2.0),
+    # Schwarz blocks of odd extent (15 x 15; no site pairs in the FP16 path)
+    # and an odd 5 x 5 coarsest lattice
+    "30_2lvl_odd_ext": (30, 30, [(0, 5, 5, 4, 2, 2)], -0.05, 2.0),
 }
 
 
