@@ -1,4 +1,4 @@
-"""HBM traffic of the SAP Arnoldi step A, from rocprofv3 hardware counters.
+"""HBM traffic of the SAP Arnoldi steps A and B, from rocprofv3 hardware counters.
 
 Two passes of the same short workload, one per counter (FETCH_SIZE, then
 WRITE_SIZE, both in KiB per dispatch), run from the repository root:
@@ -80,6 +80,15 @@ def summarize(fetch_dir, write_dir, out_json):
                     "sap_a_launches": len(sap), "sap_a_bytes_per_launch": float(np.mean(sap) * 1024 * scale)}
     res["hbm_bytes_per_launch"] = res["fetch"]["sap_a_bytes_per_launch"] + res["write"]["sap_a_bytes_per_launch"]
     res["kernel"] = "k_sap_arnoldi_a<float, J>, all J"
+    # step B of the same run, with the same copy scale
+    b = {}
+    for key, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        vals = [v for name, vs in _per_kernel(d, counter).items() if "k_sap_arnoldi_b<float" in name for v in vs]
+        b["launches"] = len(vals)
+        b[f"{key}_bytes_per_launch"] = float(np.mean(vals) * 1024 * res[key]["scale"])
+    b["hbm_bytes_per_launch"] = b["fetch_bytes_per_launch"] + b["write_bytes_per_launch"]
+    b["kernel"] = "k_sap_arnoldi_b<float, true>"
+    res["step_b"] = b
     with open(out_json, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res, indent=1))

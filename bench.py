@@ -171,9 +171,12 @@ def main():
     # roofline pass: the same batched solve with every launch of SAP step A
     # bracketed by HIP events on the library stream (eager launches)
     sm.kernel_timing(0, True)
+    sm.kernel_timing(1, True)
     ms_roof, _, _ = timed(sm, a.rhs, 1, 0, f"mixed {a.rhs} rhs, kernel timing")
     n_launch, k_ms, k_bytes = sm.kernel_stats(0)
+    nb_launch, kb_ms, kb_bytes = sm.kernel_stats(1)
     sm.kernel_timing(0, False)
+    sm.kernel_timing(1, False)
     log(f"mixed batched: setup {setup_mixed:.0f} ms, solve {ms_step:.1f} ms for {a.rhs} rhs -> {ms_rhs:.2f} ms/rhs, "
         f"iterations {its.min()}..{its.max()}, max relres {rrs.max():.2e}")
     mixed = {"precision": a.precision, "graph": bool(a.graph), "sap_workgroups_autotuned": tuned_wgs,
@@ -235,15 +238,25 @@ def main():
     avg_us = k_ms * 1e3 / max(n_launch, 1)
     bytes_per_launch = k_bytes / max(n_launch, 1)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    traffic = None
+    traffic = traffic_b = None
     tr_path = os.path.join(ROOT, "profiles", "sap_arnoldi_a_traffic.json")
     if os.path.exists(tr_path):
-        traffic = json.load(open(tr_path)).get("hbm_bytes_per_launch")
+        tr = json.load(open(tr_path))
+        traffic = tr.get("hbm_bytes_per_launch")
+        traffic_b = tr.get("step_b", {}).get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                        "kernel": "k_sap_arnoldi_a<float,J> (SAP block stencil + Gram dots, all J)",
                        "launches": n_launch, "avg_launch_us": round(avg_us, 2),
                        "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+    # step B (orthogonalisation against the stored basis + norm) from the same
+    # timing solve, the second-largest level-0 kernel
+    achieved_b = kb_bytes / (kb_ms * 1e-3) / 1e9 if kb_ms > 0 else 0.0
+    out["roofline_sap_step_b"] = {"bound": "hbm", "achieved": round(achieved_b, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(achieved_b / HBM_PEAK_GBS, 4), "traffic": traffic_b,
+                                  "kernel": "k_sap_arnoldi_b<float,H> (W -= sum H_ij Vt_i, |W|^2, Vt_j+1 stored)",
+                                  "launches": nb_launch, "avg_launch_us": round(kb_ms * 1e3 / max(nb_launch, 1), 2),
+                                  "algorithmic_bytes_per_launch": round(kb_bytes / max(nb_launch, 1))}
     sm.close()
 
     # ---- the named mass m0 = -0.20: bounded sample --------------------------
