@@ -595,6 +595,9 @@ static void outer_alloc(Ctx& c, int cap) {
     c.ocap = cap;
     c.oV.alloc(sizeof(c64) * (M + 1) * cap * n0);
     c.oZ.alloc(sizeof(c64) * M * cap * n0);
+    // mixed modes: the multigrid preconditioner's FP32 outputs are kept as
+    // they are (FP32 Z_j, exactly representable), not widened to FP64
+    if (c.mixed()) c.oZ32.alloc(sizeof(c32) * M * cap * n0);
     c.ox.alloc(sizeof(c64) * cap * n0);
     c.ob.alloc(sizeof(c64) * cap * n0);
     c.or_.alloc(sizeof(c64) * cap * n0);
@@ -675,8 +678,9 @@ static hipGraphExec_t precond_graph(Ctx& c, int nrhs) {
 }
 
 // eta32_ready: the FP32 staging input already holds vj (written by the
-// kernel that produced vj)
-static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_ready) {
+// kernel that produced vj); zj32: in the mixed modes, the FP32 Z_j that
+// receives the result instead of zj
+static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_ready, c32* zj32) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
     // graph replay (kernel timers record eager launches only)
     if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on) {
@@ -686,7 +690,9 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_r
             hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                                c.oeta32.as<c32>());
         HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs), c.stream));
-        if (c.mixed())
+        if (c.mixed() && zj32)
+            HIP_CHECK(hipMemcpyAsync(zj32, c.opsi32.p, sizeof(c32) * tot, hipMemcpyDeviceToDevice, c.stream));
+        else if (c.mixed())
             hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
                                (const c32*)c.opsi32.as<c32>(), zj);
         else
@@ -707,9 +713,10 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_r
         if (!eta32_ready)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                                c.oeta32.as<c32>());
-        c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
-                           (const c32*)c.opsi32.as<c32>(), zj);
+        c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), zj32 ? zj32 : c.opsi32.as<c32>());
+        if (!zj32)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                               (const c32*)c.opsi32.as<c32>(), zj);
     } else {
         c.e64.kcycle0(nrhs, vj, zj);
     }
@@ -755,6 +762,7 @@ static void outer_solve(Ctx& c, int nrhs) {
     // scaling kernel that writes V_j writes it too
     const bool eta32 = c.mixed() && c.outer_prec == 0;
     c32* oeta32 = eta32 ? c.oeta32.as<c32>() : nullptr;
+    c32* Z32 = eta32 ? c.oZ32.as<c32>() : nullptr;
     HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nrhs * n0, s));
     HIP_CHECK(hipMemsetAsync(st.done, 0, sizeof(int) * nrhs, s));
     HIP_CHECK(hipMemsetAsync(st.nhist, 0, sizeof(int) * nrhs, s));
@@ -773,9 +781,16 @@ static void outer_solve(Ctx& c, int nrhs) {
         if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
         hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0, eta32 ? oeta32 : nullptr);
         for (int j = 0; j < M; j++) {
-            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride, eta32);
-            hipLaunchKernelGGL(k_outer_a, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(),
-                               (const c64*)(Z + (size_t)j * vstride), (const c64*)V, w, st, j, vstride);
+            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride, eta32,
+                          Z32 ? Z32 + (size_t)j * vstride : nullptr);
+            if (Z32)
+                hipLaunchKernelGGL(k_outer_a<float>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
+                                   (const c64*)c.U64.as<c64>(), (const c32*)(Z32 + (size_t)j * vstride),
+                                   (const c64*)V, w, st, j, vstride);
+            else
+                hipLaunchKernelGGL(k_outer_a<double>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
+                                   (const c64*)c.U64.as<c64>(), (const c64*)(Z + (size_t)j * vstride),
+                                   (const c64*)V, w, st, j, vstride);
             hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
             hipLaunchKernelGGL(k_outer_b, g2, dim3(FT), 0, s, g.ns, (const c64*)V, w, st, j, vstride);
             hipLaunchKernelGGL(k_outer_fin, dim3(nrhs), dim3(FT), 0, s, st, j, tol);
@@ -801,7 +816,8 @@ static void outer_solve(Ctx& c, int nrhs) {
             if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
         }
         hipLaunchKernelGGL(k_outer_backsolve, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs);
-        hipLaunchKernelGGL(k_outer_update, g2, dim3(FT), 0, s, g.ns, (const c64*)Z, x, st, vstride);
+        if (Z32) hipLaunchKernelGGL(k_outer_update<float>, g2, dim3(FT), 0, s, g.ns, (const c32*)Z32, x, st, vstride);
+        else hipLaunchKernelGGL(k_outer_update<double>, g2, dim3(FT), 0, s, g.ns, (const c64*)Z, x, st, vstride);
         hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
                            (const c64*)x, r, st);
         if (k + 1 == c.P.out_restarts)

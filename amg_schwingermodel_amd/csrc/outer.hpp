@@ -94,9 +94,21 @@ __global__ void __launch_bounds__(FT) k_outer_scale(int ns, const c64* __restric
 }
 
 // w = D Z_j; partials <w, V_i> (i <= j), <V_j, V_k> (k < j)
-__global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __restrict__ U, const c64* __restrict__ Zj,
-                                                const c64* __restrict__ V, c64* __restrict__ w, OuterState st, int j,
-                                                size_t vstride) {
+// Z entries read in FP64: stored FP64, or FP32 (the mixed modes' preconditioner
+// outputs) converted exactly
+template <typename ZT> struct ZIn {
+    const cplx<ZT>* p;
+    __device__ __forceinline__ void operator()(int n, c64& a, c64& b) const {
+        const unsigned u = 2u * (unsigned)n;
+        a = cvt<double>(p[u]);
+        b = cvt<double>(p[u + 1u]);
+    }
+};
+
+template <typename ZT>
+__global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __restrict__ U,
+                                                const cplx<ZT>* __restrict__ Zj, const c64* __restrict__ V,
+                                                c64* __restrict__ w, OuterState st, int j, size_t vstride) {
     constexpr int NQ = 2 * OUT_MAXM + 1;
     __shared__ c64 red[(FT / 64) * NQ];
     const int rhs = blockIdx.y;
@@ -115,7 +127,7 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
         live[s] = n < g.ns;
         o0[s] = o1[s] = a0[s] = a1[s] = c64{0, 0};
         if (live[s]) {
-            wilson_site<double, false>(U, Zj + off, n, g.nx, g.nt, mm, AllSites{}, o0[s], o1[s]);
+            wilson_site_in<double, false>(U, ZIn<ZT>{Zj + off}, n, g.nx, g.nt, mm, AllSites{}, o0[s], o1[s]);
             w[off + 2 * n] = o0[s];
             w[off + 2 * n + 1] = o1[s];
             a0[s] = vj[2 * n];
@@ -278,7 +290,8 @@ __global__ void k_outer_backsolve(OuterState st, int nrhs) {
 }
 
 // x += sum_{j < maxIt} eta_j Z_j (src/fgmres.cpp:55-59)
-__global__ void __launch_bounds__(FT) k_outer_update(int ns, const c64* __restrict__ Z, c64* __restrict__ x,
+template <typename ZT>
+__global__ void __launch_bounds__(FT) k_outer_update(int ns, const cplx<ZT>* __restrict__ Z, c64* __restrict__ x,
                                                      OuterState st, size_t zstride) {
     const int rhs = blockIdx.y;
     if (st.done[rhs]) return;
@@ -289,9 +302,9 @@ __global__ void __launch_bounds__(FT) k_outer_update(int ns, const c64* __restri
     c64 x0 = x[off + 2 * n], x1 = x[off + 2 * n + 1];
     for (int j = 0; j < mi; j++) {
         const c64 e = st.eta[(size_t)rhs * M + j];
-        const c64* zj = Z + (size_t)j * zstride + off;
-        x0 = x0 + e * zj[2 * n];
-        x1 = x1 + e * zj[2 * n + 1];
+        const cplx<ZT>* zj = Z + (size_t)j * zstride + off;
+        x0 = x0 + e * cvt<double>(zj[2 * n]);
+        x1 = x1 + e * cvt<double>(zj[2 * n + 1]);
     }
     x[off + 2 * n] = x0;
     x[off + 2 * n + 1] = x1;
