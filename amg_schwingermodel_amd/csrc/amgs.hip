@@ -651,19 +651,20 @@ static void outer_alloc(Ctx& c, int cap) {
 // The multigrid cycle at level 0 on the fixed staging buffers (oeta32 ->
 // opsi32 in mixed precision, oeta64 -> opsi64 in FP64): the part of the
 // preconditioner that is captured into a HIP graph.
-static void precond_body(Ctx& c, int nrhs) {
-    if (c.mixed()) c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), c.opsi32.as<c32>());
+static void precond_body(Ctx& c, int nrhs, c32* out32) {
+    if (c.mixed()) c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), out32 ? out32 : c.opsi32.as<c32>());
     else c.e64.kcycle0(nrhs, c.oeta64.as<c64>(), c.opsi64.as<c64>());
 }
 
-// Executable graph of precond_body for this batch size (captured on first use).
-static hipGraphExec_t precond_graph(Ctx& c, int nrhs) {
+// Executable graph of precond_body for this batch size and output (captured
+// on first use; the outer FGMRES's buffers are fixed for the context's life).
+static hipGraphExec_t precond_graph(Ctx& c, int nrhs, c32* out32) {
     for (auto& g : c.pgraphs)
-        if (g.first == nrhs) return g.second;
+        if (g.nrhs == nrhs && g.out == out32) return g.exec;
     hipGraph_t graph = nullptr;
     HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     try {
-        precond_body(c, nrhs);
+        precond_body(c, nrhs, out32);
     } catch (...) {
         if (hipStreamEndCapture(c.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
         throw;
@@ -673,7 +674,7 @@ static hipGraphExec_t precond_graph(Ctx& c, int nrhs) {
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIP_CHECK(e);
-    c.pgraphs.emplace_back(nrhs, exec);
+    c.pgraphs.push_back({nrhs, out32, exec});
     return exec;
 }
 
@@ -689,14 +690,13 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_r
         else if (!eta32_ready)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
                                c.oeta32.as<c32>());
-        HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs), c.stream));
-        if (c.mixed() && zj32)
-            HIP_CHECK(hipMemcpyAsync(zj32, c.opsi32.p, sizeof(c32) * tot, hipMemcpyDeviceToDevice, c.stream));
-        else if (c.mixed())
+        // mixed modes: the graph of this Z_j writes it directly
+        HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs, c.mixed() ? zj32 : nullptr), c.stream));
+        if (!c.mixed())
+            HIP_CHECK(hipMemcpyAsync(zj, c.opsi64.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
+        else if (!zj32)
             hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
                                (const c32*)c.opsi32.as<c32>(), zj);
-        else
-            HIP_CHECK(hipMemcpyAsync(zj, c.opsi64.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
         return;
     }
     if (c.outer_prec == 1) {

@@ -185,10 +185,16 @@ struct Ctx {
     // HIP-graph replay of the level-0 preconditioner, one executable graph per
     // batch size; FP64 inputs/outputs are staged through fixed buffers
     bool graph = false;
-    std::vector<std::pair<int, hipGraphExec_t>> pgraphs;
+    // keyed by batch size and output buffer (mixed modes: one per FP32 Z_j)
+    struct PGraph {
+        int nrhs;
+        const void* out;
+        hipGraphExec_t exec;
+    };
+    std::vector<PGraph> pgraphs;
     DevBuf oeta64, opsi64;
     void clear_graphs() {
-        for (auto& g : pgraphs) (void)hipGraphExecDestroy(g.second);
+        for (auto& g : pgraphs) (void)hipGraphExecDestroy(g.exec);
         pgraphs.clear();
     }
 
