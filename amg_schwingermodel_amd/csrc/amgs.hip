@@ -594,10 +594,12 @@ static void outer_alloc(Ctx& c, int cap) {
     const int M = c.P.out_m;
     c.ocap = cap;
     c.oV.alloc(sizeof(c64) * (M + 1) * cap * n0);
-    c.oZ.alloc(sizeof(c64) * M * cap * n0);
     // mixed modes: the multigrid preconditioner's FP32 outputs are kept as
-    // they are (FP32 Z_j, exactly representable), not widened to FP64
+    // they are (FP32 Z_j, exactly representable), not widened to FP64; the
+    // FP64 basis is then allocated only by the solvers that use it
+    // (solve_with: one SAP sweep or no preconditioner)
     if (c.mixed()) c.oZ32.alloc(sizeof(c32) * M * cap * n0);
+    else c.oZ.alloc(sizeof(c64) * M * cap * n0);
     c.ox.alloc(sizeof(c64) * cap * n0);
     c.ob.alloc(sizeof(c64) * cap * n0);
     c.or_.alloc(sizeof(c64) * cap * n0);
@@ -610,6 +612,8 @@ static void outer_alloc(Ctx& c, int cap) {
         c.opsi64.alloc(sizeof(c64) * cap * n0);
     }
     const int ntile = (int)cdiv_u(c.geo[0].ns, FT);
+    // per-tile partial sums of the final true residual check
+    c.rel_part.alloc(sizeof(double) * 2 * (size_t)cap * ntile);
     const int hcap = M * c.P.out_restarts;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
@@ -844,17 +848,16 @@ static void true_relres(Ctx& c, int nrhs, double* out) {
     const Geo& g = c.geo[0];
     const size_t n0 = c.nvec(0);
     const unsigned ntile = cdiv_u(g.ns, FT);
-    DevBuf pr, pb;
-    pr.alloc(sizeof(double) * nrhs * ntile);
-    pb.alloc(sizeof(double) * nrhs * ntile);
+    double* pr = c.rel_part.as<double>();
+    double* pb = pr + (size_t)nrhs * ntile;
     hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
                        (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), (const c64*)c.ox.as<c64>(),
-                       c.ow.as<c64>(), pr.as<double>(), (const int*)nullptr, nullptr);
+                       c.ow.as<c64>(), pr, (const int*)nullptr, nullptr);
     hipLaunchKernelGGL(k_norm_part<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g.ns,
-                       (const c64*)c.ob.as<c64>(), pb.as<double>());
+                       (const c64*)c.ob.as<c64>(), pb);
     std::vector<double> hr((size_t)nrhs * ntile), hb((size_t)nrhs * ntile);
-    HIP_CHECK(hipMemcpyAsync(hr.data(), pr.p, hr.size() * 8, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipMemcpyAsync(hb.data(), pb.p, hb.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hr.data(), pr, hr.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hb.data(), pb, hb.size() * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     (void)n0;
     for (int r = 0; r < nrhs; r++) {
@@ -1433,6 +1436,8 @@ static int solve_with(amgs_ctx* ctx, int prec, int nrhs, const double* b, double
     need(nrhs >= 1 && nrhs <= c.P.max_rhs && (prec != 1 || nrhs <= c.e64.cap), AMGS_ERR_ARG, "bad nrhs");
     const size_t n0 = c.nvec(0);
     bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    // the FP64 preconditioned basis (mixed-precision contexts allocate it here)
+    c.oZ.alloc(sizeof(c64) * c.P.out_m * c.ocap * n0);
     HIP_CHECK(hipMemcpyAsync(c.ob.p, b, sizeof(c64) * n0 * nrhs, hipMemcpyHostToDevice, c.stream));
     c.outer_prec = prec;
     int rc;

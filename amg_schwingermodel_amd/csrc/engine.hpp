@@ -166,7 +166,7 @@ struct Ctx {
     Engine<double> e64;
     Engine<float> e32;
     // outer FGMRES
-    DevBuf oV, oZ, oZ32, ox, ob, or_, ow, ost_mem, oeta32, opsi32;
+    DevBuf oV, oZ, oZ32, ox, ob, or_, ow, ost_mem, oeta32, opsi32, rel_part;
     OuterState ost{};
     int ocap = 0;
     // io staging
