@@ -86,6 +86,7 @@ SIGNATURES = {
     "amgs_apply_Pt": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_sap": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int], ctypes.c_int),
     "amgs_kcycle": ([_V, ctypes.c_int, _P, _P], ctypes.c_int),
+    "amgs_sap_dev_noalloc": ([_V, ctypes.c_int, ctypes.c_int, _V, _V, ctypes.c_int], ctypes.c_int),
     "amgs_get_tv": ([_V, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "amgs_get_links": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "amgs_gauge_metropolis": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64, _P, _P],

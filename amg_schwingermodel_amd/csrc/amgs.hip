@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -877,6 +878,31 @@ static void true_relres(Ctx& c, int nrhs, double* out) {
 // ===========================================================================
 using namespace amgs;
 
+// ---- static workspace of amgs_sap_dev_noalloc ----
+// A file-scope device array that every GPU the library runs on holds until the
+// process ends.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in symbol): amgs_create looks up its device address
+// once per device with hipGetSymbolAddress and the entry point uses that
+// address.  Every call on one GPU shares the array; calls that overlap on one
+// GPU race.
+static __device__ __align__(256) char scratch[AMGS_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device, null until looked up
+
+static void scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = ptr;
+}
+
+static void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
 struct amgs_ctx {
     Ctx c;
 };
@@ -918,6 +944,7 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
             throw ApiError(AMGS_ERR_ARG, "precision must be AMGS_PREC_FP64, AMGS_PREC_MIXED or AMGS_PREC_MIXED16");
         build_geometry(c);
         HIP_CHECK(hipGetDevice(&c.device));
+        scratch_setup(c.device);  // with c.device current
         HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&c.ev0));
         HIP_CHECK(hipEventCreate(&c.ev1));
@@ -1613,6 +1640,40 @@ extern "C" int amgs_sap(amgs_ctx* ctx, int level, int nrhs, const double* v, dou
         if (level == 0) c.e64.sap0(nrhs, a, b, nu);
         else c.e64.coarse(CM_SAP, level, nrhs, a, c.nvec(level), b, c.nvec(level), nu);
     }, x);
+    API_END
+}
+
+// amgs_sap on a coarse level (l >= 1) on device buffers with the per-rhs
+// workspace of the block solves (residual, D x, the Schwarz-block Krylov
+// vectors of the workgroup's wavefronts) in the static array above instead of
+// the context's: nothing is allocated.  Every workspace entry is written before
+// it is read, so the array needs no clearing between calls.
+extern "C" int amgs_sap_dev_noalloc(amgs_ctx* ctx, int level, int nrhs, const void* v_dev, void* x_dev, int nu) {
+    if (!ctx || !v_dev || !x_dev) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "coarse SAP needs amgs_setup");
+    need(level >= 1 && level < c.L - 1, AMGS_ERR_ARG, "level must be a coarse level with SAP smoothing (1 .. levels-2)");
+    need(nrhs >= 1, AMGS_ERR_ARG, "nrhs must be >= 1");
+    void* ws = scratch_ptr(c.device);
+    need(ws != nullptr, AMGS_ERR_STATE, "static workspace not set up on this device");
+    const size_t n = c.nvec(level);
+    CArgs<double> a = c.e64.ca;
+    CLevel<double>& lv = a.lv[level];
+    lv.sR = 0;
+    lv.sDx = n;
+    lv.blk = 2 * n;
+    a.ws_per_rhs = (2 * n + CW * lv.blkStride + 15) & ~size_t(15);  // 256-byte aligned per rhs
+    a.ws = (c64*)ws;
+    const size_t need_bytes = sizeof(c64) * a.ws_per_rhs * (size_t)nrhs;
+    if (need_bytes > AMGS_NOALLOC_SCRATCH_BYTES)
+        throw ApiError(AMGS_ERR_ARG, "amgs_sap_dev_noalloc: " + std::to_string(nrhs) + " right-hand sides need " +
+                                         std::to_string(need_bytes) + " bytes of workspace, above the static " +
+                                         std::to_string((size_t)AMGS_NOALLOC_SCRATCH_BYTES) + "-byte limit");
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
+    c.e64.coarse(CM_SAP, level, nrhs, (const c64*)v_dev, n, (c64*)x_dev, n, nu, &a);
+    c.sync();
+    return AMGS_OK;
     API_END
 }
 

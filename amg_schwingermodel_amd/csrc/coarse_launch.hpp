@@ -8,10 +8,10 @@ namespace amgs {
 
 template <typename R>
 void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out,
-                       size_t out_stride, int nu) {
-    if (nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
+                       size_t out_stride, int nu, const CArgs<R>* base) {
+    if (!base && nrhs > cap) throw ApiError(AMGS_ERR_ARG, "batch exceeds engine capacity");
     static_assert(sizeof(CArgs<R>) <= 4096, "coarse-kernel arguments exceed the kernel-argument segment");
-    CArgs<R> a = ca;
+    CArgs<R> a = base ? *base : ca;
     a.mm = R(c->P.m0 + 2.0);
     for (int l = 1; l < c->L; l++) {
         a.lv[l].links = links[l];

@@ -143,8 +143,10 @@ template <typename R> struct Engine {
     void init(Ctx* ctx, int capacity);
     void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
     void kcycle0(int nrhs, const cplx<R>* eta, cplx<R>* psi, bool from_zero = true);
+    // base: kernel arguments to start from instead of ca (a caller-provided
+    // workspace layout); the batch is then limited by that workspace, not cap
     void coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t in_stride, cplx<R>* out, size_t out_stride,
-                int nu = 0);
+                int nu = 0, const CArgs<R>* base = nullptr);
     void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
     // level-0 SAP launch shape chosen by the autotuner, per batch size
     std::vector<std::pair<int, int>> sap_tuned;

@@ -154,6 +154,17 @@ int amgs_apply_P(amgs_ctx* c, int level, int nrhs, const double* coarse, double*
 int amgs_apply_Pt(amgs_ctx* c, int level, int nrhs, const double* fine, double* coarse);
 /* SAP_C::SAP(v, x, nu) (src/sap.cpp:64-174); x is updated in place. */
 int amgs_sap(amgs_ctx* c, int level, int nrhs, const double* v, double* x, int nu);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): amgs_sap at a
+ * coarse level (1 .. levels-2) on device buffers, with the block solves'
+ * workspace in a static device array of AMGS_NOALLOC_SCRATCH_BYTES per GPU
+ * instead of allocated memory; the call allocates nothing.  The workspace
+ * holds 16 * (2 n + 8 * (sap_m + 3) * sspb * dof) bytes per right-hand side
+ * (n = nsites * dof of the level, sspb = sites per Schwarz block); larger
+ * batches return AMGS_ERR_ARG.  Calls that overlap on one GPU share the array
+ * and race.  Results equal amgs_sap's. */
+#define AMGS_NOALLOC_SCRATCH_BYTES (8 << 20)
+int amgs_sap_dev_noalloc(amgs_ctx* c, int level, int nrhs, const void* v_dev, void* x_dev, int nu);
 /* One preconditioner application psi = K(eta) from psi = 0 (include/amg.h:134-144):
  * the K-cycle (src/amg.cpp:135-178), or the V-cycle (src/amg.cpp:90-132) when
  * params.cycle == 0. */
