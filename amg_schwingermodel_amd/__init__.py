@@ -108,6 +108,8 @@ SIGNATURES = {
     "amgs_multigrid": ([_V, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _I, _P], ctypes.c_int),
     "amgs_set_graph": ([_V, ctypes.c_int], ctypes.c_int),
     "amgs_autotune": ([_V, ctypes.c_int, _I, _P], ctypes.c_int),
+    "amgs_get_tuning": ([_V, ctypes.c_int, _I, _I], ctypes.c_int),
+    "amgs_set_tuning": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_solve_multi": ([ctypes.c_int, ctypes.POINTER(_V), _I, ctypes.POINTER(_V), ctypes.POINTER(_V), _I, _P, _I],
                          ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -251,6 +253,17 @@ class Solver:
         ms = ctypes.c_double(0)
         self._chk(lib().amgs_autotune(self.h, nrhs, ctypes.byref(best), ctypes.byref(ms)), "amgs_autotune")
         return best.value, ms.value
+
+    def tuning(self, nrhs=1):
+        """(level-0 SAP workgroup target, coarse workgroups per rhs) for batches of nrhs."""
+        w = ctypes.c_int(0)
+        g = ctypes.c_int(0)
+        self._chk(lib().amgs_get_tuning(self.h, nrhs, ctypes.byref(w), ctypes.byref(g)), "amgs_get_tuning")
+        return w.value, g.value
+
+    def set_tuning(self, nrhs, sap_wgs=-1, coarse_grp=-1):
+        """Fix launch shapes for batches of nrhs (> 0 sets, 0 default, < 0 keeps)."""
+        self._chk(lib().amgs_set_tuning(self.h, nrhs, sap_wgs, coarse_grp), "amgs_set_tuning")
 
     def set_graph(self, enable=True):
         """Replay the level-0 preconditioner from a captured HIP graph."""

@@ -200,14 +200,38 @@ template <typename R> void Engine<R>::init(Ctx* ctx, int capacity) {
             lv.blk = wo; wo += CW * lv.blkStride;
         }
     }
+    // private copy of a grouped launch's output (any coarse level)
+    size_t nco = 0;
+    for (int l = 1; l < c->L; l++) nco = std::max(nco, c->nvec(l));
+    ca.x_top = wo;
+    wo += nco;
     ca.ws_per_rhs = (wo + 63) & ~size_t(63);
     const char* pe = std::getenv("AMGS_PROFILE");
     if (pe && pe[0] && pe[0] != '0') {
         prof.alloc(sizeof(unsigned long long) * 32 * cap);
         HIP_CHECK(hipMemsetAsync(prof.p, 0, prof.bytes, c->stream));
     }
-    cws.alloc(sizeof(cplx<R>) * ca.ws_per_rhs * cap);
+    // grouped coarse launches: up to one member per Schwarz block of a colour
+    // on the coarse levels with SAP smoothing (at most GRP_MAX)
+    grp_max = 1;
+    size_t nsap = 0;
+    for (int l = 1; l < c->L - 1; l++) {
+        grp_max = std::max(grp_max, c->geo[l].ncol);
+        nsap = std::max(nsap, c->nvec(l));
+    }
+    grp_max = std::min(grp_max, GRP_MAX);
+    cws.alloc(sizeof(cplx<R>) * ca.ws_per_rhs * cap * grp_max);
     ca.ws = cws.as<cplx<R>>();
+    ca.grp = 1;
+    ca.err = c->cerr.as<int>();
+    if (grp_max > 1) {
+        bar.alloc(sizeof(unsigned) * cap);
+        HIP_CHECK(hipMemsetAsync(bar.p, 0, bar.bytes, c->stream));
+        xchg.alloc(sizeof(cplx<R>) * cap * nsap);
+        ca.bar = bar.as<unsigned>();
+        ca.xchg = xchg.as<cplx<R>>();
+        ca.xchg_n = nsap;
+    }
 }
 
 template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out) {
@@ -950,6 +974,8 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
         HIP_CHECK(hipEventCreate(&c.ev1));
         const size_t n0 = c.nvec(0);
         c.U64.alloc(sizeof(c64) * n0);
+        c.cerr.alloc(sizeof(int));
+        HIP_CHECK(hipMemsetAsync(c.cerr.p, 0, sizeof(int), c.stream));
         {
             // level-0 Schwarz block flags: [0][blk] red, [1][blk] black
             const Geo& g0 = c.geo[0];
@@ -1048,6 +1074,17 @@ extern "C" int amgs_setup_from_tv(amgs_ctx* ctx, const double* const* tv, int sm
     API_END
 }
 
+// A member barrier of a grouped coarse launch that waited beyond its bound
+// (the members of a right-hand side did not all run): the results are void.
+static void check_coarse_error(Ctx& c) {
+    int e = 0;
+    HIP_CHECK(hipMemcpy(&e, c.cerr.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) {
+        HIP_CHECK(hipMemset(c.cerr.p, 0, sizeof(int)));
+        throw ApiError(AMGS_ERR_HIP, "coarse-level solve: a member barrier timed out (results discarded)");
+    }
+}
+
 static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist, int hist_cap, int* nhist) {
     need(c.setup_done || c.outer_prec != 0, AMGS_ERR_STATE, "amgs_setup first");
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
@@ -1055,6 +1092,7 @@ static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist
     auto t0 = std::chrono::steady_clock::now();
     outer_solve(c, nrhs);
     c.sync();
+    check_coarse_error(c);
     c.last_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c.last_ms[0] = c.last_ms[1] + c.last_ms[2];
     std::vector<int> it(nrhs), nh(nrhs);
@@ -1163,6 +1201,48 @@ template <typename R> static int autotune_sap(Ctx& c, Engine<R>& e, int nrhs, do
     return best;
 }
 
+// Workgroups per right-hand side of the coarse K-cycle launch: times the
+// coarse solve of one K-cycle (FGMRES at level 1 and everything below it) on a
+// fixed input for each candidate and keeps the fastest for this batch size.
+// The members split only the Schwarz blocks, so every candidate gives the same
+// result bit for bit.
+template <typename R> static int autotune_coarse(Ctx& c, Engine<R>& e, int nrhs, double* best_ms) {
+    if (e.grp_max <= 1) {
+        if (best_ms) *best_ms = 0;
+        return 1;
+    }
+    const size_t n1 = c.nvec(1), tot = (size_t)nrhs * n1;
+    DevBuf v, x;
+    v.alloc(sizeof(cplx<R>) * tot);
+    x.alloc(sizeof(cplx<R>) * tot);
+    const bool f32 = std::is_same<R, float>::value;
+    hipLaunchKernelGGL(k_fill_pattern, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                       f32 ? (c32*)v.p : nullptr, f32 ? nullptr : (c64*)v.p);
+    const int mode = c.P.cycle == 0 ? CM_CYCLE : CM_KC_SOLVE;
+    int best = 1;
+    double bms = 1e30;
+    for (int g = 1; g <= e.grp_max; g *= 2) {
+        e.coarse_tuned.assign(1, {nrhs, g});
+        float ms = 0, acc = 0;
+        for (int rep = 0; rep < 3; rep++) {
+            HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+            e.coarse(mode, 1, nrhs, v.as<cplx<R>>(), n1, x.as<cplx<R>>(), n1);
+            HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+            HIP_CHECK(hipEventSynchronize(c.ev1));
+            HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+            if (rep > 0) acc += ms;  // the first run warms up
+        }
+        check_coarse_error(c);
+        if (acc < bms) {
+            bms = acc;
+            best = g;
+        }
+    }
+    e.coarse_tuned.assign(1, {nrhs, best});
+    if (best_ms) *best_ms = bms / 2;
+    return best;
+}
+
 extern "C" int amgs_autotune(amgs_ctx* ctx, int nrhs, int* best_wgs, double* sweep_ms) {
     if (!ctx) return AMGS_ERR_ARG;
     API_BEGIN
@@ -1171,7 +1251,45 @@ extern "C" int amgs_autotune(amgs_ctx* ctx, int nrhs, int* best_wgs, double* swe
     need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
     c.clear_graphs();
     const int best = c.mixed() ? autotune_sap(c, c.e32, nrhs, sweep_ms) : autotune_sap(c, c.e64, nrhs, sweep_ms);
+    if (c.mixed()) autotune_coarse(c, c.e32, nrhs, nullptr);
+    else autotune_coarse(c, c.e64, nrhs, nullptr);
     if (best_wgs) *best_wgs = best;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_get_tuning(amgs_ctx* ctx, int nrhs, int* sap_wgs, int* coarse_grp) {
+    if (!ctx || nrhs < 1) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    if (c.mixed()) {
+        if (sap_wgs) *sap_wgs = c.e32.sap_wgs(nrhs);
+        if (coarse_grp) *coarse_grp = c.e32.coarse_grp(nrhs);
+    } else {
+        if (sap_wgs) *sap_wgs = c.e64.sap_wgs(nrhs);
+        if (coarse_grp) *coarse_grp = c.e64.coarse_grp(nrhs);
+    }
+    return AMGS_OK;
+    API_END
+}
+
+template <typename R> static void set_tuning(Engine<R>& e, int nrhs, int sap_wgs, int coarse_grp) {
+    auto put = [nrhs](std::vector<std::pair<int, int>>& v, int val) {
+        for (auto it = v.begin(); it != v.end();)
+            it = it->first == nrhs ? v.erase(it) : it + 1;
+        if (val > 0) v.push_back({nrhs, val});
+    };
+    if (sap_wgs >= 0) put(e.sap_tuned, sap_wgs);
+    if (coarse_grp >= 0) put(e.coarse_tuned, coarse_grp);
+}
+
+extern "C" int amgs_set_tuning(amgs_ctx* ctx, int nrhs, int sap_wgs, int coarse_grp) {
+    if (!ctx || nrhs < 1 || coarse_grp > GRP_MAX) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    c.clear_graphs();
+    if (c.mixed()) set_tuning(c.e32, nrhs, sap_wgs, coarse_grp);
+    set_tuning(c.e64, nrhs, sap_wgs, coarse_grp);
     return AMGS_OK;
     API_END
 }

@@ -23,6 +23,7 @@ constexpr int CW = CT / 64;    // wavefronts per coarse workgroup
 constexpr int MAXL = 5;        // maximum number of levels
 constexpr int KC_MAXM = 8;     // max restart length of K-cycle FGMRES and SAP GMRES
 constexpr int CRS_MAXM = 24;   // max restart length of the coarsest GMRES
+constexpr int GRP_MAX = 8;     // max workgroups per right-hand side
 
 enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4, CM_CYCLE = 5 };
 
@@ -55,9 +56,24 @@ template <typename R> struct CArgs {
     int nu1, nu2;
     int vcycle;  // 1: V-cycle (src/amg.cpp:90-132), 0: K-cycle (src/amg.cpp:135-178)
     cplx<R>* ws;
-    size_t ws_per_rhs;
+    size_t ws_per_rhs;  // workspace entries of one workgroup
+    // Workgroups per right-hand side (grp >= 1, modes CM_KC_SOLVE and
+    // CM_CYCLE): the grp members of a right-hand side run the whole nested
+    // solve redundantly, each in its own workspace, and split the Schwarz
+    // blocks of every SAP colour between them; the blocks' results meet in the
+    // exchange vector xchg[rhs] behind a barrier of the members (arrival
+    // counter bar[rhs], zero at launch).  x_top: workspace offset of the
+    // member's private copy of the solve's output.
+    int grp;
+    unsigned* bar;
+    cplx<R>* xchg;
+    size_t xchg_n, x_top;
+    int* err;  // set when a member barrier waits beyond its bound
     int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
     int sap_lds[MAXL];  // SAP block vectors of level l in dynamic LDS (CW * blkStride entries)
+    // level l's Schwarz blocks solved one per member by the whole workgroup
+    // with the block's links staged in dynamic LDS (g_block_gmres_2x2)
+    int sap_wg[MAXL];
     unsigned long long* prof;  // optional phase timers [rhs][32] (wall clock ticks)
     int mode, level, nu;
     const cplx<R>* in;
@@ -76,6 +92,7 @@ struct WaveScal {
     c64 R[KC_MAXM][KC_MAXM];
     c64 g[KC_MAXM + 1], cn[KC_MAXM], sn[KC_MAXM], eta[KC_MAXM];
     c64 col[KC_MAXM + 1];
+    double bsc[2];  // |b_B|, |r_B| broadcast to the workgroup (g_block_gmres_2x2)
     int brk;
 };
 constexpr int VIN_BYTES = 16384;  // LDS staging of stencil inputs
@@ -95,6 +112,7 @@ struct CLds {
     WaveScal wv[CW];
     CFrame fg[MAXL], kcf[MAXL], crsf;  // frames per level and kind
     int stk[2 * MAXL + 2], sp;         // frame stack: kind * 16 + level
+    unsigned bar_epi;                  // member barriers passed so far
 };
 
 // ---------------------------------------------------------------------------
@@ -102,11 +120,49 @@ struct CLds {
 // ---------------------------------------------------------------------------
 template <typename R> struct CK {
     const CArgs<R>& A;
-    cplx<R>* W;    // this rhs's workspace (global memory)
+    cplx<R>* W;    // this workgroup's workspace (global memory)
     CLds& S;
     cplx<R>* dyn;  // dynamic LDS region
     unsigned long long* prof;
+    int rhs, member;  // right-hand side and member index (0 .. grp-1)
 };
+
+// Barrier of the grp members of one right-hand side.  Every thread makes its
+// global stores visible device-wide (agent-scope release: the members may sit
+// on different XCDs, whose L2 caches are not coherent with each other), one
+// thread counts the arrival and waits for the episode's total, and every
+// thread then acquires at agent scope before it reads what the others wrote.
+// The wait is bounded: past the bound the error flag is raised and the member
+// goes on (the host reports the error), so a launch always drains.
+constexpr unsigned MEMBER_SPIN_LIMIT = 1u << 22;
+template <typename R> __device__ __forceinline__ void member_sync(CK<R>& K) {
+    const CArgs<R>& A = K.A;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned target = (unsigned)A.grp * ++K.S.bar_epi;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(A.bar + K.rhs, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(A.bar + K.rhs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > MEMBER_SPIN_LIMIT) {
+                __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// zero the member-barrier arrival counters before a grouped launch (a kernel,
+// not a memset, so that the launch sequence can be captured in a HIP graph)
+static __global__ void k_member_reset(int n, unsigned* bar) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bar[i] = 0u;
+}
 
 // phase timers (thread 0, device wall clock), only when a buffer is given
 struct PhaseTimer {
@@ -642,6 +698,182 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
     }
 }
 
+// Dynamic LDS of g_block_gmres_2x2 (bytes): the block's links in column-pair
+// order, one partial sum per (row, link block) and the block's m + 3 vectors.
+__host__ __device__ inline size_t wg_block_lds_bytes(int dof, int m) {
+    const size_t nb = 4 * (size_t)dof, nu = 3 * nb, np = dof / 2;
+    return np * nu * 16 + nu * 8 + (size_t)(m + 3) * nb * 8;
+}
+
+// The block stencil of g_block_gmres_2x2: out = D_B in on a 2x2-site block,
+// in and out in LDS.  Thread u < 12 D owns one (row, link block) partial sum
+// over the row's D columns, reading the staged links and the input in
+// 16-byte pairs; the row sums are formed in the order of w_dlocal_2x2p.
+template <typename R>
+__device__ __forceinline__ void wg_dlocal_2x2(R mm, int D, const float __attribute__((ext_vector_type(4)))* Lk,
+                                              cplx<R>* part, const cplx<R>* in, cplx<R>* out) {
+    using V4 = float __attribute__((ext_vector_type(4)));
+    const int nb = 4 * D, nu = 3 * nb, np = D / 2, u = threadIdx.x;
+    __syncthreads();  // in is complete
+    if (u < nu) {
+        const int q = u / nb, e = u - q * nb, m = e / D;
+        const V4* vp = (const V4*)(in + (q == 0 ? m : q == 1 ? (m ^ 1) : (m ^ 2)) * D);
+        cplx<R> acc = czero<R>();
+        for (int p = 0; p < np; p++) {
+            const V4 lv = Lk[p * nu + u], v = vp[p];
+            acc += cplx<R>{lv.x, lv.y} * cplx<R>{v.x, v.y};
+            acc += cplx<R>{lv.z, lv.w} * cplx<R>{v.z, v.w};
+        }
+        part[u] = acc;
+    }
+    __syncthreads();
+    if (u < nb) {
+        cplx<R> o = mm * in[u];
+        o -= part[u];
+        o -= part[nb + u];
+        o -= part[2 * nb + u];
+        out[u] = o;
+    }
+    __syncthreads();
+}
+
+// w_block_gmres for one 2x2-site Schwarz block run by the whole workgroup (a
+// member of a grouped launch that owns one block of the colour): the block's
+// links are staged in LDS once and serve every stencil of the block's
+// GMRES(m, restarts), which the workgroup's threads share one (row, link
+// block) partial sum each; the Krylov vector operations and the scalars stay
+// on wavefront 0 in the order of w_block_gmres.  Needs the FP32 engine's
+// column-pair links and sap_wg[l].
+template <typename R>
+__device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob,
+                                               cplx<R>* x) {
+    static_assert(sizeof(R) == 4, "FP32 engine only");
+    using V4 = float __attribute__((ext_vector_type(4)));
+    const CLevel<R>& L = A.lv[l];
+    const Geo& g = L.g;
+    const int D = g.dof, nb = 4 * D, nu = 3 * nb, np = D / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m = A.sap_m;
+    WaveScal& S = K.S.wv[0];
+    V4* Lk = (V4*)K.dyn;                      // [np][nu]
+    cplx<R>* part = (cplx<R>*)(Lk + np * nu);  // [nu]
+    cplx<R>* base = part + nu;                 // V_0 .. V_m, x_B, b_B
+    auto V = [&](int j) { return base + (size_t)j * nb; };
+    cplx<R>* xb = base + (size_t)(m + 1) * nb;
+    cplx<R>* bb = base + (size_t)(m + 2) * nb;
+    // stage the links; gather b_B; x_B = b_B
+    {
+        const int msz = D * D, ksz = g.ns * msz;
+        const gptr<cplx<R>> links2 = as_global(L.links2);
+        for (int i = tid; i < np * nu; i += CT) {
+            const int p = i / nu, u = i - p * nu;
+            const int q = u / nb, e = u - q * nb, mm = e / D, r = e - mm * D;
+            const int n = sap_site(g, blk, mm);
+            const int k = q == 0 ? 0 : q == 1 ? ((mm & 1) ? 3 : 1) : ((mm & 2) ? 4 : 2);
+            Lk[p * nu + u] = ((const __attribute__((address_space(1))) V4*)(links2 + k * ksz + n * msz + 2 * r))[p * D];
+        }
+        for (int e = tid; e < nb; e += CT) {
+            const int mm = e / D, d = e - mm * D;
+            const cplx<R> v = rglob[(size_t)sap_site(g, blk, mm) * D + d];
+            bb[e] = v;
+            xb[e] = v;
+        }
+    }
+    wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1));
+    if (wv == 0) {
+        for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
+        wave_sync();
+        const double nrm = sqrt(w_dot(bb, bb, nb).re);
+        const double er = sqrt(w_dot(V(0), V(0), nb).re);
+        if (lane == 0) {
+            S.bsc[0] = nrm;
+            S.bsc[1] = er;
+        }
+    }
+    __syncthreads();
+    const double normb = S.bsc[0];
+    double err = S.bsc[1];
+    if (normb == 0.0) return;  // zero block right-hand side: x_B = 0
+    int maxIt = m;
+    for (int k = 0; k < A.sap_restarts; k++) {
+        if (wv == 0) {
+            const R ib = R(1.0 / err);
+            for (int e = lane; e < nb; e += 64) V(0)[e] = ib * V(0)[e];
+            if (lane == 0) S.g[0] = c64{err, 0.0};
+        }
+        for (int j = 0; j < m; j++) {
+            wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1));
+            if (wv == 0) {
+                cplx<R>* w = V(j + 1);
+                for (int i = 0; i <= j; i++) {
+                    const c64 h = w_dot(w, V(i), nb);
+                    if (lane == 0) S.col[i] = h;
+                    const cplx<R> hr = cvt<R>(h);
+                    const cplx<R>* vi = V(i);
+                    for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
+                    wave_sync();
+                }
+                const double hn = sqrt(w_dot(w, w, nb).re);
+                if (hn > 0) {
+                    const R ih = R(1.0 / hn);
+                    for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
+                }
+                if (lane == 0) {
+                    S.col[j + 1] = c64{hn, 0.0};
+                    c64 cnj, snj;
+                    givens_col(S.col, S.cn, S.sn, j, cnj, snj);
+                    for (int i = 0; i <= j; i++) S.R[i][j] = S.col[i];
+                    const c64 gj = S.g[j];
+                    const c64 gn = -(snj * gj);
+                    S.cn[j] = cnj;
+                    S.sn[j] = snj;
+                    S.g[j] = conj(cnj) * gj;
+                    S.g[j + 1] = gn;
+                    S.brk = sqrt(norm2(gn)) < A.sap_gtol * normb;
+                }
+            }
+            __syncthreads();
+            if (S.brk) {
+                maxIt = j + 1;
+                break;
+            }
+        }
+        if (wv == 0) {
+            if (lane == 0) {
+                for (int i = maxIt - 1; i >= 0; i--) {
+                    c64 e = S.g[i];
+                    for (int q = i + 1; q < maxIt; q++) e -= S.R[i][q] * S.eta[q];
+                    S.eta[i] = cdiv(e, S.R[i][i]);
+                }
+            }
+            wave_sync();
+            for (int e = lane; e < nb; e += 64) {
+                c64 s = cvt<double>(xb[e]);
+                for (int j = 0; j < maxIt; j++) s += S.eta[j] * cvt<double>(V(j)[e]);
+                xb[e] = cvt<R>(s);
+            }
+        }
+        if (k + 1 == A.sap_restarts) break;  // the last cycle's residual decides nothing
+        wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1));
+        if (wv == 0) {
+            for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
+            wave_sync();
+            const double er = sqrt(w_dot(V(0), V(0), nb).re);
+            if (lane == 0) S.bsc[1] = er;
+        }
+        __syncthreads();
+        err = S.bsc[1];
+        if (err < A.sap_gtol * normb) break;
+    }
+    if (wv == 0) {
+        wave_sync();
+        for (int e = lane; e < nb; e += 64) {
+            const int mm = e / D, d = e - mm * D;
+            x[(size_t)sap_site(g, blk, mm) * D + d] += xb[e];
+        }
+    }
+}
+
 // SAP_C::SAP (src/sap.cpp:64-174): red then black blocks, residual after each colour.
 template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, CK<R>& K, int l, const cplx<R>* v, cplx<R>* x, int nu) {
     const CLevel<R>& L = A.lv[l];
@@ -651,12 +883,53 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
     const double vn = sqrt(c_dot(K.S, v, v, n).re);
     c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
     for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+    const int grp = A.grp, mem = K.member;
+    cplx<R>* xs = grp > 1 ? A.xchg + (size_t)K.rhs * A.xchg_n : nullptr;
+    const int D = L.g.dof, nbs = L.g.sspb * D;
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             const int* list = colour == 0 ? L.red : L.black;
             __syncthreads();
             PhaseTimer T(K.prof);
-            for (int bi = threadIdx.x >> 6; bi < L.g.ncol; bi += CW) w_block_gmres(A, K, l, list[bi], r, x);
+            bool wg = false;
+            if constexpr (sizeof(R) == 4) {
+                // one block per member, solved by the whole workgroup
+                if (A.sap_wg[l]) {
+                    wg = true;
+                    if (mem < L.g.ncol) {
+                        g_block_gmres_2x2<R>(A, K, l, list[mem], r, x);
+                        if (xs && threadIdx.x < 64) {
+                            for (int e = threadIdx.x; e < nbs; e += 64) {
+                                const int m = e / D, d = e - m * D;
+                                const size_t i = (size_t)sap_site(L.g, list[mem], m) * D + d;
+                                xs[i] = x[i];
+                            }
+                        }
+                    }
+                }
+            }
+            // block bi of the colour: member bi % grp, wavefront (bi / grp) % CW
+            for (int bi = mem + grp * (threadIdx.x >> 6); !wg && bi < L.g.ncol; bi += grp * CW) {
+                w_block_gmres(A, K, l, list[bi], r, x);
+                if (xs) {  // publish the block's new x for the other members
+                    for (int e = threadIdx.x & 63; e < nbs; e += 64) {
+                        const int m = e / D, d = e - m * D;
+                        const size_t i = (size_t)sap_site(L.g, list[bi], m) * D + d;
+                        xs[i] = x[i];
+                    }
+                }
+            }
+            if (xs) {
+                member_sync(K);
+                // the blocks that the other members solved
+                for (int e = threadIdx.x; e < L.g.ncol * nbs; e += CT) {
+                    const int bi = e / nbs;
+                    if (bi % grp == mem) continue;
+                    const int q = e - bi * nbs, m = q / D, d = q - m * D;
+                    const size_t i = (size_t)sap_site(L.g, list[bi], m) * D + d;
+                    x[i] = xs[i];
+                }
+            }
             __syncthreads();
             T.lap(20);
             // the residual after the last colour of the last sweep and its
@@ -941,18 +1214,23 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
     const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ CLds S;
     extern __shared__ __align__(16) char dyn_lds[];
-    const int rhs = blockIdx.x;
-    CK<R> K{A, A.ws + (size_t)rhs * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
-            A.prof ? A.prof + (size_t)rhs * 32 : nullptr};
+    const int rhs = blockIdx.x / A.grp, member = blockIdx.x - rhs * A.grp;
+    CK<R> K{A, A.ws + (size_t)blockIdx.x * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
+            A.prof && member == 0 ? A.prof + (size_t)rhs * 32 : nullptr, rhs, member};
     PhaseTimer Ttot(K.prof);
     const cplx<R>* in = A.in + (size_t)rhs * A.in_stride;
     cplx<R>* out = A.out + (size_t)rhs * A.out_stride;
+    // with several members each iterates on a private copy of the output (a
+    // member ahead of another must not change what the other still reads);
+    // member 0 stores the result
+    cplx<R>* xo = A.grp > 1 ? K.W + A.x_top : out;
+    if (threadIdx.x == 0) S.bar_epi = 0;
     const int l = A.level;
     switch (A.mode) {
         case CM_KC_SOLVE:  // FGMRES_k_cycle[1].fgmres(eta, eta, psi) (src/amg.cpp:161)
             if (threadIdx.x == 0) {
                 S.fg[1].in = in;
-                S.fg[1].out = out;
+                S.fg[1].out = xo;
                 S.fg[1].pc = 0;
                 S.sp = -1;
                 frame_push(S, FR_FG, 1);
@@ -965,10 +1243,10 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
         case CM_PROLONG: c_prolong(A, l, in, out, false); break;
         case CM_CYCLE: {  // the cycle at level l from psi = 0
             const int n = A.lv[l].g.ns * A.lv[l].g.dof;
-            for (int e = threadIdx.x; e < n; e += CT) out[e] = czero<R>();
+            for (int e = threadIdx.x; e < n; e += CT) xo[e] = czero<R>();
             if (threadIdx.x == 0) {
                 S.kcf[l].in = in;
-                S.kcf[l].out = out;
+                S.kcf[l].out = xo;
                 S.kcf[l].pc = 0;
                 S.sp = -1;
                 frame_push(S, FR_KC, l);
@@ -976,6 +1254,11 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
             c_run(A, K);
             break;
         }
+    }
+    if (xo != out && member == 0 && (A.mode == CM_KC_SOLVE || A.mode == CM_CYCLE)) {
+        const int n = A.lv[l].g.ns * A.lv[l].g.dof;
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += CT) out[e] = xo[e];
     }
     Ttot.lap(31);
 }

@@ -38,6 +38,26 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     a.out = out;
     a.in_stride = in_stride;
     a.out_stride = out_stride;
+    // the K-cycle / cycle solves split their SAP blocks over grp workgroups per rhs
+    a.grp = (!base && (mode == CM_KC_SOLVE || mode == CM_CYCLE)) ? coarse_grp(nrhs) : 1;
+    if (a.grp > 1) {
+        hipLaunchKernelGGL(k_member_reset, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, c->stream, nrhs, a.bar);
+        HIP_CHECK(hipGetLastError());
+    }
+    // FP32, one Schwarz block of a colour per member, 2x2-site blocks on a
+    // square level: the member's whole workgroup solves its block with the
+    // block's links staged in LDS
+    for (int l = 0; l < MAXL; l++) {
+        a.sap_wg[l] = 0;
+        if (!std::is_same<R, float>::value || l < 1 || l >= c->L - 1 || !links2[l]) continue;
+        const Geo& g = c->geo[l];
+        const size_t need = wg_block_lds_bytes(g.dof, a.sap_m);
+        if (a.grp >= g.ncol && g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2 && g.dof % 2 == 0 &&
+            12 * g.dof <= CT && need <= lds_cap) {
+            a.sap_wg[l] = 1;
+            dyn_bytes = std::max(dyn_bytes, need);
+        }
+    }
     if (dyn_bytes > 0) {
         static bool attr_set[2] = {false, false};
         bool& done = attr_set[sizeof(R) == 8];
@@ -47,7 +67,7 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
             done = true;
         }
     }
-    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs), dim3(CT), dyn_bytes, c->stream, a);
+    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
     HIP_CHECK(hipGetLastError());
 }
 

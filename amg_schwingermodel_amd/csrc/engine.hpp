@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -137,8 +138,21 @@ template <typename R> struct Engine {
     // K-cycle(0) temporaries
     DevBuf eta1, psi1, res0;
     // coarse workspace; the kernel arguments (ca + per-launch fields) travel by value
-    DevBuf cws, prof;
+    DevBuf cws, prof, bar, xchg;
     CArgs<R> ca{};
+    // workgroups per right-hand side of the coarse K-cycle launches: at most
+    // grp_max; the autotuned value per batch size, else the default
+    int grp_max = 1;
+    std::vector<std::pair<int, int>> coarse_tuned;
+    int coarse_grp(int nrhs) const {
+        for (const auto& t : coarse_tuned)
+            if (t.first == nrhs) return std::min(t.second, grp_max);
+        const char* e = std::getenv("AMGS_COARSE_GRP");
+        const int v = e ? std::atoi(e) : 0;
+        if (v > 0) return std::min(v, grp_max);
+        // default: fill the chip's 256 compute units, one workgroup each
+        return std::max(1, std::min(grp_max, 256 / std::max(nrhs, 1)));
+    }
 
     void init(Ctx* ctx, int capacity);
     void sap0(int nrhs, const cplx<R>* v, cplx<R>* x, int nu);
@@ -169,6 +183,7 @@ struct Ctx {
     Engine<float> e32;
     // outer FGMRES
     DevBuf oV, oZ, oZ32, ox, ob, or_, ow, ost_mem, oeta32, opsi32, rel_part;
+    DevBuf cerr;  // raised by a coarse-kernel member barrier that timed out
     OuterState ost{};
     int ocap = 0;
     // io staging

@@ -122,12 +122,21 @@ int amgs_set_graph(amgs_ctx* c, int enable);
 
 /* Autotuner (new): times one level-0 SAP sweep for a set of launch shapes
  * (workgroups per launch, i.e. lattice chunks per Schwarz block and rhs) and
- * keeps the fastest for batches of nrhs right-hand sides in this context.
- * Results stay deterministic: the choice fixes the chunking (and with it the
- * order of the per-chunk partial sums) for all later solves of that size.
- * best_wgs (may be NULL) receives the workgroup target, sweep_ms (may be
- * NULL) the sweep time it measured.  Captured graphs are dropped. */
+ * keeps the fastest for batches of nrhs right-hand sides in this context;
+ * then times the coarse-level solve of one K-cycle for 1, 2, 4, 8 workgroups
+ * per right-hand side (the coarse levels' Schwarz blocks of a colour split
+ * over them) and keeps the fastest.  Results stay deterministic: the SAP
+ * choice fixes the chunking (and with it the order of the per-chunk partial
+ * sums) for all later solves of that size, and the coarse choice does not
+ * change any arithmetic.  best_wgs (may be NULL) receives the workgroup
+ * target, sweep_ms (may be NULL) the sweep time it measured.  Captured
+ * graphs are dropped. */
 int amgs_autotune(amgs_ctx* c, int nrhs, int* best_wgs, double* sweep_ms);
+/* The launch shapes in force for batches of nrhs (new): level-0 SAP
+ * workgroup target and coarse workgroups per right-hand side.  set: a value
+ * > 0 fixes it, 0 restores the default, < 0 leaves it (drops captured graphs). */
+int amgs_get_tuning(amgs_ctx* c, int nrhs, int* sap_wgs, int* coarse_grp);
+int amgs_set_tuning(amgs_ctx* c, int nrhs, int sap_wgs, int coarse_grp);
 
 /* Uneven multi-configuration batch (new; the reference solves one gauge
  * configuration per process): the solves of nctx independent contexts - each

@@ -490,6 +490,49 @@ def test_graph_replay_across_restarts(prec):
             assert int(ite[i]) <= ito + 3 and rel(xe[i], xo) < 1e-8
 
 
+@pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED16])
+def test_coarse_member_groups_bitwise(prec):
+    """The coarse-level solve of a K-cycle with 1, 2, 4 or 8 workgroups per
+    right-hand side (the Schwarz blocks of each level-1 colour split over the
+    members, their results exchanged behind a member barrier): the solve is
+    the same bit for bit, eager and from a captured graph, and matches the
+    oracle.  64x64 with the 512^2 workload's coarse blocking (8x8 level-1
+    sites in 16 Schwarz blocks, 8 per colour)."""
+    nx = nt = 64
+    rows = [(0, 8, 8, 6, 4, 4), (1, 4, 4, 6, 4, 4)]
+    U, _ = amgs.gauge_metropolis(nx, nt, 3.0, 30, 1234)
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=-0.05, precision=prec, max_rhs=3), U)
+    s.setup()
+    B = np.stack([cvec(s.size(0), 600 + i) for i in range(3)])
+    ref = None
+    for g in (1, 2, 4, 8):
+        s.set_tuning(3, coarse_grp=g)
+        assert s.tuning(3)[1] == g
+        for graph in (False, True):
+            s.set_graph(graph)
+            X, its, rrs, hists = s.solve(B)
+            if ref is None:
+                ref = (X, its, hists)
+                continue
+            assert np.array_equal(its, ref[1]) and np.array_equal(X, ref[0]), (g, graph)
+            for i in range(3):
+                assert np.array_equal(hists[i], ref[2][i]), (g, graph, i)
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=-0.05, threads=8), U)
+    o.setup(draw_order=1)
+    xo, ito, histo, _ = o.solve(B[0])
+    if prec == amgs.PREC_FP64:
+        assert abs(int(ref[1][0]) - ito) <= 1 and rel(ref[0][0], xo) < 1e-9
+        assert_history(ref[2][0], histo, B[0])
+    else:
+        assert int(ref[1][0]) <= ito + 3 and rel(ref[0][0], xo) < 1e-8
+    # the autotuner picks one of the candidates; the default fills the chip
+    s.set_tuning(3, coarse_grp=0)
+    assert s.tuning(3)[1] == 8
+    s.autotune(3)
+    assert s.tuning(3)[1] in (1, 2, 4, 8)
+    s.close()
+
+
 def test_error_paths():
     """Status codes in place of the reference's exit(1) / silent returns:
     NaN or Inf in the gauge field or the right-hand side -> AMGS_ERR_NAN;
