@@ -82,6 +82,8 @@ SIGNATURES = {
     "amgs_solve_dev": ([_V, ctypes.c_int, _V, _V, _I, _P], ctypes.c_int),
     "amgs_apply_D": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_apply_D_dev": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _V, _V], ctypes.c_int),
+    "amgs_apply_D_batched_dev": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _V, _V], ctypes.c_int),
+    "amgs_autotune_batched_D": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _P], ctypes.c_int),
     "amgs_apply_P": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_apply_Pt": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_sap": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int], ctypes.c_int),
@@ -302,6 +304,33 @@ class Solver:
     def apply_D(self, level, x, dagger=False, nrhs=1):
         n = self.size(level)
         return self._apply(lib().amgs_apply_D, n, n, x, level, 1 if dagger else 0, nrhs, nrhs=nrhs)
+
+    def apply_D_batched(self, level, X, precision=PREC_FP64):
+        """The coarse operator on a block of vectors X [nrhs, n_l] by MFMA
+        block-GEMMs (device round trip through torch); returns [nrhs, n_l]."""
+        import torch
+
+        X = _c(X).reshape(-1, self.size(level))
+        nrhs = X.shape[0]
+        if precision == PREC_FP64:
+            xd = torch.from_numpy(X.view(np.float64).copy()).to("cuda:0")
+        else:
+            xd = torch.from_numpy(X.astype(np.complex64).view(np.float32).copy()).to("cuda:0")
+        od = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        self._chk(lib().amgs_apply_D_batched_dev(self.h, level, nrhs, precision, ctypes.c_void_p(xd.data_ptr()),
+                                                 ctypes.c_void_p(od.data_ptr())), "amgs_apply_D_batched_dev")
+        o = od.cpu().numpy()
+        o = o.view(np.complex128) if precision == PREC_FP64 else o.view(np.complex64).astype(np.complex128)
+        return o.reshape(nrhs, -1)
+
+    def autotune_batched_D(self, level, nrhs, precision=PREC_FP64):
+        """(wavefronts per workgroup, microseconds per apply) chosen for this level and batch size."""
+        w = ctypes.c_int(0)
+        us = ctypes.c_double(0)
+        self._chk(lib().amgs_autotune_batched_D(self.h, level, nrhs, precision, ctypes.byref(w), ctypes.byref(us)),
+                  "amgs_autotune_batched_D")
+        return w.value, us.value
 
     def apply_P(self, level, xc, nrhs=1):
         return self._apply(lib().amgs_apply_P, self.size(level + 1), self.size(level), xc, level, nrhs, nrhs=nrhs)

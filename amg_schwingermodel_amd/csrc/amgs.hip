@@ -33,6 +33,7 @@
 #include "fine.hpp"
 #include "host_util.hpp"
 #include "krylov.hpp"
+#include "mfma.hpp"
 #include "outer.hpp"
 #include "setup.hpp"
 
@@ -517,6 +518,19 @@ static void ortho_links(Ctx& c, int l) {
     HIP_CHECK(hipGetLastError());
 }
 
+// ---- the coarse operator over a block of right-hand sides (MFMA) ----
+// Level::D_operator (src/level.cpp:161-180) at level l >= 1 on nrhs vectors as
+// matrix-core block-GEMMs (k_cdop_mfma) in the engine's precision.
+template <typename R>
+static void cdop_mfma(Ctx& c, Engine<R>& e, int l, int nrhs, const cplx<R>* in, cplx<R>* out, int wpg = 0) {
+    const Geo& g = c.geo[l];
+    if (wpg <= 0) wpg = e.cdop_wpg(l, nrhs);
+    const size_t tasks = (size_t)g.ns * cdiv_u(nrhs, 16) * cdiv_u(g.dof, 16);
+    hipLaunchKernelGGL(k_cdop_mfma<R>, dim3(cdiv_u(tasks, wpg)), dim3(64 * wpg), 0, c.stream, g, R(c.P.m0 + 2.0),
+                       e.links[l], in, out, nrhs);
+    HIP_CHECK(hipGetLastError());
+}
+
 // Bootstrap (src/amg.cpp:52-84) in FP64: per level, every test vector v
 // becomes v + cycle_l(v - D_l v) from a zero guess with the current
 // hierarchy (all test vectors of the level as one batch), then the level's
@@ -539,7 +553,7 @@ static void bootstrap(Ctx& c) {
                                    (double*)nullptr, (const int*)nullptr, nullptr);
                 e.kcycle0(nv, rhs.as<c64>(), tvn.as<c64>());
             } else {
-                e.coarse(CM_DOP, l, nv, w, n, rhs.as<c64>(), n);
+                cdop_mfma(c, e, l, nv, (const c64*)w, rhs.as<c64>());
                 hipLaunchKernelGGL(k_lin<double>, dim3(nb), dim3(256), 0, c.stream, tot, 1.0, (const c64*)w, -1.0,
                                    (const c64*)rhs.as<c64>(), rhs.as<c64>());
                 e.coarse(CM_CYCLE, l, nv, rhs.as<c64>(), n, tvn.as<c64>(), n);
@@ -1653,6 +1667,75 @@ extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, int form, const double* phi, 
     API_END
 }
 
+// Tile autotuner of the batched coarse operator: wavefronts per workgroup
+// (each owns one 16-rhs x 16-row tile of a site) from 1 to 8, timed on a
+// fixed input; the fastest is kept per (level, batch size).  Every candidate
+// computes the same tiles in the same order, so results do not depend on it.
+template <typename R> static int autotune_cdop(Ctx& c, Engine<R>& e, int l, int nrhs, double* best_us) {
+    const size_t tot = (size_t)nrhs * c.nvec(l);
+    DevBuf v, x;
+    v.alloc(sizeof(cplx<R>) * tot);
+    x.alloc(sizeof(cplx<R>) * tot);
+    const bool f32 = std::is_same<R, float>::value;
+    hipLaunchKernelGGL(k_fill_pattern, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
+                       f32 ? (c32*)v.p : nullptr, f32 ? nullptr : (c64*)v.p);
+    int best = 4;
+    double bus = 1e30;
+    const int reps = 20;
+    for (int wpg : {1, 2, 4, 8}) {
+        cdop_mfma(c, e, l, nrhs, v.as<cplx<R>>(), x.as<cplx<R>>(), wpg);  // warm-up
+        HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+        for (int r = 0; r < reps; r++) cdop_mfma(c, e, l, nrhs, v.as<cplx<R>>(), x.as<cplx<R>>(), wpg);
+        HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+        HIP_CHECK(hipEventSynchronize(c.ev1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        if (ms * 1e3 / reps < bus) {
+            bus = ms * 1e3 / reps;
+            best = wpg;
+        }
+    }
+    const long long key = (long long)l << 32 | (unsigned)nrhs;
+    for (auto it = e.cdop_tuned.begin(); it != e.cdop_tuned.end();)
+        it = it->first == key ? e.cdop_tuned.erase(it) : it + 1;
+    e.cdop_tuned.push_back({key, best});
+    if (best_us) *best_us = bus;
+    return best;
+}
+
+extern "C" int amgs_apply_D_batched_dev(amgs_ctx* ctx, int level, int nrhs, int precision, const void* in, void* out) {
+    if (!ctx || !in || !out) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "coarse operators need amgs_setup");
+    need(level >= 1 && level < c.L, AMGS_ERR_ARG, "level must be a coarse level (1 .. levels-1)");
+    need(nrhs >= 1, AMGS_ERR_ARG, "nrhs must be >= 1");
+    if (precision == AMGS_PREC_FP64) {
+        cdop_mfma(c, c.e64, level, nrhs, (const c64*)in, (c64*)out);
+    } else {
+        need(c.mixed(), AMGS_ERR_STATE, "the FP32 hierarchy exists in mixed-precision contexts only");
+        cdop_mfma(c, c.e32, level, nrhs, (const c32*)in, (c32*)out);
+    }
+    c.sync();
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_autotune_batched_D(amgs_ctx* ctx, int level, int nrhs, int precision, int* best_wpg,
+                                       double* best_us) {
+    if (!ctx) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(level >= 1 && level < c.L && nrhs >= 1, AMGS_ERR_ARG, "bad level / nrhs");
+    need(precision == AMGS_PREC_FP64 || c.mixed(), AMGS_ERR_STATE, "FP32 hierarchy needs a mixed-precision context");
+    const int w = precision == AMGS_PREC_FP64 ? autotune_cdop(c, c.e64, level, nrhs, best_us)
+                                              : autotune_cdop(c, c.e32, level, nrhs, best_us);
+    if (best_wpg) *best_wpg = w;
+    return AMGS_OK;
+    API_END
+}
+
 // ---- component entry points (FP64 engine) ----
 static size_t level_n(Ctx& c, int l) { return c.nvec(l); }
 
@@ -1676,9 +1759,8 @@ extern "C" int amgs_apply_D_dev(amgs_ctx* ctx, int level, int dagger, int nrhs, 
     } else {
         need(!dagger, AMGS_ERR_ARG, "D^dagger only on the fine level");
         need(c.setup_done, AMGS_ERR_STATE, "coarse operators need amgs_setup");
-        need(nrhs <= c.e64.cap, AMGS_ERR_ARG, "nrhs exceeds capacity");
-        const size_t n = c.nvec(level);
-        c.e64.coarse(CM_DOP, level, nrhs, (const c64*)in, n, (c64*)out, n);
+        // all right-hand sides at once as matrix-core block-GEMMs
+        cdop_mfma(c, c.e64, level, nrhs, (const c64*)in, (c64*)out);
     }
     c.sync();
     return AMGS_OK;

@@ -164,6 +164,15 @@ template <typename R> struct Engine {
     void restrict0(int nrhs, const cplx<R>* w, const cplx<R>* v, cplx<R>* out);
     // level-0 SAP launch shape chosen by the autotuner, per batch size
     std::vector<std::pair<int, int>> sap_tuned;
+    // wavefronts per workgroup of the batched coarse operator (k_cdop_mfma),
+    // per (level, batch size): the autotuned value, else 4
+    std::vector<std::pair<long long, int>> cdop_tuned;
+    int cdop_wpg(int level, int nrhs) const {
+        const long long key = (long long)level << 32 | (unsigned)nrhs;
+        for (const auto& t : cdop_tuned)
+            if (t.first == key) return t.second;
+        return 4;
+    }
     int sap_wgs(int nrhs) const;
 };
 

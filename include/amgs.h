@@ -158,6 +158,19 @@ int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const voi
  * Level::D_operator (src/level.cpp:161-180) at level >= 1 (dagger must be 0). */
 int amgs_apply_D(amgs_ctx* c, int level, int dagger, int nrhs, const double* in, double* out);
 int amgs_apply_D_dev(amgs_ctx* c, int level, int dagger, int nrhs, const void* in_dev, void* out_dev);
+/* Level::D_operator (src/level.cpp:161-180) at a coarse level (1 .. levels-1)
+ * on nrhs device vectors [nrhs][nsites][dof] at once, as matrix-core
+ * block-GEMMs (new): per coarse site the five dof x dof link blocks against
+ * the five neighbour vectors of all right-hand sides, in 16x16x4 MFMA tiles
+ * (16 rhs x 16 rows x 4 columns; FP64 MFMA for precision AMGS_PREC_FP64, FP32
+ * MFMA on the FP32 hierarchy of a mixed-precision context otherwise).
+ * amgs_apply_D at coarse levels runs the FP64 form. */
+int amgs_apply_D_batched_dev(amgs_ctx* c, int level, int nrhs, int precision, const void* in_dev, void* out_dev);
+/* Tile autotuner of amgs_apply_D_batched_dev (new): times 1, 2, 4, 8
+ * wavefronts (one tile each) per workgroup for this level and batch size and
+ * keeps the fastest; results do not depend on the choice.  best_wpg, best_us
+ * (may be NULL): the choice and its time per apply in microseconds. */
+int amgs_autotune_batched_D(amgs_ctx* c, int level, int nrhs, int precision, int* best_wpg, double* best_us);
 /* Level::P_v and Level::Pt_v (src/level.cpp:182-237), level = the finer one. */
 int amgs_apply_P(amgs_ctx* c, int level, int nrhs, const double* coarse, double* fine);
 int amgs_apply_Pt(amgs_ctx* c, int level, int nrhs, const double* fine, double* coarse);

@@ -533,6 +533,34 @@ def test_coarse_member_groups_bitwise(prec):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["32_3lvl", "32_4lvl_odd", "16x32_3lvl", "30_2lvl_odd_ext"])
+def test_batched_coarse_operator_mfma(case):
+    """The coarse operator on a block of right-hand sides as MFMA block-GEMMs:
+    FP64 against the oracle's Level::D_operator at the operator tolerance for
+    batches of 1, 5, 16 and 33 vectors (partial and several 16-rhs tiles,
+    odd dof 4 and 6, non-square levels), independent of the autotuned launch
+    shape bit for bit; the FP32 form on the FP32 hierarchy within FP32
+    rounding."""
+    s, o, rows = make(case, setup=False, precision=amgs.PREC_MIXED)
+    L = len(rows) + 1
+    tvs = [np.concatenate([cvec(o.size(l), 100 * l + i) for i in range(rows[l][3])]) for l in range(L - 1)]
+    s.setup_from_tv(tvs, smooth=False)
+    o.setup_from_tv(tvs, smooth=False)
+    for l in range(1, L):
+        for nrhs in (1, 5, 16, 33):
+            X = np.stack([cvec(s.size(l), 700 + 10 * l + i) for i in range(nrhs)])
+            Y = s.apply_D_batched(l, X)
+            for i in range(nrhs):
+                assert rel(Y[i], o.dop(l, X[i])) < 1e-13, (l, nrhs, i)
+            w, us = s.autotune_batched_D(l, nrhs)
+            assert w in (1, 2, 4, 8) and us > 0
+            assert np.array_equal(s.apply_D_batched(l, X), Y), (l, nrhs)
+            Y32 = s.apply_D_batched(l, X, precision=amgs.PREC_MIXED)
+            for i in range(nrhs):
+                assert rel(Y32[i], Y[i]) < 5e-6, (l, nrhs, i)
+    s.close()
+
+
 def test_error_paths():
     """Status codes in place of the reference's exit(1) / silent returns:
     NaN or Inf in the gauge field or the right-hand side -> AMGS_ERR_NAN;
