@@ -73,6 +73,8 @@ SIGNATURES = {
     "amgs_default_params": ([ctypes.POINTER(Params)], ctypes.c_int),
     "amgs_read_parameters": ([ctypes.c_char_p, ctypes.POINTER(Params)], ctypes.c_int),
     "amgs_create": ([ctypes.POINTER(Params), ctypes.POINTER(_V)], ctypes.c_int),
+    "amgs_create_multi": ([ctypes.POINTER(Params), ctypes.c_int, _P, ctypes.POINTER(_V)], ctypes.c_int),
+    "amgs_num_configs": ([_V], ctypes.c_int),
     "amgs_destroy": ([_V], None),
     "amgs_last_error": ([_V], ctypes.c_char_p),
     "amgs_set_gauge": ([_V, _P], ctypes.c_int),
@@ -208,12 +210,21 @@ def solve_multi(solvers, b_ptrs, x_ptrs, nrhs=None):
 
 
 class Solver:
-    """One hierarchy on the GPU (one amgs context)."""
+    """One hierarchy on the GPU (one amgs context).  With m0s (a list of bare
+    masses) the context holds len(m0s) gauge configurations of one lattice
+    shape (amgs_create_multi); U is then [ncfg, nx*nt, 2] and every batch has
+    one right-hand side per configuration."""
 
-    def __init__(self, params, U=None):
+    def __init__(self, params, U=None, m0s=None):
         self.p = params
         h = _V()
-        rc = lib().amgs_create(ctypes.byref(params), ctypes.byref(h))
+        if m0s is None:
+            self.ncfg = 1
+            rc = lib().amgs_create(ctypes.byref(params), ctypes.byref(h))
+        else:
+            m = np.ascontiguousarray(m0s, dtype=np.float64)
+            self.ncfg = len(m)
+            rc = lib().amgs_create_multi(ctypes.byref(params), len(m), _p(m), ctypes.byref(h))
         if rc != AMGS_OK:
             raise AmgsError(f"amgs_create failed ({rc})")
         self.h = h

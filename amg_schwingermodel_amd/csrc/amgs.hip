@@ -240,8 +240,9 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
     const Geo& g1 = c->geo[1];
     const dim3 grid(g1.ns, nrhs);
     const int nv = g.ntest;
+    const size_t wcs = tvcs[0];
     if constexpr (std::is_same<R, float>::value) {
-        if (nv <= 12) {
+        if (nv <= 12 && wcs == 0) {
             const dim3 g2(g1.ns, cdiv_u(nrhs, 2));
             if (nv <= 8)
                 hipLaunchKernelGGL(k_restrict0_g<8>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs);
@@ -252,11 +253,11 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
         }
     }
     if (nv <= 8)
-        hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+        hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
     else if (nv <= 16)
-        hipLaunchKernelGGL((k_restrict0<R, 16>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+        hipLaunchKernelGGL((k_restrict0<R, 16>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
     else
-        hipLaunchKernelGGL((k_restrict0<R, 24>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns);
+        hipLaunchKernelGGL((k_restrict0<R, 24>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -318,12 +319,14 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     const unsigned char* mred = c->colmask.as<unsigned char>();
     const unsigned char* mblack = mred + g.nsb;
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
-                       (double*)nullptr, skip.as<int>(), mred);
+                       (double*)nullptr, skip.as<int>(), mred, ucs, mmv);
     const int nchunk = sap_chunks(g, nrhs, sap_wgs(nrhs)), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
     a.mm = mm;
     a.U = U;
+    a.ucs = ucs;
+    a.mmv = mmv;
     a.ncol = g.ncol;
     a.nchunk = (int)cdiv_u(g.sspb, cs);
     a.cs = cs;
@@ -414,10 +417,10 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
             // last sweep and is left out there
             if (colour == 0)
                 hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
-                                   (double*)nullptr, skip.as<int>(), mblack);
+                                   (double*)nullptr, skip.as<int>(), mblack, ucs, mmv);
             else if (it + 1 < nu)
                 hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
-                                   part_r.as<double>(), skip.as<int>(), (const unsigned char*)nullptr);
+                                   part_r.as<double>(), skip.as<int>(), (const unsigned char*)nullptr, ucs, mmv);
         }
         if (it + 1 < nu)
             hipLaunchKernelGGL(k_sap_check, dim3(nrhs), dim3(FT), 0, s, (const double*)part_r.as<double>(),
@@ -443,13 +446,16 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
     if (c->P.nu1 > 0 || !from_zero) {
         if (c->P.nu1 > 0) sap0(nrhs, eta, psi, c->P.nu1);
         hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, R(c->P.m0 + 2.0), U, eta, psi,
-                           res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr, nullptr);
+                           res0.as<cplx<R>>(), (double*)nullptr, (const int*)nullptr, nullptr, ucs, mmv);
         rr = res0.as<cplx<R>>();
     }
     restrict0(nrhs, tv[0], rr, eta1.as<cplx<R>>());
     // K-cycle: FGMRES_k_cycle[1] (src/amg.cpp:161); V-cycle: v_cycle(1) from zero (src/amg.cpp:110)
     coarse(c->P.cycle == 0 ? CM_CYCLE : CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
-    if constexpr (std::is_same<R, float>::value) {
+    if (tvcs[0] != 0) {  // one test-vector set per configuration
+        hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
+                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc, tvcs[0]);
+    } else if constexpr (std::is_same<R, float>::value) {
         const dim3 gp(ntile, cdiv_u(nrhs, 8));
         const int nv = g.ntest, dofc = c->geo[1].dof, nsc = c->geo[1].ns;
         const c32* pc = (const c32*)psi1.as<c32>();
@@ -469,13 +475,51 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
 // ---------------------------------------------------------------------------
 // hierarchy set-up (src/amg.cpp:3-46)
 // ---------------------------------------------------------------------------
-template <typename R> static void bind_engine(Ctx& c, Engine<R>& e, DevBuf& U, DevBuf* tv, DevBuf* links) {
+// multi: the configurations side by side (rhs r = configuration r); otherwise
+// one configuration at the buffers' start
+template <typename R>
+static void bind_engine(Ctx& c, Engine<R>& e, DevBuf& U, DevBuf* tv, DevBuf* links, bool multi = false) {
     e.U = U.as<cplx<R>>();
     for (int l = 0; l < c.L; l++) {
         e.tv[l] = tv[l].as<cplx<R>>();
         e.links[l] = links[l].as<cplx<R>>();
+        e.tvcs[l] = multi && l < c.L - 1 ? (size_t)c.geo[l].ntest * c.nvec(l) : 0;
+        e.lkcs[l] = multi && l >= 1 ? (size_t)5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof : 0;
     }
+    e.ucs = multi ? c.nvec(0) : 0;
+    if constexpr (std::is_same<R, double>::value) e.mmv = multi ? c.mm64.as<double>() : nullptr;
+    else e.mmv = multi ? c.mm32.as<float>() : nullptr;
 }
+
+// The FP64 hierarchy buffers of configuration i (of a multi-configuration
+// context) seen as a one-configuration context's for the set-up: the
+// buffers' pointers and the mass are switched for the object's life.
+struct CfgView {
+    Ctx& c;
+    void* U;
+    void* tv[MAXL];
+    void* lk[MAXL];
+    double m0;
+    CfgView(Ctx& cc, int i) : c(cc), U(cc.U64.p), m0(cc.P.m0) {
+        c.U64.p = (char*)U + sizeof(c64) * c.nvec(0) * i;
+        for (int l = 0; l < c.L; l++) {
+            tv[l] = c.tv64[l].p;
+            lk[l] = c.links64[l].p;
+            if (l < c.L - 1) c.tv64[l].p = (char*)tv[l] + sizeof(c64) * c.geo[l].ntest * c.nvec(l) * i;
+            if (l >= 1)
+                c.links64[l].p = (char*)lk[l] + sizeof(c64) * 5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof * i;
+        }
+        c.P.m0 = c.m0cfg[i];
+    }
+    ~CfgView() {
+        c.U64.p = U;
+        for (int l = 0; l < c.L; l++) {
+            c.tv64[l].p = tv[l];
+            c.links64[l].p = lk[l];
+        }
+        c.P.m0 = m0;
+    }
+};
 
 // column-major links [5][ns][D][D] (entry (c, r) at c*D + r) to column-pair
 // order [5][ns][D/2][D][2] (entry (c, r) at (c/2)*2D + 2r + c%2), D even
@@ -565,26 +609,34 @@ static void bootstrap(Ctx& c) {
     c.sync();
 }
 
-static void setup_common(Ctx& c, bool smooth) {
-    c.clear_graphs();
+// the FP64 part of the set-up of one configuration (after the raw test vectors)
+static void setup_fp64(Ctx& c, bool smooth) {
     bind_engine(c, c.e64, c.U64, c.tv64, c.links64);
     for (int l = 0; l < c.L - 1; l++) {
         if (smooth) smooth_level(c, l);
         ortho_links(c, l);
     }
     if (c.P.nit > 0) bootstrap(c);
+}
+
+// FP32 copies of every configuration's hierarchy, engine bindings
+static void setup_finish(Ctx& c) {
+    const bool multi = c.ncfg > 1;
+    const size_t nc = (size_t)c.ncfg;
+    bind_engine(c, c.e64, c.U64, c.tv64, c.links64, multi);
     if (c.mixed()) {
-        convert64to32(c, c.U64, c.U32, c.nvec(0));
+        convert64to32(c, c.U64, c.U32, nc * c.nvec(0));
         for (int l = 0; l < c.L; l++) {
-            if (l < c.L - 1) convert64to32(c, c.tv64[l], c.tv32[l], (size_t)c.geo[l].ntest * c.nvec(l));
-            if (l >= 1) convert64to32(c, c.links64[l], c.links32[l], (size_t)5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
+            if (l < c.L - 1) convert64to32(c, c.tv64[l], c.tv32[l], nc * c.geo[l].ntest * c.nvec(l));
+            if (l >= 1)
+                convert64to32(c, c.links64[l], c.links32[l], nc * 5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
         }
-        bind_engine(c, c.e32, c.U32, c.tv32, c.links32);
+        bind_engine(c, c.e32, c.U32, c.tv32, c.links32, multi);
         // levels with SAP smoothing: column-pair copies for the block solves
         for (int l = 1; l < c.L - 1; l++) {
             const Geo& g = c.geo[l];
             if (g.dof % 2) continue;
-            const size_t nl = (size_t)5 * g.ns * g.dof * g.dof;
+            const size_t nl = nc * 5 * g.ns * g.dof * g.dof;
             c.links2[l].alloc(sizeof(c32) * nl);
             hipLaunchKernelGGL(k_links_pairs, dim3(cdiv_u(nl, 256)), dim3(256), 0, c.stream, nl, g.dof,
                                (const c32*)c.links32[l].as<c32>(), c.links2[l].as<c32>());
@@ -596,7 +648,13 @@ static void setup_common(Ctx& c, bool smooth) {
     c.setup_done = true;
 }
 
-static void setup_random(Ctx& c) {
+static void setup_common(Ctx& c, bool smooth) {
+    c.clear_graphs();
+    setup_fp64(c, smooth);
+    setup_finish(c);
+}
+
+static void setup_random(Ctx& c, bool finish = true) {
     TvStream rng(c.P.tv_seed);
     std::vector<double> host;
     const Geo& g0 = c.geo[0];
@@ -622,7 +680,21 @@ static void setup_random(Ctx& c) {
         }
         c.sync();  // host buffer reuse
     }
-    setup_common(c, true);
+    c.clear_graphs();
+    setup_fp64(c, true);
+    if (finish) setup_finish(c);
+}
+
+// every configuration of a multi-configuration context: each one's set-up is
+// that of a context of its own (the same test-vector stream, its own field
+// and mass), then the FP32 copies of all of them
+static void setup_random_multi(Ctx& c) {
+    for (int i = 0; i < c.ncfg; i++) {
+        CfgView view(c, i);
+        setup_random(c, false);
+        c.sync();
+    }
+    setup_finish(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,6 +867,9 @@ static void outer_solve(Ctx& c, int nrhs) {
     const unsigned ntile = cdiv_u(g.ns, FT);
     hipStream_t s = c.stream;
     OuterState& st = c.ost;
+    // several configurations: rhs r's links and mass
+    st.ucs = c.ncfg > 1 ? n0 : 0;
+    st.mmv = c.ncfg > 1 ? c.mm64.as<double>() : nullptr;
     c64* V = c.oV.as<c64>();
     c64* Z = c.oZ.as<c64>();
     c64* x = c.ox.as<c64>();
@@ -891,7 +966,8 @@ static void true_relres(Ctx& c, int nrhs, double* out) {
     double* pb = pr + (size_t)nrhs * ntile;
     hipLaunchKernelGGL(k_resid<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
                        (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), (const c64*)c.ox.as<c64>(),
-                       c.ow.as<c64>(), pr, (const int*)nullptr, nullptr);
+                       c.ow.as<c64>(), pr, (const int*)nullptr, nullptr, c.ncfg > 1 ? c.nvec(0) : 0,
+                       c.ncfg > 1 ? c.mm64.as<double>() : nullptr);
     hipLaunchKernelGGL(k_norm_part<double>, dim3(ntile, nrhs), dim3(FT), 0, c.stream, g.ns,
                        (const c64*)c.ob.as<c64>(), pb);
     std::vector<double> hr((size_t)nrhs * ntile), hb((size_t)nrhs * ntile);
@@ -965,19 +1041,31 @@ static void need(bool cond, int code, const char* msg) {
     if (!cond) throw ApiError(code, msg);
 }
 
+// entry points that serve one gauge configuration per context
+static void single_config(const Ctx& c) {
+    need(c.ncfg == 1, AMGS_ERR_STATE, "not available on a multi-configuration context");
+}
+
 static void h2d(Ctx& c, DevBuf& dst, const double* src, size_t ncomplex) {
     dst.alloc(ncomplex * sizeof(c64));
     HIP_CHECK(hipMemcpyAsync(dst.p, src, ncomplex * sizeof(c64), hipMemcpyHostToDevice, c.stream));
 }
 
-extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
-    if (!p || !out) return AMGS_ERR_ARG;
+static int create_impl(const amgs_params* p, int ncfg, const double* m0, amgs_ctx** out) {
+    if (!p || !out || ncfg < 1 || (ncfg > 1 && !m0)) return AMGS_ERR_ARG;
     *out = nullptr;
     auto* ctx = new amgs_ctx();
     try {
         Ctx& c = ctx->c;
         c.P = *p;
         if (c.P.max_rhs < 1) c.P.max_rhs = 1;
+        c.ncfg = ncfg;
+        if (ncfg > 1) {
+            c.P.max_rhs = ncfg;  // one right-hand side per configuration
+            c.m0cfg.assign(m0, m0 + ncfg);
+            for (double v : c.m0cfg)
+                if (!std::isfinite(v)) throw ApiError(AMGS_ERR_NAN, "bare mass holds NaN/Inf");
+        }
         if (c.P.precision != AMGS_PREC_FP64 && c.P.precision != AMGS_PREC_MIXED && c.P.precision != AMGS_PREC_MIXED16)
             throw ApiError(AMGS_ERR_ARG, "precision must be AMGS_PREC_FP64, AMGS_PREC_MIXED or AMGS_PREC_MIXED16");
         build_geometry(c);
@@ -986,8 +1074,20 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
         HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&c.ev0));
         HIP_CHECK(hipEventCreate(&c.ev1));
-        const size_t n0 = c.nvec(0);
-        c.U64.alloc(sizeof(c64) * n0);
+        const size_t n0 = c.nvec(0), nc = (size_t)ncfg;
+        c.U64.alloc(sizeof(c64) * n0 * nc);
+        if (ncfg > 1) {
+            std::vector<double> m64(ncfg);
+            std::vector<float> m32(ncfg);
+            for (int i = 0; i < ncfg; i++) {
+                m64[i] = c.m0cfg[i] + 2.0;
+                m32[i] = float(c.m0cfg[i] + 2.0);
+            }
+            c.mm64.alloc(sizeof(double) * ncfg);
+            c.mm32.alloc(sizeof(float) * ncfg);
+            HIP_CHECK(hipMemcpy(c.mm64.p, m64.data(), sizeof(double) * ncfg, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(c.mm32.p, m32.data(), sizeof(float) * ncfg, hipMemcpyHostToDevice));
+        }
         c.cerr.alloc(sizeof(int));
         HIP_CHECK(hipMemsetAsync(c.cerr.p, 0, sizeof(int), c.stream));
         {
@@ -1005,8 +1105,8 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
             HIP_CHECK(hipMemcpyAsync(c.redl[l].p, c.red[l].data(), sizeof(int) * c.red[l].size(), hipMemcpyHostToDevice, c.stream));
             HIP_CHECK(hipMemcpyAsync(c.blackl[l].p, c.black[l].data(), sizeof(int) * c.black[l].size(),
                                      hipMemcpyHostToDevice, c.stream));
-            if (l < c.L - 1) c.tv64[l].alloc(sizeof(c64) * c.geo[l].ntest * c.nvec(l));
-            if (l >= 1) c.links64[l].alloc(sizeof(c64) * 5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof);
+            if (l < c.L - 1) c.tv64[l].alloc(sizeof(c64) * c.geo[l].ntest * c.nvec(l) * nc);
+            if (l >= 1) c.links64[l].alloc(sizeof(c64) * 5 * c.geo[l].ns * c.geo[l].dof * c.geo[l].dof * nc);
         }
         int ntest_max = 1;
         for (int l = 0; l < c.L - 1; l++) ntest_max = std::max(ntest_max, c.geo[l].ntest);
@@ -1028,6 +1128,14 @@ extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) {
     *out = ctx;
     return AMGS_OK;
 }
+
+extern "C" int amgs_create(const amgs_params* p, amgs_ctx** out) { return create_impl(p, 1, nullptr, out); }
+
+extern "C" int amgs_create_multi(const amgs_params* p, int ncfg, const double* m0, amgs_ctx** out) {
+    return create_impl(p, ncfg, m0, out);
+}
+
+extern "C" int amgs_num_configs(const amgs_ctx* ctx) { return ctx ? ctx->c.ncfg : 0; }
 
 extern "C" void amgs_destroy(amgs_ctx* ctx) {
     if (!ctx) return;
@@ -1051,7 +1159,7 @@ extern "C" int amgs_set_gauge(amgs_ctx* ctx, const double* U) {
     if (!ctx || !U) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
-    const size_t n = c.nvec(0);
+    const size_t n = c.nvec(0) * c.ncfg;  // [ncfg][ns][2]
     for (size_t i = 0; i < 2 * n; i++) need(std::isfinite(U[i]), AMGS_ERR_NAN, "gauge field holds NaN/Inf");
     h2d(c, c.U64, U, n);
     c.sync();
@@ -1067,7 +1175,8 @@ extern "C" int amgs_setup(amgs_ctx* ctx) {
     Ctx& c = ctx->c;
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     auto t0 = std::chrono::steady_clock::now();
-    setup_random(c);
+    if (c.ncfg > 1) setup_random_multi(c);
+    else setup_random(c);
     c.last_ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return AMGS_OK;
     API_END
@@ -1078,6 +1187,7 @@ extern "C" int amgs_setup_from_tv(amgs_ctx* ctx, const double* const* tv, int sm
     API_BEGIN
     Ctx& c = ctx->c;
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
+    single_config(c);
     for (int l = 0; l < c.L - 1; l++) {
         need(tv[l] != nullptr, AMGS_ERR_ARG, "missing test vectors");
         HIP_CHECK(hipMemcpyAsync(c.tv64[l].p, tv[l], sizeof(c64) * c.geo[l].ntest * c.nvec(l), hipMemcpyHostToDevice,
@@ -1103,6 +1213,7 @@ static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist
     need(c.setup_done || c.outer_prec != 0, AMGS_ERR_STATE, "amgs_setup first");
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
+    need(c.ncfg == 1 || nrhs == c.ncfg, AMGS_ERR_ARG, "a multi-configuration context solves one rhs per configuration");
     auto t0 = std::chrono::steady_clock::now();
     outer_solve(c, nrhs);
     c.sync();
@@ -1521,6 +1632,7 @@ static void cg_impl(Ctx& c, int nrhs, int max_iter, double tol, int* iters) {
 }
 
 static void krylov_prologue(Ctx& c, int nrhs) {
+    single_config(c);
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(nrhs >= 1 && nrhs <= std::min(c.P.max_rhs, KMAXB), AMGS_ERR_ARG, "nrhs must be in [1, min(max_rhs, 32)]");
 }
@@ -1591,6 +1703,7 @@ extern "C" int amgs_cg_normal(amgs_ctx* ctx, int nrhs, const double* phi, double
 static int solve_with(amgs_ctx* ctx, int prec, int nrhs, const double* b, double* x, int* iters, double* relres,
                       double* hist, int hist_cap, int* nhist) {
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(nrhs >= 1 && nrhs <= c.P.max_rhs && (prec != 1 || nrhs <= c.e64.cap), AMGS_ERR_ARG, "bad nrhs");
     const size_t n0 = c.nvec(0);
@@ -1638,6 +1751,7 @@ extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, int form, const double* phi, 
     API_BEGIN
     Ctx& c = ctx->c;
     need(nrhs >= 1 && nrhs <= std::min(c.P.max_rhs, KMAXB), AMGS_ERR_ARG, "nrhs must be in [1, min(max_rhs, 32)]");
+    need(c.ncfg == 1 || nrhs == c.ncfg, AMGS_ERR_ARG, "a multi-configuration context takes one phi per configuration");
     const Geo& g = c.geo[0];
     const size_t tot = c.nvec(0) * nrhs, nsr = (size_t)g.ns * nrhs;
     DevBuf Yb, Fb;
@@ -1657,7 +1771,8 @@ extern "C" int amgs_force(amgs_ctx* ctx, int nrhs, int form, const double* phi, 
     hipLaunchKernelGGL(k_sigma3, dim3(cdiv_u(nsr, 256)), dim3(256), 0, c.stream, nsr, (const c64*)c.ox.as<c64>(),
                        c.ox.as<c64>());
     hipLaunchKernelGGL(k_force, dim3(cdiv_u(g.ns, FT), nrhs), dim3(FT), 0, c.stream, g, (const c64*)c.U64.as<c64>(),
-                       (const c64*)c.ox.as<c64>(), (const c64*)Yb.as<c64>(), Fb.as<double>());
+                       (const c64*)c.ox.as<c64>(), (const c64*)Yb.as<c64>(), Fb.as<double>(),
+                       c.ncfg > 1 ? c.nvec(0) : 0);
     KryRun K(c, nrhs);
     K.vec(KOP_DOT, Yb.as<c64>(), Yb.as<c64>(), nullptr, nullptr, nullptr, nullptr, true);
     K.fold();
@@ -1707,6 +1822,7 @@ extern "C" int amgs_apply_D_batched_dev(amgs_ctx* ctx, int level, int nrhs, int 
     if (!ctx || !in || !out) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "coarse operators need amgs_setup");
     need(level >= 1 && level < c.L, AMGS_ERR_ARG, "level must be a coarse level (1 .. levels-1)");
     need(nrhs >= 1, AMGS_ERR_ARG, "nrhs must be >= 1");
@@ -1726,6 +1842,7 @@ extern "C" int amgs_autotune_batched_D(amgs_ctx* ctx, int level, int nrhs, int p
     if (!ctx) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(level >= 1 && level < c.L && nrhs >= 1, AMGS_ERR_ARG, "bad level / nrhs");
     need(precision == AMGS_PREC_FP64 || c.mixed(), AMGS_ERR_STATE, "FP32 hierarchy needs a mixed-precision context");
@@ -1749,16 +1866,20 @@ extern "C" int amgs_apply_D_dev(amgs_ctx* ctx, int level, int dagger, int nrhs, 
     const Geo& g = c.geo[level];
     if (level == 0) {
         const dim3 grid(cdiv_u(g.ns, FT), nrhs);
+        const size_t ucs = c.ncfg > 1 ? c.nvec(0) : 0;
+        const double* mmv = c.ncfg > 1 ? c.mm64.as<double>() : nullptr;
+        need(c.ncfg == 1 || nrhs == c.ncfg, AMGS_ERR_ARG, "one right-hand side per configuration");
         if (dagger)
             hipLaunchKernelGGL((k_wilson<double, true>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
-                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out);
+                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out, ucs, mmv);
         else
             hipLaunchKernelGGL((k_wilson<double, false>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
-                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out);
+                               (const c64*)c.U64.as<c64>(), (const c64*)in, (c64*)out, ucs, mmv);
         HIP_CHECK(hipGetLastError());
     } else {
         need(!dagger, AMGS_ERR_ARG, "D^dagger only on the fine level");
         need(c.setup_done, AMGS_ERR_STATE, "coarse operators need amgs_setup");
+        single_config(c);
         // all right-hand sides at once as matrix-core block-GEMMs
         cdop_mfma(c, c.e64, level, nrhs, (const c64*)in, (c64*)out);
     }
@@ -1797,6 +1918,7 @@ extern "C" int amgs_apply_P(amgs_ctx* ctx, int level, int nrhs, const double* co
     if (!ctx || !coarse || !fine) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
     const size_t nf = c.nvec(level), nc = c.nvec(level + 1);
@@ -1817,6 +1939,7 @@ extern "C" int amgs_apply_Pt(amgs_ctx* ctx, int level, int nrhs, const double* f
     if (!ctx || !coarse || !fine) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
     const size_t nf = c.nvec(level), nc = c.nvec(level + 1);
@@ -1831,6 +1954,7 @@ extern "C" int amgs_sap(amgs_ctx* ctx, int level, int nrhs, const double* v, dou
     if (!ctx || !v || !x) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(level >= 0 && level < c.L - 1 && nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad level / nrhs");
     need(level == 0 || c.setup_done, AMGS_ERR_STATE, "coarse SAP needs amgs_setup");
@@ -1852,6 +1976,7 @@ extern "C" int amgs_sap_dev_noalloc(amgs_ctx* ctx, int level, int nrhs, const vo
     if (!ctx || !v_dev || !x_dev) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "coarse SAP needs amgs_setup");
     need(level >= 1 && level < c.L - 1, AMGS_ERR_ARG, "level must be a coarse level with SAP smoothing (1 .. levels-2)");
     need(nrhs >= 1, AMGS_ERR_ARG, "nrhs must be >= 1");
@@ -1881,6 +2006,7 @@ extern "C" int amgs_kcycle(amgs_ctx* ctx, int nrhs, const double* eta, double* p
     if (!ctx || !eta || !psi) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(nrhs >= 1 && nrhs <= c.e64.cap, AMGS_ERR_ARG, "bad nrhs");
     const size_t n = c.nvec(0) * nrhs;
@@ -1896,6 +2022,7 @@ extern "C" int amgs_multigrid(amgs_ctx* ctx, int nrhs, const double* b, double* 
     if (!ctx || !b || !x || !cycles || !relres || max_cycles < 1) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(nrhs >= 1 && nrhs <= c.e64.cap && nrhs <= c.ocap, AMGS_ERR_ARG, "bad nrhs");
     const Geo& g = c.geo[0];
@@ -1956,6 +2083,7 @@ extern "C" int amgs_get_tv(amgs_ctx* ctx, int level, int i, double* out) {
     if (!ctx || !out) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(level >= 0 && level < c.L - 1 && i >= 0 && i < c.geo[level].ntest, AMGS_ERR_ARG, "bad level / index");
     const size_t n = c.nvec(level);
@@ -1968,6 +2096,7 @@ extern "C" int amgs_get_links(amgs_ctx* ctx, int level, int which, int mu, doubl
     if (!ctx || !out) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
     need(level >= 1 && level < c.L && which >= 0 && which <= 2 && mu >= 0 && mu <= 1, AMGS_ERR_ARG, "bad arguments");
     const Geo& g = c.geo[level];
@@ -1992,6 +2121,7 @@ extern "C" int amgs_time_kernel(amgs_ctx* ctx, int kernel, int nrhs, int reps, d
     if (!ctx || !avg_ms || reps < 1) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
+    single_config(c);
     need(c.gauge_set, AMGS_ERR_STATE, "amgs_set_gauge first");
     need(nrhs >= 1 && nrhs <= c.ocap, AMGS_ERR_ARG, "nrhs exceeds capacity");
     const Geo& g = c.geo[0];
@@ -1999,11 +2129,13 @@ extern "C" int amgs_time_kernel(amgs_ctx* ctx, int kernel, int nrhs, int reps, d
     auto launch = [&]() {
         if (kernel == 0)
             hipLaunchKernelGGL((k_wilson<double, false>), grid, dim3(FT), 0, c.stream, g, c.P.m0 + 2.0,
-                               (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), c.ow.as<c64>());
+                               (const c64*)c.U64.as<c64>(), (const c64*)c.ob.as<c64>(), c.ow.as<c64>(), (size_t)0,
+                               (const double*)nullptr);
         else if (kernel == 1) {
             need(c.mixed() && c.setup_done, AMGS_ERR_STATE, "FP32 operator needs mixed setup");
             hipLaunchKernelGGL((k_wilson<float, false>), grid, dim3(FT), 0, c.stream, g, float(c.P.m0 + 2.0),
-                               (const c32*)c.U32.as<c32>(), (const c32*)c.oeta32.as<c32>(), c.opsi32.as<c32>());
+                               (const c32*)c.U32.as<c32>(), (const c32*)c.oeta32.as<c32>(), c.opsi32.as<c32>(),
+                               (size_t)0, (const float*)nullptr);
         } else
             throw ApiError(AMGS_ERR_ARG, "unknown kernel id");
     };

@@ -65,6 +65,10 @@ template <typename R> struct CArgs {
     // counter bar[rhs], zero at launch).  x_top: workspace offset of the
     // member's private copy of the solve's output.
     int grp;
+    // several gauge configurations (k_coarse_cfg): configuration r's links and
+    // test vectors at lcs[l], wcs[l] entries past the first's, m0 + 2 = mmv[r]
+    size_t lcs[MAXL], wcs[MAXL];
+    const R* mmv;
     unsigned* bar;
     cplx<R>* xchg;
     size_t xchg_n, x_top;
@@ -1209,11 +1213,7 @@ template <typename R> __device__ __forceinline__ void c_run(const CArgs<R>& A, C
 // The arguments travel by value in the kernel-argument segment (no per-launch
 // copy, capturable in a HIP graph) and are read in place through the segment
 // pointer (scalar loads; a copy of the parameter would land in scratch).
-template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R> Aval) {
-    (void)Aval;
-    const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
-    __shared__ CLds S;
-    extern __shared__ __align__(16) char dyn_lds[];
+template <typename R> __device__ __forceinline__ void coarse_main(const CArgs<R>& A, CLds& S, char* dyn_lds) {
     const int rhs = blockIdx.x / A.grp, member = blockIdx.x - rhs * A.grp;
     CK<R> K{A, A.ws + (size_t)blockIdx.x * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
             A.prof && member == 0 ? A.prof + (size_t)rhs * 32 : nullptr, rhs, member};
@@ -1261,6 +1261,40 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs
         for (int e = threadIdx.x; e < n; e += CT) out[e] = xo[e];
     }
     Ttot.lap(31);
+}
+
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R> Aval) {
+    (void)Aval;
+    const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
+    __shared__ CLds S;
+    extern __shared__ __align__(16) char dyn_lds[];
+    coarse_main(A, S, dyn_lds);
+}
+
+// Several gauge configurations in one launch (right-hand side r belongs to
+// configuration r): each workgroup copies the arguments to LDS with its
+// configuration's coarse links, test vectors and mass term in place.
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse_cfg(const CArgs<R> Aval) {
+    (void)Aval;
+    const CArgs<R>& Ak = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
+    __shared__ CLds S;
+    __shared__ CArgs<R> Ac;
+    extern __shared__ __align__(16) char dyn_lds[];
+    static_assert(sizeof(CArgs<R>) % 4 == 0, "argument copy by words");
+    for (int i = threadIdx.x; i < (int)(sizeof(CArgs<R>) / 4); i += CT)
+        reinterpret_cast<int*>(&Ac)[i] = reinterpret_cast<const int*>(&Ak)[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int cfg = blockIdx.x / Ak.grp;
+        for (int l = 1; l < MAXL; l++) {
+            if (Ac.lv[l].links) Ac.lv[l].links += cfg * Ak.lcs[l];
+            if (Ac.lv[l].links2) Ac.lv[l].links2 += cfg * Ak.lcs[l];
+            if (Ac.lv[l].w) Ac.lv[l].w += cfg * Ak.wcs[l];
+        }
+        Ac.mm = Ak.mmv[cfg];
+    }
+    __syncthreads();
+    coarse_main(Ac, S, dyn_lds);
 }
 
 }  // namespace amgs

@@ -17,14 +17,19 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
         a.lv[l].links = links[l];
         a.lv[l].links2 = links2[l];
         a.lv[l].w = l < c->L - 1 ? tv[l] : nullptr;
+        a.lcs[l] = lkcs[l];
+        a.wcs[l] = tvcs[l];
     }
+    a.mmv = mmv;
+    // one workgroup group per configuration: the arguments go through LDS
+    const bool cfg = mmv != nullptr;
     a.mode = mode;
     a.level = level;
     a.nu = nu;
     // coarsest GMRES vectors in dynamic LDS if they fit beside the static part
     const size_t ncrs = c->nvec(c->L - 1);
     const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
-    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024;
+    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024 - (cfg ? sizeof(CArgs<R>) : 0);
     a.crs_lds = dyn <= lds_cap ? 1 : 0;
     size_t dyn_bytes = a.crs_lds ? dyn : 0;
     // SAP block vectors of a level share that region (never live at the same time)
@@ -59,15 +64,20 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
         }
     }
     if (dyn_bytes > 0) {
-        static bool attr_set[2] = {false, false};
-        bool& done = attr_set[sizeof(R) == 8];
+        static bool attr_set[2][2] = {{false, false}, {false, false}};
+        bool& done = attr_set[sizeof(R) == 8][cfg];
         if (!done) {
-            HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)(160 * 1024 - sizeof(CLds))));
+            if (cfg)
+                HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse_cfg<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)(160 * 1024 - sizeof(CLds) - sizeof(CArgs<R>) - 1024)));
+            else
+                HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)(160 * 1024 - sizeof(CLds))));
             done = true;
         }
     }
-    hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+    if (cfg) hipLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+    else hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -131,6 +131,11 @@ template <typename R> struct Engine {
     const cplx<R>* tv[MAXL] = {};
     const cplx<R>* links[MAXL] = {};
     const cplx<R>* links2[MAXL] = {};  // column-pair copies for the coarse SAP block stencil (FP32 engine)
+    // several gauge configurations (rhs r = configuration r): entries between
+    // consecutive configurations' links U, test vectors tv[l] and coarse links
+    // links[l] (0 with one configuration), and m0 + 2 per configuration
+    size_t ucs = 0, tvcs[MAXL] = {}, lkcs[MAXL] = {};
+    const R* mmv = nullptr;
     // fine SAP workspace
     DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero, rsig;
     SapState st{};
@@ -193,6 +198,11 @@ struct Ctx {
     // outer FGMRES
     DevBuf oV, oZ, oZ32, ox, ob, or_, ow, ost_mem, oeta32, opsi32, rel_part;
     DevBuf cerr;  // raised by a coarse-kernel member barrier that timed out
+    // several gauge configurations of one lattice shape (amgs_create_multi):
+    // ncfg hierarchies side by side, right-hand side r solved with configuration r
+    int ncfg = 1;
+    std::vector<double> m0cfg;
+    DevBuf mm64, mm32;  // m0 + 2 per configuration
     OuterState ost{};
     int ocap = 0;
     // io staging

@@ -195,15 +195,21 @@ template <typename R, typename In> struct BlockLocalIn {
 };
 
 // out = D in  (or D^dagger), all sites, all right-hand sides
+// Several gauge configurations in one batch: right-hand side r uses the links
+// at U + r * ucs and the mass term mmv[r] (ucs = 0 and mmv = null for one).
+template <typename R>
+__device__ __forceinline__ R cfg_mm(R mm, const R* __restrict__ mmv, int rhs) { return mmv ? mmv[rhs] : mm; }
+
 template <typename R, bool DAG>
 __global__ void __launch_bounds__(FT) k_wilson(Geo g, R mm, const cplx<R>* __restrict__ U,
-                                               const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out) {
+                                               const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                               size_t ucs, const R* __restrict__ mmv) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int n = tile * FT + threadIdx.x;
     if (n >= g.ns) return;
     const size_t off = (size_t)blockIdx.y * g.ns * 2;
     cplx<R> o0, o1;
-    wilson_site<R, DAG>(U, in + off, n, g.nx, g.nt, mm, AllSites{}, o0, o1);
+    wilson_site<R, DAG>(U + blockIdx.y * ucs, in + off, n, g.nx, g.nt, cfg_mm(mm, mmv, blockIdx.y), AllSites{}, o0, o1);
     out[off + 2 * n] = o0;
     out[off + 2 * n + 1] = o1;
 }
@@ -217,10 +223,13 @@ __global__ void __launch_bounds__(FT) k_resid(Geo g, R mm, const cplx<R>* __rest
                                               const cplx<R>* __restrict__ v, const cplx<R>* __restrict__ x,
                                               cplx<R>* __restrict__ r, double* __restrict__ part,
                                               const int* __restrict__ skip,
-                                              const unsigned char* __restrict__ want) {
+                                              const unsigned char* __restrict__ want, size_t ucs = 0,
+                                              const R* __restrict__ mmv = nullptr) {
     __shared__ c64 red[4];
     const int rhs = blockIdx.y;
     if (skip && skip[rhs]) return;
+    U += rhs * ucs;
+    mm = cfg_mm(mm, mmv, rhs);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int n = tile * FT + threadIdx.x;
     const size_t off = (size_t)rhs * g.ns * 2;
@@ -272,6 +281,10 @@ template <typename R> struct SapArgs {
     Geo g;
     R mm;
     const cplx<R>* U;
+    // several gauge configurations in one batch: rhs r uses the links at
+    // U + r * ucs and the mass term mmv[r] (ucs = 0, mmv = null: one configuration)
+    size_t ucs;
+    const R* mmv;
     const int* blocks;  // block ids of this colour [ncol]
     int ncol, nchunk, cs, nrhs, m;  // cs: sites per chunk (a multiple of FT)
     double tol;
@@ -290,6 +303,7 @@ template <typename R> struct SapArgs {
 struct ItemIdx {
     int item, chunk, rhs, blk, m_lo, m_hi;
     int x0, t0;  // lattice coordinates of the block's first site
+    double mm;   // m0 + 2 of the item's configuration
 };
 
 template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R>& a) {
@@ -304,6 +318,7 @@ template <typename R> __device__ __forceinline__ ItemIdx item_of(const SapArgs<R
     t.t0 = (t.blk - X * a.g.sbt) * a.g.stel;
     t.m_lo = t.chunk * a.cs;
     t.m_hi = min(a.g.sspb, t.m_lo + a.cs);
+    t.mm = a.mmv ? (double)a.mmv[t.rhs] : (double)a.mm;
     return t;
 }
 
@@ -332,6 +347,8 @@ template <bool SQ, typename R, typename In>
 __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx& t, const BSite& b, In in, cplx<R>& o0,
                                               cplx<R>& o1, cplx<R>& c0, cplx<R>& c1) {
     const Geo& g = a.g;
+    const cplx<R>* U = a.U + (size_t)t.rhs * a.ucs;
+    const R mm = R(t.mm);
     if constexpr (SQ) {
         const bool tr = b.t + 1 == g.nt, tl = b.t == 0;
         const int r0 = tr ? b.n - b.t : b.n + 1;
@@ -339,10 +356,10 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
         const int r1 = b.x + 1 == g.nx ? b.t : b.n + g.nt;
         const int l1 = b.x == 0 ? b.n + (g.nx - 1) * g.nt : b.n - g.nt;
         const bool wt = g.stel == g.nt, wx = g.sxel == g.nx;  // block spans the extent
-        wilson_core<R, false>(a.U, in, b.n, r0, r1, l0, l1, wt || b.mt + 1 < g.stel, wx || b.mx + 1 < g.sxel,
-                              wt || b.mt > 0, wx || b.mx > 0, tr, tl, a.mm, o0, o1, c0, c1);
+        wilson_core<R, false>(U, in, b.n, r0, r1, l0, l1, wt || b.mt + 1 < g.stel, wx || b.mx + 1 < g.sxel,
+                              wt || b.mt > 0, wx || b.mx > 0, tr, tl, mm, o0, o1, c0, c1);
     } else {
-        wilson_site_in<R, false>(a.U, BlockLocalIn<R, In>{in, g, t.blk, b.n}, b.n, g.nx, g.nt, a.mm,
+        wilson_site_in<R, false>(U, BlockLocalIn<R, In>{in, g, t.blk, b.n}, b.n, g.nx, g.nt, mm,
                                  InBlock{g, t.blk}, o0, o1);
         in(b.n, c0, c1);
     }
@@ -362,9 +379,11 @@ __device__ __forceinline__ void block_stencil(const SapArgs<R>& a, const ItemIdx
 // of two single-site stencils, with the same arithmetic per site.
 // o[s][spin] are the outputs, c[s][spin] the pair's own input values.
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ void block_stencil_pair(const SapArgs<float>& a, const BSite& b,
+__device__ __forceinline__ void block_stencil_pair(const SapArgs<float>& a, const ItemIdx& t, const BSite& b,
                                                    const f16x4* __restrict__ in, c32 (&o)[2][2], c32 (&c)[2][2]) {
     const Geo& g = a.g;
+    const c32* Ut = a.U + (size_t)t.rhs * a.ucs;
+    const float mm = float(t.mm);
     const int n = b.n;
     const bool tl = b.t == 0, tr1 = b.t + 2 == g.nt;  // wraps: site n to the left, site n+1 to the right
     const bool ml0 = b.mt > 0, mr1t = b.mt + 2 < g.stel;         // outer t-neighbours in the block
@@ -379,10 +398,10 @@ __device__ __forceinline__ void block_stencil_pair(const SapArgs<float>& a, cons
     const f16x8 lp = *reinterpret_cast<const f16x8*>(in + (unsigned)(mx_l ? xl : n));
     const f16x4 tL = in[(unsigned)(ml0 ? l0 : n)], tR = in[(unsigned)(mr1t ? r0b : n)];
     using F4 = float __attribute__((ext_vector_type(4)));
-    const F4* U4 = reinterpret_cast<const F4*>(a.U);
+    const F4* U4 = reinterpret_cast<const F4*>(Ut);
     const F4 ua = U4[(unsigned)n], ub = U4[(unsigned)n + 1u];  // f0, f1 of n; f0, f1 of n+1
-    const c32 ul0 = a.U[2u * (unsigned)l0];
-    const c32 ux0 = a.U[2u * (unsigned)xl + 1u], ux1 = a.U[2u * (unsigned)xl + 3u];
+    const c32 ul0 = Ut[2u * (unsigned)l0];
+    const c32 ux0 = Ut[2u * (unsigned)xl + 1u], ux1 = Ut[2u * (unsigned)xl + 3u];
     const f16x4 h0 = cp.lo, h1 = cp.hi, r0v = rp.lo, r1v = rp.hi, l0v = lp.lo, l1v = lp.hi;
     c[0][0] = c32{float(h0.x), float(h0.y)};
     c[0][1] = c32{float(h0.z), float(h0.w)};
@@ -396,9 +415,9 @@ __device__ __forceinline__ void block_stencil_pair(const SapArgs<float>& a, cons
     const H16Proj P;
     // site n: right t = n+1, left t = tL; site n+1: right t = tR, left t = n
     wilson_combine<float, false>(c[0][0], c[0][1], P.sum(h1, true), P.sumi(r0v, false), P.sum(tL, false),
-                                 P.sumi(l0v, true), f00, f01, b00, b01, true, mx_r, ml0, mx_l, a.mm, o[0][0], o[0][1]);
+                                 P.sumi(l0v, true), f00, f01, b00, b01, true, mx_r, ml0, mx_l, mm, o[0][0], o[0][1]);
     wilson_combine<float, false>(c[1][0], c[1][1], P.sum(tR, true), P.sumi(r1v, false), P.sum(h0, false),
-                                 P.sumi(l1v, true), f10, f11, b10, b11, mr1t, mx_r, true, mx_l, a.mm, o[1][0], o[1][1]);
+                                 P.sumi(l1v, true), f10, f11, b10, b11, mr1t, mx_r, true, mx_l, mm, o[1][0], o[1][1]);
 }
 
 template <typename R> __device__ __forceinline__ cplx<R>* slot(const SapArgs<R>& a, int j, int rhs) {
@@ -653,7 +672,7 @@ __global__ void __launch_bounds__(FT, (H && SQ && J <= 3 ? 8 : 1)) k_sap_arnoldi
                 const BSite b = bsite(a, t, m);
                 const unsigned n = (unsigned)b.n;
                 c32 o[2][2], cc[2][2];
-                block_stencil_pair(a, b, vin, o, cc);
+                block_stencil_pair(a, t, b, vin, o, cc);
                 const f16x4 w0 = to_h16(o[0][0], o[0][1]), w1 = to_h16(o[1][0], o[1][1]);
                 *reinterpret_cast<f16x8*>(reinterpret_cast<f16x4*>(w) + n) =
                     f16x8{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
@@ -1013,9 +1032,10 @@ __global__ void __launch_bounds__(FT) k_norm_part(int ns, const cplx<R>* __restr
 template <typename R, int NV>
 __global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* __restrict__ w,
                                                   const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
-                                                  int dofc, int nsc) {
+                                                  int dofc, int nsc, size_t wcs = 0) {
     __shared__ cplx<R> red[4 * 2 * NV];
     const int X = blockIdx.x, rhs = blockIdx.y;
+    w += rhs * wcs;
     const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
     cplx<R> acc[2 * NV];
 #pragma unroll
@@ -1050,10 +1070,11 @@ __global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* 
 template <typename R>
 __global__ void __launch_bounds__(FT) k_prolong0(Geo g, int nv, const cplx<R>* __restrict__ w,
                                                  const cplx<R>* __restrict__ vc, cplx<R>* __restrict__ out,
-                                                 int dofc, int nsc, int accumulate) {
+                                                 int dofc, int nsc, int accumulate, size_t wcs = 0) {
     const int n = blockIdx.x * FT + threadIdx.x;
     if (n >= g.ns) return;
     const int rhs = blockIdx.y;
+    w += rhs * wcs;
     const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
     const cplx<R>* c = vc + (size_t)rhs * nsc * dofc + (size_t)agg_block_of(g, n) * dofc;
     cplx<R> o0 = czero<R>(), o1 = czero<R>();
@@ -1176,10 +1197,11 @@ __global__ void k_sigma3(size_t nsites_total, const c64* in, c64* out) {
 // dU/dtheta = iU, i.e. Re[i s U_mu(n) X(n)^+(1-s_mu)Y(n+mu) - i s U_mu(n)^* X(n+mu)^+(1+s_mu)Y(n)]
 // (s = -1 on the antiperiodic time hop).  F: [nrhs][ns][2] real.
 __global__ void __launch_bounds__(FT) k_force(Geo g, const c64* __restrict__ U, const c64* __restrict__ X,
-                                              const c64* __restrict__ Y, double* __restrict__ F) {
+                                              const c64* __restrict__ Y, double* __restrict__ F, size_t ucs) {
     const int n = blockIdx.x * FT + threadIdx.x;
     if (n >= g.ns) return;
     const size_t off = (size_t)blockIdx.y * g.ns * 2;
+    U += blockIdx.y * ucs;
     const c64* x = X + off;
     const c64* y = Y + off;
     const c64 xn0 = x[2 * n], xn1 = x[2 * n + 1], yn0 = y[2 * n], yn1 = y[2 * n + 1];

@@ -35,6 +35,9 @@ struct OuterState {
     double* qr;  // [rhs][ntile][2]
     int M, ntile, hcap;
     int ntile_a;  // workgroups per rhs of the Arnoldi step A (OUT_SPT tiles of FT sites each)
+    // several gauge configurations: rhs r's links at U + r * ucs, m0 + 2 = mmv[r]
+    size_t ucs;
+    const double* mmv;
 };
 
 }  // namespace amgs

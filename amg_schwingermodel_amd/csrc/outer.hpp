@@ -14,6 +14,8 @@ __global__ void __launch_bounds__(FT) k_outer_resid(Geo g, double mm, const c64*
     __shared__ c64 red[8];
     const int rhs = blockIdx.y;
     if (st.done[rhs]) return;
+    U += rhs * st.ucs;
+    mm = cfg_mm(mm, st.mmv, rhs);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int n = tile * FT + threadIdx.x;
     const size_t off = (size_t)rhs * g.ns * 2;
@@ -113,6 +115,8 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
     __shared__ c64 red[(FT / 64) * NQ];
     const int rhs = blockIdx.y;
     if (!st.active[rhs]) return;
+    U += rhs * st.ucs;
+    mm = cfg_mm(mm, st.mmv, rhs);
     // OUT_SPT consecutive tiles of FT sites per workgroup: the 2j+1 block
     // reductions are shared by OUT_SPT sites per thread
     const int tile = xcd_tile(blockIdx.x, gridDim.x);

@@ -86,10 +86,23 @@ int amgs_read_parameters(const char* path, amgs_params* p);
 /* Replaces the FGMRES_AMG constructor's allocation part (include/amg.h:101-104,
  * include/level.h:91-134).  Validates the blocking; allocates device memory. */
 int amgs_create(const amgs_params* p, amgs_ctx** out);
+/* Several gauge configurations of one lattice shape and blocking in one
+ * context (new; the reference solves one configuration per process,
+ * src/main.cpp:82-89): ncfg configurations, each with its own bare mass m0[i],
+ * gauge field and multigrid hierarchy (set up as a context of its own would
+ * be), share every kernel launch.  Right-hand side i of a batch belongs to
+ * configuration i: solves and forces take exactly ncfg right-hand sides
+ * (p->max_rhs is set to ncfg), amgs_set_gauge takes U [ncfg][nx*nt][2], and
+ * amgs_apply_D at level 0 applies configuration i's D to vector i.  The
+ * component entry points of single hierarchies (coarse levels, P, SAP, the
+ * comparison solvers) return AMGS_ERR_STATE on such a context. */
+int amgs_create_multi(const amgs_params* p, int ncfg, const double* m0, amgs_ctx** out);
+int amgs_num_configs(const amgs_ctx* c);
 void amgs_destroy(amgs_ctx* c);
 const char* amgs_last_error(const amgs_ctx* c);
 
-/* GaugeConf::read_conf / setGconf (src/gauge_conf.cpp:18-32): U [nx*nt][2]. */
+/* GaugeConf::read_conf / setGconf (src/gauge_conf.cpp:18-32): U [nx*nt][2]
+ * ([ncfg][nx*nt][2] on a multi-configuration context). */
 int amgs_set_gauge(amgs_ctx* c, const double* U);
 
 /* AlgebraicMG::setUpPhase(Nit = 0) (src/amg.cpp:3-46): test vectors from
