@@ -18,7 +18,6 @@ i solved with configuration i's links, mass and multigrid hierarchy.
   multi-configuration batch.
 """
 import os
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -74,3 +73,124 @@ def test_members_equal_their_own_contexts(prec):
     with pytest.raises(amgs.AmgsError):
         sm.apply_P(0, np.zeros(sm.size(1), np.complex128))
     sm.close()
+
+
+@pytest.fixture(scope="module")
+def uneven():
+    from amg_schwingermodel_amd import workloads
+
+    return workloads.uneven_batch(log=lambda m: print(m, flush=True))
+
+
+@pytest.mark.timeout(900)
+def test_uneven_batch_in_one_call(uneven):
+    """48 configurations of three lattice sizes, one multi-configuration
+    context per size, solved by one amgs_solve_multi call (mixed precision,
+    FP16 SAP basis, FGMRES(20) with 10 restarts)."""
+    import torch
+
+    from amg_schwingermodel_amd import workloads
+
+    groups = uneven
+    assert sum(len(g["members"]) for g in groups) == 48 and len(groups) == 3
+    solvers, bd, xd = [], [], []
+    for g in groups:
+        p = workloads.uneven_params(g["n"], precision=amgs.PREC_MIXED16, out_restarts=10)
+        s = amgs.Solver(p, g["U"], m0s=g["m0"])
+        s.setup()
+        s.set_graph(True)
+        solvers.append(s)
+        print(f"{g['n']}^2 group: {len(g['members'])} configurations set up", flush=True)
+        bd.append(torch.from_numpy(g["b"].view(np.float64).copy()).to("cuda:0"))
+        xd.append(torch.zeros_like(bd[-1]))
+    torch.cuda.synchronize()
+    st, its, rrs = amgs.solve_multi(solvers, [b.data_ptr() for b in bd], [x.data_ptr() for x in xd],
+                                    [len(g["members"]) for g in groups])
+    nconv = 0
+    for k, g in enumerate(groups):
+        X = xd[k].cpu().numpy().view(np.complex128).reshape(len(g["members"]), -1)
+        conv = rrs[k] < 1e-10
+        nconv += int(conv.sum())
+        print(f"{g['n']}^2: {len(g['members'])} members, {int(conv.sum())} converged, iterations "
+              f"{its[k].tolist()}, status {st[k]}")
+        # every member either converged or used the whole budget (10 x 20)
+        assert np.all(conv | (its[k] == 200)), (its[k], rrs[k])
+        assert st[k] == (amgs.AMGS_OK if conv.all() else amgs.AMGS_NOT_CONVERGED)
+        # sample: the first converged member against the oracle (and its
+        # residual recomputed on the CPU), the first stalled one's oracle stalls too
+        for want in (True, False):
+            sel = np.flatnonzero(conv == want)
+            if len(sel) == 0 or (not want and g["n"] != 128):
+                continue  # a stalled oracle solve is checked on the smallest lattice only
+            # the converged member with the fewest iterations (the oracle's CPU time)
+            i = int(sel[np.argmin(its[k][sel])]) if want else int(sel[0])
+            o = O.Oracle(O.default_params(g["n"], g["n"], workloads.UNEVEN_ROWS, m0=float(g["m0"][i]),
+                                          threads=THREADS, out_restarts=10), g["U"][i])
+            o.setup(draw_order=1)
+            xo, ito, _, rro = o.solve(g["b"][i])
+            print(f"  member {g['members'][i]} (beta {g['beta'][i]:.3f}, m0 {g['m0'][i]:.4f}): GPU {its[k][i]} "
+                  f"iterations relres {rrs[k][i]:.2e}; oracle {ito} relres {rro:.2e}")
+            if want:
+                rcpu = np.linalg.norm(g["b"][i] - o.dphi(X[i])) / np.linalg.norm(g["b"][i])
+                assert rcpu < 1e-10 and rel(X[i], xo) < 1e-8 and int(its[k][i]) <= ito + 3
+            else:
+                assert ito == 200 and rro > 1e-10
+    assert 0 < nconv < 48
+    for s in solvers:
+        s.close()
+
+
+@pytest.mark.timeout(900)
+def test_force_of_the_256_members(uneven):
+    """dS/dtheta of S = phi^+ (D^+ D)^-1 phi for the 256^2 members whose
+    solves converge, batched in one FP64 multi-configuration context: against
+    the oracle's FP64 gradient (two members), and against central differences
+    (h = 1e-5) of S on 64 random links of one member, the 128 perturbed
+    fields solved as one batch of a multi-configuration context."""
+    from amg_schwingermodel_amd import workloads
+
+    g = [g for g in uneven if g["n"] == 256][0]
+    n = 256
+    kw = dict(out_tol=1e-13)
+    s = amgs.Solver(workloads.uneven_params(n, out_restarts=10, **kw), g["U"], m0s=g["m0"])
+    s.setup()
+    s3 = np.tile([1.0, -1.0], n * n)
+    _, _, rr, _ = s.solve(s3 * g["b"])
+    ok = [i for i in range(len(g["members"])) if rr[i] < 1e-12]
+    s.close()
+    assert len(ok) >= 2, rr
+    U, m0s, phi = g["U"][ok], g["m0"][ok], g["b"][ok]
+    s = amgs.Solver(workloads.uneven_params(n, **kw), U, m0s=m0s)
+    s.setup()
+    S, F = s.force(phi, "DdagD")
+    s.close()
+    for i in range(2):
+        o = O.Oracle(O.default_params(n, n, workloads.UNEVEN_ROWS, m0=float(m0s[i]), threads=THREADS, **kw), U[i])
+        o.setup(draw_order=1)
+        So, Fo = o.force(s3 * phi[i])
+        print(f"member {g['members'][ok[i]]}: S {S[i]:.10e} (oracle {So:.10e}), max |F - F_oracle| / max |F| "
+              f"{np.abs(F[i] - Fo).max() / np.abs(Fo).max():.2e}")
+        assert abs(S[i] - So) < 1e-9 * So
+        assert np.abs(F[i] - Fo).max() < 1e-8 * np.abs(Fo).max()
+    # central differences on 64 random links of member 0: 128 perturbed fields in one batch
+    rng = np.random.default_rng(901)
+    links = [(int(rng.integers(n * n)), int(rng.integers(2))) for _ in range(64)]
+    h = 1e-5
+    th = np.angle(U[0])
+    Up = []
+    for (site, mu) in links:
+        for sg in (1.0, -1.0):
+            t = th.copy()
+            t[site, mu] += sg * h
+            Up.append(np.exp(1j * t))
+    sp = amgs.Solver(workloads.uneven_params(n, **kw), np.stack(Up), m0s=[float(m0s[0])] * len(Up))
+    sp.setup()
+    Xp, _, rrp, _ = sp.solve(np.tile(s3 * phi[0], (len(Up), 1)))
+    sp.close()
+    assert np.all(rrp < 1e-12)
+    Sp = np.sum(np.abs(Xp) ** 2, axis=1)  # S = |D^-1 s3 phi|^2
+    fd = (Sp[0::2] - Sp[1::2]) / (2 * h)
+    Fl = np.array([F[0][site, mu] for (site, mu) in links])
+    err = np.abs(fd - Fl)
+    print(f"FD on 64 links: max |fd - F| {err.max():.2e}, max |F| {np.abs(F[0]).max():.2e}")
+    assert np.all(err < 1e-4 * np.abs(F[0]).max() + 1e-6 * np.abs(fd))
