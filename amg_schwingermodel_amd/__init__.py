@@ -118,6 +118,8 @@ SIGNATURES = {
                          ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
+    "amgs_kernel_stats_ex": ([_V, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P, _P],
+                             ctypes.c_int),
 }
 
 
@@ -451,6 +453,17 @@ class Solver:
         self._chk(lib().amgs_kernel_stats(self.h, kclass, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
                   "amgs_kernel_stats")
         return n.value, ms.value, b.value
+
+    def kernel_stats_ex(self, kclass, tag=-1):
+        """(launches, device ms, algorithmic bytes, flops) recorded so far; tag >= 0
+        selects the launches with that tag (SAP step A: the Arnoldi index J)."""
+        n = ctypes.c_longlong(0)
+        ms = ctypes.c_double(0)
+        b = ctypes.c_double(0)
+        f = ctypes.c_double(0)
+        self._chk(lib().amgs_kernel_stats_ex(self.h, kclass, tag, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b),
+                                             ctypes.byref(f)), "amgs_kernel_stats_ex")
+        return n.value, ms.value, b.value, f.value
 
     def last_timings(self):
         a = np.zeros(3)

@@ -373,7 +373,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
                     const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
                     const double pa = h16 ? 16.0 + 8.0 * j : 2 * S * (2 + j);
                     KTimer& ka = c->kt[KT_SAP_A];
-                    int* act = ka.begin(s, sites * 2 * S, g.sspb * pa);
+                    int* act = ka.begin(s, sites * 2 * S, g.sspb * pa, j);
                     if (h16) {
                         if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act, ka);
                         else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act, ka);
@@ -799,7 +799,7 @@ static hipGraphExec_t precond_graph(Ctx& c, int nrhs, c32* out32) {
 static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_ready, c32* zj32) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
     // graph replay (kernel timers record eager launches only)
-    if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on) {
+    if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on && !c.kt[KT_COARSE].on) {
         if (!c.mixed())
             HIP_CHECK(hipMemcpyAsync(c.oeta64.p, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
         else if (!eta32_ready)
@@ -889,6 +889,9 @@ static void outer_solve(Ctx& c, int nrhs) {
     std::vector<int> flags(nrhs);
     const char* ve = std::getenv("AMGS_VERBOSE");
     const bool verbose = ve && ve[0] && ve[0] != '0';
+    // one line per restart cycle (long solves report their progress)
+    const char* pe = std::getenv("AMGS_PROGRESS");
+    const bool progress = pe && pe[0] && pe[0] != '0';
     const auto tstart = std::chrono::steady_clock::now();
     int step = 0;
     hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
@@ -896,6 +899,14 @@ static void outer_solve(Ctx& c, int nrhs) {
     for (int k = 0; k < c.P.out_restarts; k++) {
         hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k, tol, M);
         read_flags(c, st.active, nrhs, flags);
+        if (progress && k > 0) {
+            int act = 0;
+            for (int f : flags) act += f != 0;
+            const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+            std::fprintf(stderr, "[amgs] %dx%d: restart cycle %d done, %d of %d rhs still active, %.0f ms\n", g.nx, g.nt, k,
+                         act, nrhs, el);
+            std::fflush(stderr);
+        }
         if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
         hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0, eta32 ? oeta32 : nullptr);
         for (int j = 0; j < M; j++) {
@@ -2158,6 +2169,29 @@ extern "C" int amgs_kernel_timing(amgs_ctx* ctx, int kclass, int enable) {
     k.resolve(ctx->c.stream);
     k.reset();
     k.on = enable != 0;
+    return AMGS_OK;
+    API_END
+}
+
+extern "C" int amgs_kernel_stats_ex(amgs_ctx* ctx, int kclass, int tag, long long* launches, double* total_ms,
+                                    double* bytes, double* flops) {
+    if (!ctx || kclass < 0 || kclass >= KT_CLASSES || tag < -1 || tag >= KTimer::TAGS || !launches || !total_ms ||
+        !bytes || !flops)
+        return AMGS_ERR_ARG;
+    API_BEGIN
+    KTimer& k = ctx->c.kt[kclass];
+    k.resolve(ctx->c.stream);
+    if (tag < 0) {
+        *launches = k.launches;
+        *total_ms = k.ms;
+        *bytes = k.bytes;
+        *flops = k.flops;
+    } else {
+        *launches = k.t_launches[tag];
+        *total_ms = k.t_ms[tag];
+        *bytes = k.t_bytes[tag];
+        *flops = 0;
+    }
     return AMGS_OK;
     API_END
 }

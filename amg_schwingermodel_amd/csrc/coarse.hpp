@@ -73,6 +73,10 @@ template <typename R> struct CArgs {
     cplx<R>* xchg;
     size_t xchg_n, x_top;
     int* err;  // set when a member barrier waits beyond its bound
+    // algorithmic stencil flops of this launch (member 0 of each rhs adds
+    // them), or null: 8 flops per complex multiply-add of every level
+    // operator and Schwarz-block stencil applied
+    unsigned long long* cnt;
     int crs_lds;  // coarsest GMRES vectors in dynamic LDS ((crs_m + 4) * n entries)
     int sap_lds[MAXL];  // SAP block vectors of level l in dynamic LDS (CW * blkStride entries)
     // level l's Schwarz blocks solved one per member by the whole workgroup
@@ -296,6 +300,8 @@ template <typename R>
 __device__ __forceinline__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* in, cplx<R>* out, cplx<R>* stage = nullptr,
                                       int stage_cap = 0) {
     const int n = A.lv[l].g.ns * A.lv[l].g.dof;
+    if (A.cnt && threadIdx.x == 0 && blockIdx.x % A.grp == 0)
+        atomicAdd(A.cnt, 8ull * 5 * A.lv[l].g.dof * (unsigned long long)n);
     __syncthreads();
     if (stage && n <= stage_cap) {
         for (int e = threadIdx.x; e < n; e += CT) stage[e] = in[e];
@@ -559,6 +565,11 @@ template <typename R>
 __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, const cplx<R>* in, cplx<R>* out,
                                          cplx<R>* part = nullptr) {
     const Geo& g = A.lv[l].g;
+    // stencil flops (member 0 of the rhs): rows x in-block hops x dof
+    if (A.cnt && (threadIdx.x & 63) == 0 && blockIdx.x % A.grp == 0) {
+        const bool b22 = g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2;
+        atomicAdd(A.cnt, 8ull * (b22 ? 3 : 5) * g.dof * (unsigned long long)(g.sspb * g.dof));
+    }
     if (g.sxel == 2 && g.stel == 2 && g.nx == g.nt && g.nx > 2) {
         if constexpr (std::is_same<R, float>::value) {
             if (part && 12 * g.dof <= 4 * 64) {
@@ -715,9 +726,11 @@ __host__ __device__ inline size_t wg_block_lds_bytes(int dof, int m) {
 // 16-byte pairs; the row sums are formed in the order of w_dlocal_2x2p.
 template <typename R>
 __device__ __forceinline__ void wg_dlocal_2x2(R mm, int D, const float __attribute__((ext_vector_type(4)))* Lk,
-                                              cplx<R>* part, const cplx<R>* in, cplx<R>* out) {
+                                              cplx<R>* part, const cplx<R>* in, cplx<R>* out,
+                                              unsigned long long* cnt = nullptr) {
     using V4 = float __attribute__((ext_vector_type(4)));
     const int nb = 4 * D, nu = 3 * nb, np = D / 2, u = threadIdx.x;
+    if (cnt && u == 0) atomicAdd(cnt, 8ull * 3 * D * (unsigned long long)nb);
     __syncthreads();  // in is complete
     if (u < nu) {
         const int q = u / nb, e = u - q * nb, m = e / D;
@@ -759,6 +772,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int m = A.sap_m;
     WaveScal& S = K.S.wv[0];
+    unsigned long long* cnt = K.member == 0 ? A.cnt : nullptr;
     V4* Lk = (V4*)K.dyn;                      // [np][nu]
     cplx<R>* part = (cplx<R>*)(Lk + np * nu);  // [nu]
     cplx<R>* base = part + nu;                 // V_0 .. V_m, x_B, b_B
@@ -783,7 +797,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
             xb[e] = v;
         }
     }
-    wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1));
+    wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1), cnt);
     if (wv == 0) {
         for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
         wave_sync();
@@ -806,7 +820,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
             if (lane == 0) S.g[0] = c64{err, 0.0};
         }
         for (int j = 0; j < m; j++) {
-            wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1));
+            wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1), cnt);
             if (wv == 0) {
                 cplx<R>* w = V(j + 1);
                 for (int i = 0; i <= j; i++) {
@@ -858,7 +872,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
             }
         }
         if (k + 1 == A.sap_restarts) break;  // the last cycle's residual decides nothing
-        wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1));
+        wg_dlocal_2x2<R>(A.mm, D, Lk, part, xb, V(1), cnt);
         if (wv == 0) {
             for (int e = lane; e < nb; e += 64) V(0)[e] = bb[e] - V(1)[e];
             wave_sync();

@@ -2,6 +2,8 @@
 // translation units of its own (one per precision) so that the large kernel
 // compiles in parallel with the rest of the library and only when it changes.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "engine.hpp"
 
 namespace amgs {
@@ -76,8 +78,23 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
             done = true;
         }
     }
-    if (cfg) hipLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
-    else hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+    // per-launch timing (kernel class KT_COARSE): events of the launch's own
+    // dispatch, and the stencil flops the kernel counts (member 0 of each rhs)
+    KTimer& kt = c->kt[KT_COARSE];
+    a.cnt = nullptr;
+    if (kt.begin(c->stream, 0.0, 0.0)) {
+        a.cnt = kt.flop_counter();
+        if (cfg)
+            hipExtLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, kt.start(),
+                                  kt.stop(), 0, a);
+        else
+            hipExtLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, kt.start(),
+                                  kt.stop(), 0, a);
+    } else if (cfg) {
+        hipLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+    } else {
+        hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+    }
     HIP_CHECK(hipGetLastError());
 }
 

@@ -64,28 +64,39 @@ struct Ctx;
 // ---------------------------------------------------------------------------
 struct KTimer {
     static constexpr int CAP = 8192;  // launches recorded between two resolves
+    static constexpr int TAGS = 16;   // per-tag totals (the SAP step's Arnoldi index J)
     bool on = false;
     std::vector<hipEvent_t> ev;
     std::vector<double> fixed, per_item;  // bytes = fixed + per_item * active items
-    DevBuf actlog;
+    std::vector<int> tag;
+    DevBuf actlog, flog;  // per launch: active items; algorithmic flops (kernels that count them)
     long long launches = 0;
-    double ms = 0, bytes = 0;
+    double ms = 0, bytes = 0, flops = 0;
+    long long t_launches[TAGS] = {};
+    double t_ms[TAGS] = {}, t_bytes[TAGS] = {};
 
     ~KTimer() {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
     void reset() {
         launches = 0;
-        ms = bytes = 0;
+        ms = bytes = flops = 0;
+        for (int t = 0; t < TAGS; t++) {
+            t_launches[t] = 0;
+            t_ms[t] = t_bytes[t] = 0;
+        }
         fixed.clear();
         per_item.clear();
+        tag.clear();
     }
     // returns the counter for this launch, or nullptr when not recording
-    int* begin(hipStream_t s, double fx, double pi) {
+    int* begin(hipStream_t s, double fx, double pi, int tg = 0) {
         if (!on || (int)fixed.size() >= CAP) return nullptr;
         if (!actlog.p) {
             actlog.alloc(sizeof(int) * CAP);
             HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+            flog.alloc(sizeof(unsigned long long) * CAP);
+            HIP_CHECK(hipMemset(flog.p, 0, sizeof(unsigned long long) * CAP));
         }
         const size_t i = fixed.size();
         while (ev.size() < 2 * i + 2) {
@@ -95,8 +106,11 @@ struct KTimer {
         }
         fixed.push_back(fx);
         per_item.push_back(pi);
+        tag.push_back(tg < 0 || tg >= TAGS ? 0 : tg);
         return actlog.as<int>() + i;
     }
+    // the flop counter of the launch that the last begin() opened
+    unsigned long long* flop_counter() const { return flog.as<unsigned long long>() + (fixed.size() - 1); }
     // the start and stop events of the launch that the last begin() opened
     hipEvent_t start() const { return ev[2 * (fixed.size() - 1)]; }
     hipEvent_t stop() const { return ev[2 * (fixed.size() - 1) + 1]; }
@@ -105,20 +119,29 @@ struct KTimer {
         if (fixed.empty()) return;
         HIP_CHECK(hipStreamSynchronize(s));
         std::vector<int> act(fixed.size());
+        std::vector<unsigned long long> fl(fixed.size());
         HIP_CHECK(hipMemcpy(act.data(), actlog.p, sizeof(int) * act.size(), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(fl.data(), flog.p, sizeof(unsigned long long) * fl.size(), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < fixed.size(); i++) {
             float t = 0;
             HIP_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            const double b = act[i] > 0 ? fixed[i] + per_item[i] * act[i] : 0.0;
             ms += t;
             launches++;
-            if (act[i] > 0) bytes += fixed[i] + per_item[i] * act[i];
+            bytes += b;
+            flops += (double)fl[i];
+            t_launches[tag[i]]++;
+            t_ms[tag[i]] += t;
+            t_bytes[tag[i]] += b;
         }
         HIP_CHECK(hipMemset(actlog.p, 0, sizeof(int) * CAP));
+        HIP_CHECK(hipMemset(flog.p, 0, sizeof(unsigned long long) * CAP));
         fixed.clear();
         per_item.clear();
+        tag.clear();
     }
 };
-enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_CLASSES = 2 };
+enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_COARSE = 2, KT_CLASSES = 3 };
 
 // ---------------------------------------------------------------------------
 // per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments

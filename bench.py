@@ -26,13 +26,19 @@ A step is one batched solve of the 32 right-hand sides.  Inputs are resident
 in device memory before the timed region; set-up (test vectors, coarse
 operators) runs before it and is reported separately.
 
-Prints one JSON line with "roofline" (the dominant kernel, the SAP Arnoldi
-step A: block Wilson stencil + Gram dot products, timed per launch with HIP
-events on the library stream during the timed solves) and "cpu_baseline"
-(the oracle, a C restatement of the reference path, timed on this host on a
-bounded sample of the same workload).  Beside the value, "large_lattice"
-times a 1024x1024 4-level solve of 16 right-hand sides (beta = 4, m0 = -0.06,
-12 test vectors per level; --no-large skips it).
+Prints one JSON line with "roofline" (the dominant fine-lattice kernel, the
+SAP Arnoldi step A: block Wilson stencil + Gram dot products, timed per launch
+with HIP events on the library stream during the timed solves, also per
+Arnoldi index J), "roofline_coarse" (the persistent coarse-level kernel: its
+counted stencil flops over its launch times), "roofline_coarse_mfma" (the
+coarse operator over the 32 right-hand sides as MFMA block-GEMMs) and
+"cpu_baseline" (the oracle, a C restatement of the reference path, timed on
+this host on a bounded sample of the same workload).  Beside the value,
+"large_lattice" times a 1024x1024 4-level solve of 16 right-hand sides
+(beta = 4, m0 = -0.09, the most negative converging mass on a 0.01 grid;
+12 test vectors per level; --no-large skips it) and "uneven_batch" the 48
+seeded configurations of mixed lattice sizes solved in one call (--no-uneven
+skips it).
 """
 import argparse
 import json
@@ -47,6 +53,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time/RHS to 1e-10 residual, 512² 3-lvl: mixed+batched vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFS, FP64_PEAK_TFS = 157.3, 78.6
+UNEVEN_RESTARTS = 5  # outer FGMRES(20) restarts of the uneven batch (members below critical use all)
 ROWS = [(0, 8, 8, 10, 4, 4), (1, 4, 4, 10, 4, 4)]
 BETA, SWEEPS, GAUGE_SEED = 3.0, 500, 20260512
 M0_NAMED = -0.20
@@ -80,6 +88,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024 4-level solve beside the value")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the 48-configuration uneven batch (128^2/256^2/512^2) solved in one call")
     return ap.parse_args()
 
 
@@ -170,24 +180,30 @@ def main():
     ms_rhs = ms_step / a.rhs
     # roofline pass: the same batched solve with every launch of SAP step A
     # bracketed by HIP events on the library stream (eager launches)
-    sm.kernel_timing(0, True)
-    sm.kernel_timing(1, True)
+    for kc in (0, 1, 2):
+        sm.kernel_timing(kc, True)
     ms_roof, _, _ = timed(sm, a.rhs, 1, 0, f"mixed {a.rhs} rhs, kernel timing")
     n_launch, k_ms, k_bytes = sm.kernel_stats(0)
     nb_launch, kb_ms, kb_bytes = sm.kernel_stats(1)
-    sm.kernel_timing(0, False)
-    sm.kernel_timing(1, False)
+    nc_launch, kc_ms, _, kc_flops = sm.kernel_stats_ex(2)
+    per_j = [sm.kernel_stats_ex(0, j) for j in range(5)]
+    for kc in (0, 1, 2):
+        sm.kernel_timing(kc, False)
+    sap_wgs, coarse_grp = sm.tuning(a.rhs)
     log(f"mixed batched: setup {setup_mixed:.0f} ms, solve {ms_step:.1f} ms for {a.rhs} rhs -> {ms_rhs:.2f} ms/rhs, "
         f"iterations {its.min()}..{its.max()}, max relres {rrs.max():.2e}")
     mixed = {"precision": a.precision, "graph": bool(a.graph), "sap_workgroups_autotuned": tuned_wgs,
+             "coarse_workgroups_per_rhs_autotuned": coarse_grp,
              "setup_ms": round(setup_mixed, 1),
              "iterations_min": int(its.min()), "iterations_max": int(its.max()), "relres_max": float(rrs.max()),
              "eager_timed_solve_ms_per_rhs": round(ms_roof / a.rhs, 3)}
     if not a.no_plain:
         x0 = Xd[0].cpu().numpy().view(np.complex128)
         mixed["x_relerr_vs_fp64_rhs0"] = float(np.linalg.norm(x0 - xs_plain) / np.linalg.norm(xs_plain))
+    one_ok = bool(rrs.max() < 1e-10)
+    mixed["converged"] = one_ok
     out["mixed_batched"] = mixed
-    ms_best, concurrent = ms_step, 1
+    ms_best, concurrent = (ms_step, 1) if one_ok else (float("inf"), 1)
 
     def concurrent_solve(nl, rows, Ul, m0, nrhs, split, bd, xd, tag):
         """The nrhs right-hand sides as `split` concurrent sub-batches, one
@@ -231,6 +247,9 @@ def main():
         out["mixed_batched_concurrent"] = rec
         if ms_split < ms_best and ok:
             ms_best, concurrent = ms_split, a.split
+    converged = ms_best != float("inf")
+    if not converged:  # neither form reached 1e-10: report the one-batch time, flagged
+        ms_best = ms_step
     if "plain_fp64" in out:
         out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] * a.rhs / ms_best, 2)
 
@@ -257,6 +276,33 @@ def main():
                                   "kernel": "k_sap_arnoldi_b<float,H> (W -= sum H_ij Vt_i, |W|^2, Vt_j+1 stored)",
                                   "launches": nb_launch, "avg_launch_us": round(kb_ms * 1e3 / max(nb_launch, 1), 2),
                                   "algorithmic_bytes_per_launch": round(kb_bytes / max(nb_launch, 1))}
+    out["roofline"]["per_J"] = [
+        {"J": j, "launches": nj, "avg_launch_us": round(msj * 1e3 / max(nj, 1), 2),
+         "algorithmic_bytes_per_launch": round(bj / max(nj, 1)),
+         "frac": round(bj / (msj * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if msj > 0 else None}
+        for j, (nj, msj, bj, _) in enumerate(per_j)]
+    # the persistent coarse-level kernel: latency-bound (one nested solve per
+    # rhs); its counted stencil flops (8 per complex multiply-add of every level
+    # operator and Schwarz-block stencil, once per rhs) over its device time
+    achieved_c = kc_flops / (kc_ms * 1e-3) / 1e12 if kc_ms > 0 else 0.0
+    out["roofline_coarse"] = {"bound": "latency", "achieved": round(achieved_c, 3), "peak": FP32_PEAK_TFS,
+                              "unit": "TFLOP/s", "frac": round(achieved_c / FP32_PEAK_TFS, 5), "traffic": None,
+                              "kernel": "k_coarse<float> (K-cycle levels >= 1: FGMRES, SAP, coarsest GMRES)",
+                              "launches": nc_launch, "avg_launch_us": round(kc_ms * 1e3 / max(nc_launch, 1), 1),
+                              "stencil_flops_per_launch": round(kc_flops / max(nc_launch, 1)),
+                              "workgroups_per_launch": a.rhs * coarse_grp,
+                              "compute_units_busy": min(256, a.rhs * coarse_grp)}
+    # the coarse operator over the whole batch as MFMA block-GEMMs (level 1)
+    mf = {}
+    for prec, tag, peak in ((amgs.PREC_FP64, "fp64", FP64_PEAK_TFS), (precs[a.precision], "fp32", FP32_PEAK_TFS)):
+        w, us = sm.autotune_batched_D(1, a.rhs, prec)
+        dof, ns1 = 2 * ROWS[0][3], ROWS[0][1] * ROWS[0][2]
+        fl = 8.0 * 5 * dof * dof * ns1 * a.rhs
+        mf[tag] = {"achieved": round(fl / (us * 1e-6) / 1e12, 3), "peak": peak, "unit": "TFLOP/s",
+                   "frac": round(fl / (us * 1e-6) / 1e12 / peak, 5), "avg_launch_us": round(us, 2),
+                   "flops_per_launch": fl, "waves_per_workgroup_autotuned": w}
+    out["roofline_coarse_mfma"] = {"bound": "mfma", "kernel": "k_cdop_mfma (level-1 operator, 8x8 sites, dof 20, "
+                                                             f"{a.rhs} rhs, 16x16x4 MFMA tiles)", **mf}
     sm.close()
 
     # ---- the named mass m0 = -0.20: bounded sample --------------------------
@@ -276,7 +322,7 @@ def main():
 
     # ---- a larger lattice: 1024^2, 4 levels, 16 rhs (not the value) ---------
     if not a.no_large:
-        nl, nrl, m0l = 1024, 16, -0.06
+        nl, nrl, m0l = 1024, 16, -0.09
         t0 = time.time()
         Ul, plq = amgs.gauge_metropolis(nl, nl, BETA_L, SWEEPS, GAUGE_SEED_L)
         log(f"gauge field {nl}x{nl} beta={BETA_L}: plaquette {plq:.5f} ({time.time() - t0:.1f}s)")
@@ -302,6 +348,51 @@ def main():
         out["large_lattice"] = large
         del Bld, Xld
 
+    # ---- uneven batch: 48 configurations, three lattice sizes, one call -----
+    if not a.no_uneven:
+        from amg_schwingermodel_amd import workloads
+
+        t0 = time.time()
+        groups = workloads.uneven_batch(log=None)
+        log(f"uneven batch: 48 gauge fields ({time.time() - t0:.1f}s)")
+        solvers, bds, xds = [], [], []
+        t = time.perf_counter()
+        for g in groups:
+            s = amgs.Solver(workloads.uneven_params(g["n"], precision=precs[a.precision], out_restarts=UNEVEN_RESTARTS),
+                            g["U"], m0s=g["m0"])
+            s.setup()
+            solvers.append(s)
+            bds.append(torch.from_numpy(g["b"].view(np.float64).copy()).to(dev))
+            xds.append(torch.zeros_like(bds[-1]))
+            log(f"uneven batch: {g['n']}^2 group of {len(g['members'])} configurations set up")
+        setup_u = (time.perf_counter() - t) * 1e3
+        nr = [len(g["members"]) for g in groups]
+        # one call, eager launches (no graph capture inside the timed call);
+        # members below their critical mass use the whole budget, with the
+        # coarse solves at their iteration limits, so the library reports
+        # each restart cycle
+        os.environ["AMGS_PROGRESS"] = "1"
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st, its_u, rr_u = amgs.solve_multi(solvers, [b.data_ptr() for b in bds], [x.data_ptr() for x in xds], nr)
+        torch.cuda.synchronize()
+        ms_u = (time.perf_counter() - t) * 1e3
+        os.environ["AMGS_PROGRESS"] = "0"
+        conv = [int((r < 1e-10).sum()) for r in rr_u]
+        log(f"uneven batch: {ms_u:.1f} ms for 48 configurations, converged {conv} of {nr}")
+        out["uneven_batch"] = {
+            "workload": "48 seeded configurations: beta U[1,4], m0 U[-0.22,-0.10], sizes drawn from "
+                        "128^2/256^2/512^2, 3 levels (0 8 8 10 4 4 / 1 4 4 10 4 4), one Gaussian rhs each, "
+                        f"FGMRES(20, {UNEVEN_RESTARTS} restarts) to 1e-10; one multi-configuration context per "
+                        "size, all in one amgs_solve_multi call (eager launches)",
+            "members_per_size": dict(zip([str(g["n"]) for g in groups], nr)),
+            "converged_per_size": dict(zip([str(g["n"]) for g in groups], conv)),
+            "setup_ms": round(setup_u, 1), "solve_ms": round(ms_u, 1),
+            "iterations": {str(g["n"]): its_u[k].tolist() for k, g in enumerate(groups)}}
+        for s in solvers:
+            s.close()
+        del bds, xds
+
     # ---- CPU baseline: the oracle on this host, one right-hand side ----------
     if not a.no_cpu:
         import oracle as O
@@ -324,7 +415,12 @@ def main():
             est = t_solve * 1e3
             what = f"the full solve to 1e-10 ({it_cpu} outer iterations, relres {rr_cpu:.2e}, {t_solve:.1f}s)"
         log(f"oracle: setup {t_setup:.1f}s, {it_cpu} outer iterations in {t_solve:.1f}s -> {est / 1e3:.2f}s/rhs")
-        out["cpu_baseline"] = {"value": round(est, 1), "unit": "ms/RHS", "cores": threads, "kind": "port",
+        cpu = "unknown"
+        try:
+            cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        except (OSError, IndexError):
+            pass
+        out["cpu_baseline"] = {"value": round(est, 1), "unit": "ms/RHS", "cores": threads, "cpu": cpu, "kind": "port",
                                "sample": f"oracle (C restatement of the reference path, OpenMP over SAP blocks) "
                                          f"on the {nx}x{nt} m0={a.m0} workload, rhs 0 of the batch: set-up "
                                          f"({t_setup:.1f}s, not in the value), then {what}",
@@ -339,6 +435,7 @@ def main():
                                   f"(named m0={M0_NAMED} stagnates; sampled in named_mass_sample), "
                                   f"rows 0 8 8 10 4 4 / 1 4 4 10 4 4",
                       "nrhs_batch": a.rhs, "concurrent_sub_batches": concurrent, "levels": 3, "m0": a.m0,
+                      "converged": converged,
                       "parallelism": "single GPU"}}
     res.update(out)
     print(json.dumps(res), flush=True)

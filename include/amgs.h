@@ -281,12 +281,19 @@ int amgs_last_timings(amgs_ctx* c, double* ms3);
  * pair on the context's stream around every launch of the class, plus the
  * algorithmic bytes of the items that did work in that launch.
  * kclass: 0 = SAP Arnoldi step A (block stencil + dot products),
- *         1 = SAP Arnoldi step B (orthogonalisation + norm).
+ *         1 = SAP Arnoldi step B (orthogonalisation + norm),
+ *         2 = the persistent coarse-level kernel (its algorithmic stencil
+ *             flops: 8 per complex multiply-add of every level operator and
+ *             Schwarz-block stencil it applies, counted once per rhs).
  * enable != 0 starts recording (and clears the totals), 0 stops it.
  * amgs_kernel_stats returns the totals: launches, summed device time (ms)
- * and summed algorithmic bytes. */
+ * and summed algorithmic bytes; amgs_kernel_stats_ex also the flops, and
+ * with tag >= 0 the totals of the launches with that tag (step A: the
+ * Arnoldi index J). */
 int amgs_kernel_timing(amgs_ctx* c, int kclass, int enable);
 int amgs_kernel_stats(amgs_ctx* c, int kclass, long long* launches, double* total_ms, double* bytes);
+int amgs_kernel_stats_ex(amgs_ctx* c, int kclass, int tag, long long* launches, double* total_ms, double* bytes,
+                         double* flops);
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,12 @@ static void common_init(const char* params, double m0, int tv_iters) {
     AMGV::Nit = nit ? atoi(nit) : 0;
     AMGV::SAP_test_vectors_iterations = tv_iters;
     mass::m0 = m0;
-    SAPV::sap_blocks_per_proc = 1 << 20;
+    // SAP blocks per rank: every block on each rank (the default here), or
+    // AMGS_REF_BPP per rank as the reference's own runs distribute them
+    // (1 block per rank and colour with mpirun -np 8, src/variables.cpp:51,
+    // script.sh:38)
+    const char* bpp = getenv("AMGS_REF_BPP");
+    SAPV::sap_blocks_per_proc = bpp ? atoi(bpp) : 1 << 20;
 }
 
 static GaugeConf load_conf(const std::string& path) {

@@ -343,3 +343,40 @@ def test_named_mass_iteration_is_rounding_chaotic():
         dev[m0] = np.max(np.abs(hp - h) / h)
     assert dev[-0.12] < 1e-9, dev
     assert dev[-0.20] > 1e-4, dev
+
+
+# -------------------------------------------------- reference-side shim ----
+REF = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "include")), reason="the reference tree is not present")
+def test_integration_shim_compiles_against_the_reference(tmp_path):
+    """The C++ binding that INTEGRATION.md gives a maintainer of the reference
+    (FGMRES_AMG_GPU, the drop-in for FGMRES_AMG) compiles against the
+    reference's own headers and links with the reference's variables.cpp and
+    gauge_conf.cpp (compiled where they lie) and libamgs.so.  Nothing runs:
+    this container has no GPU."""
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```cpp\n(.*?)```", txt, flags=re.S)
+    shim = [b for b in blocks if "class FGMRES_AMG_GPU" in b]
+    assert len(shim) == 1
+    (tmp_path / "amgs_fgmres_amg.h").write_text(shim[0])
+    (tmp_path / "main.cpp").write_text(
+        '#include "amgs_fgmres_amg.h"\n'
+        "int main(int argc, char**) {\n"
+        "    if (argc > 99) {  // never taken: instantiates the shim without running it\n"
+        "        GaugeConf G(LV::Nx, LV::Nt);\n"
+        "        FGMRES_AMG_GPU f(LV::Ntot, 2, 20, 50, 1e-10, G, -0.18);\n"
+        "        spinor b(LV::Ntot, c_vector(2, 0)), x(LV::Ntot, c_vector(2, 0));\n"
+        "        return f.fgmres(b, b, x, true, true);\n"
+        "    }\n"
+        "    return 0;\n"
+        "}\n")
+    lib_dir = os.path.join(ROOT, "amg_schwingermodel_amd")
+    cmd = ["g++", "-std=c++20", "-O1", "-I", str(tmp_path), "-I", os.path.join(ROOT, "include"), "-I",
+           os.path.join(REF, "include"), str(tmp_path / "main.cpp"), os.path.join(REF, "src", "variables.cpp"),
+           os.path.join(REF, "src", "gauge_conf.cpp"), "-L", lib_dir, "-lamgs", "-Wl,-rpath," + lib_dir,
+           "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp_path / "shim")]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert (tmp_path / "shim").exists()

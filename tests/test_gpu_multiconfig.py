@@ -86,7 +86,9 @@ def uneven():
 def test_uneven_batch_in_one_call(uneven):
     """48 configurations of three lattice sizes, one multi-configuration
     context per size, solved by one amgs_solve_multi call (mixed precision,
-    FP16 SAP basis, FGMRES(20) with 10 restarts)."""
+    FP16 SAP basis, FGMRES(20) with 2 restarts: members below their critical
+    mass run every coarse solve to its iteration limit, about a second per
+    outer iteration at 512^2, so the budget is kept short)."""
     import torch
 
     from amg_schwingermodel_amd import workloads
@@ -95,7 +97,7 @@ def test_uneven_batch_in_one_call(uneven):
     assert sum(len(g["members"]) for g in groups) == 48 and len(groups) == 3
     solvers, bd, xd = [], [], []
     for g in groups:
-        p = workloads.uneven_params(g["n"], precision=amgs.PREC_MIXED16, out_restarts=10)
+        p = workloads.uneven_params(g["n"], precision=amgs.PREC_MIXED16, out_restarts=2)
         s = amgs.Solver(p, g["U"], m0s=g["m0"])
         s.setup()
         s.set_graph(True)
@@ -113,8 +115,8 @@ def test_uneven_batch_in_one_call(uneven):
         nconv += int(conv.sum())
         print(f"{g['n']}^2: {len(g['members'])} members, {int(conv.sum())} converged, iterations "
               f"{its[k].tolist()}, status {st[k]}")
-        # every member either converged or used the whole budget (10 x 20)
-        assert np.all(conv | (its[k] == 200)), (its[k], rrs[k])
+        # every member either converged or used the whole budget (2 x 20)
+        assert np.all(conv | (its[k] == 40)), (its[k], rrs[k])
         assert st[k] == (amgs.AMGS_OK if conv.all() else amgs.AMGS_NOT_CONVERGED)
         # sample: the first converged member against the oracle (and its
         # residual recomputed on the CPU), the first stalled one's oracle stalls too
@@ -125,7 +127,7 @@ def test_uneven_batch_in_one_call(uneven):
             # the converged member with the fewest iterations (the oracle's CPU time)
             i = int(sel[np.argmin(its[k][sel])]) if want else int(sel[0])
             o = O.Oracle(O.default_params(g["n"], g["n"], workloads.UNEVEN_ROWS, m0=float(g["m0"][i]),
-                                          threads=THREADS, out_restarts=10), g["U"][i])
+                                          threads=THREADS, out_restarts=2), g["U"][i])
             o.setup(draw_order=1)
             xo, ito, _, rro = o.solve(g["b"][i])
             print(f"  member {g['members'][i]} (beta {g['beta'][i]:.3f}, m0 {g['m0'][i]:.4f}): GPU {its[k][i]} "
@@ -134,7 +136,7 @@ def test_uneven_batch_in_one_call(uneven):
                 rcpu = np.linalg.norm(g["b"][i] - o.dphi(X[i])) / np.linalg.norm(g["b"][i])
                 assert rcpu < 1e-10 and rel(X[i], xo) < 1e-8 and int(its[k][i]) <= ito + 3
             else:
-                assert ito == 200 and rro > 1e-10
+                assert ito == 40 and rro > 1e-10
     assert 0 < nconv < 48
     for s in solvers:
         s.close()
@@ -152,11 +154,13 @@ def test_force_of_the_256_members(uneven):
     g = [g for g in uneven if g["n"] == 256][0]
     n = 256
     kw = dict(out_tol=1e-13)
-    s = amgs.Solver(workloads.uneven_params(n, out_restarts=10, **kw), g["U"], m0s=g["m0"])
+    # members that converge fast: relative residual below 1e-6 after one
+    # FGMRES(20) cycle (the others are near or below their critical mass)
+    s = amgs.Solver(workloads.uneven_params(n, out_restarts=1), g["U"], m0s=g["m0"])
     s.setup()
     s3 = np.tile([1.0, -1.0], n * n)
     _, _, rr, _ = s.solve(s3 * g["b"])
-    ok = [i for i in range(len(g["members"])) if rr[i] < 1e-12]
+    ok = [i for i in range(len(g["members"])) if rr[i] < 1e-6]
     s.close()
     assert len(ok) >= 2, rr
     U, m0s, phi = g["U"][ok], g["m0"][ok], g["b"][ok]
