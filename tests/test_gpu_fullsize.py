@@ -13,7 +13,7 @@ right-hand sides that the benchmark generates.
 * m0 = -0.20 (the named mass): below the critical mass of this ensemble.
   Ten FP64 outer iterations follow the oracle's residual history to within
   the oracle's own sensitivity to rounding (the iteration is chaotic there),
-  and the solve stagnates (a hundred iterations stay far above 1e-10).
+  and the solve stagnates (forty iterations stay far above 1e-10).
 
 1024x1024, 4 levels ("0 8 8 12 4 4" / "1 4 4 12 4 4" / "2 2 2 12 2 2"),
 beta = 4: the mixed-precision batch of 16 right-hand sides at M0_LARGE, the
@@ -111,7 +111,7 @@ def test_metric_config_mixed_batch(c3):
 
 def test_named_mass_history_and_stagnation(c3):
     """m0 = -0.20: ten outer FGMRES iterations (one cycle of FGMRES(10)) on the
-    GPU in FP64 against the oracle, and the mixed solve over a hundred
+    GPU in FP64 against the oracle, and the mixed solve over forty
     iterations, which stays far above the tolerance (D has eigenvalues with
     negative real part on this ensemble; the spectrum itself is checked on
     smaller lattices of the same coupling in the CPU tests).
@@ -141,13 +141,13 @@ def test_named_mass_history_and_stagnation(c3):
     assert dev_gpu < 10 * dev_self
     assert abs(rr - rro) < 0.05 * rro and rr > 0.05
     s.close()
-    sm = amgs.Solver(amgs.default_params(N, N, ROWS, m0=M0_NAMED, out_m=20, out_restarts=5,
+    sm = amgs.Solver(amgs.default_params(N, N, ROWS, m0=M0_NAMED, out_m=20, out_restarts=2,
                                          precision=amgs.PREC_MIXED16), U)
     sm.setup()
     sm.set_graph(True)
     _, itm, rrm, _ = sm.solve(b)
     print(f"512^2 m0={M0_NAMED}, {itm} mixed iterations: relres {rrm:.3e}")
-    assert itm == 100 and rrm > 1e-4
+    assert itm == 40 and rrm > 1e-4
     sm.close()
 
 
