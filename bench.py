@@ -36,9 +36,11 @@ coarse operator over the 32 right-hand sides as MFMA block-GEMMs) and
 this host on a bounded sample of the same workload).  Beside the value,
 "large_lattice" times a 1024x1024 4-level solve of 16 right-hand sides
 (beta = 4, m0 = -0.09, the most negative converging mass on a 0.01 grid;
-12 test vectors per level; --no-large skips it) and "uneven_batch" the 48
+12 test vectors per level; --no-large skips it), "uneven_batch" the 48
 seeded configurations of mixed lattice sizes solved in one call (--no-uneven
-skips it).
+skips it) and "small_workloads" the single-rhs 64x64 2-level and 256x256
+3-level solves (beta = 2, m0 = -0.18), set-up and solve timed apart, plain FP64
+and mixed (--no-small skips them).
 """
 import argparse
 import json
@@ -61,6 +63,9 @@ M0_NAMED = -0.20
 # the larger lattice measured beside the value (near its critical mass)
 ROWS_L = [(0, 8, 8, 12, 4, 4), (1, 4, 4, 12, 4, 4), (2, 2, 2, 12, 2, 2)]
 BETA_L, GAUGE_SEED_L = 4.0, 20260513
+# the smaller workloads beside the value: (n, rows, m0, beta, gauge seed, rhs seed)
+SMALL = {"64x64_2lvl": (64, [(0, 4, 4, 10, 4, 4)], -0.18, 2.0, 20260510, 1000),
+         "256x256_3lvl": (256, [(0, 8, 8, 10, 4, 4), (1, 4, 4, 10, 4, 4)], -0.18, 2.0, 20260511, 2001)}
 
 
 def parse():
@@ -88,6 +93,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024 4-level solve beside the value")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the single-rhs 64x64 2-level and 256x256 3-level solves beside the value")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the 48-configuration uneven batch (128^2/256^2/512^2) solved in one call")
     return ap.parse_args()
@@ -157,8 +164,35 @@ def main():
                              "iterations": int(it_plain[0]), "relres": float(rr_plain[0])}
         sp.close()
 
-    # ---- mixed precision, batched, to 1e-10 (the timed value) ---------------
+    # ---- the smaller workloads, one right-hand side each: set-up + solve ----
     precs = {"mixed": amgs.PREC_MIXED, "mixed16": amgs.PREC_MIXED16}
+    if not a.no_small:
+        small = {}
+        for name, (ns, rows_s, m0s, beta_s, gseed, bseed) in SMALL.items():
+            Us, _ = amgs.gauge_metropolis(ns, ns, beta_s, SWEEPS, gseed)
+            bs = amgs.gaussian(ns * ns * 2, bseed)
+            bsd = torch.from_numpy(bs.view(np.float64).copy()).to(dev).reshape(1, -1)
+            xsd = torch.zeros_like(bsd)
+            rec = {"workload": f"{ns}x{ns} {len(rows_s) + 1}-level K-cycle FGMRES(20,50) to 1e-10, one Gaussian "
+                               f"rhs, beta={beta_s} Metropolis ({SWEEPS} sweeps), m0={m0s}, rows "
+                               + " / ".join(" ".join(map(str, r)) for r in rows_s)}
+            for prec, tag in ((amgs.PREC_FP64, "plain_fp64"), (precs[a.precision], a.precision)):
+                s = amgs.Solver(amgs.default_params(ns, ns, rows_s, m0=m0s, max_rhs=1, precision=prec), Us)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                s.setup()
+                torch.cuda.synchronize()
+                setup_s = (time.perf_counter() - t) * 1e3
+                s.set_graph(bool(a.graph))
+                ms_s, it_s, rr_s = timed(s, 1, a.steps, a.warmup, f"{ns}^2 {tag}", bx=(bsd, xsd))
+                rec[tag] = {"setup_ms": round(setup_s, 2), "solve_ms": round(ms_s, 3), "iterations": int(it_s[0]),
+                            "relres": float(rr_s[0])}
+                s.close()
+            log(f"{name}: {rec}")
+            small[name] = rec
+        out["small_workloads"] = small
+
+    # ---- mixed precision, batched, to 1e-10 (the timed value) ---------------
     if not a.no_alt:
         alt = "mixed" if a.precision == "mixed16" else "mixed16"
         sa = amgs.Solver(amgs.default_params(nx, nt, ROWS, m0=a.m0, max_rhs=a.rhs, precision=precs[alt]), U)

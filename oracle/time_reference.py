@@ -10,7 +10,7 @@ own runs distribute them (script.sh:38, src/variables.cpp:51), and on one rank
 with every block.  The reference's own timers give set-up (the FGMRES_AMG
 constructor) and solve (fgmres) separately (src/tests.cpp:75-79).
 
-    python oracle/time_reference.py [--out oracle/_ref/timings.json] [--cases C1,C2,C3]
+    python oracle/time_reference.py [--out oracle/reference_timings.json] [--cases C1,C2,C3]
 """
 import argparse
 import json
@@ -79,7 +79,7 @@ def run_case(name, nproc_list=(8, 1)):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=os.path.join(HERE, "_ref", "timings.json"))
+    ap.add_argument("--out", default=os.path.join(HERE, "reference_timings.json"))
     ap.add_argument("--cases", default="C1,C2,C3")
     a = ap.parse_args()
     res = {"cpu": cpu_model(), "cores": os.cpu_count(), "cases": {}}
