@@ -35,6 +35,7 @@
 #include "krylov.hpp"
 #include "mfma.hpp"
 #include "outer.hpp"
+#include "sap_block.hpp"
 #include "setup.hpp"
 
 namespace amgs {
@@ -245,19 +246,24 @@ template <typename R> void Engine<R>::restrict0(int nrhs, const cplx<R>* w, cons
         if (nv <= 12 && wcs == 0) {
             const dim3 g2(g1.ns, cdiv_u(nrhs, 2));
             if (nv <= 8)
-                hipLaunchKernelGGL(k_restrict0_g<8>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs);
+                hipLaunchKernelGGL(k_restrict0_g<8>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs,
+                                   rmask);
             else
-                hipLaunchKernelGGL(k_restrict0_g<12>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs);
+                hipLaunchKernelGGL(k_restrict0_g<12>, g2, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, nrhs,
+                                   rmask);
             HIP_CHECK(hipGetLastError());
             return;
         }
     }
     if (nv <= 8)
-        hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
+        hipLaunchKernelGGL((k_restrict0<R, 8>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs,
+                           rmask);
     else if (nv <= 16)
-        hipLaunchKernelGGL((k_restrict0<R, 16>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
+        hipLaunchKernelGGL((k_restrict0<R, 16>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs,
+                           rmask);
     else
-        hipLaunchKernelGGL((k_restrict0<R, 24>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs);
+        hipLaunchKernelGGL((k_restrict0<R, 24>), grid, dim3(FT), 0, c->stream, g, nv, w, v, out, g1.dof, g1.ns, wcs,
+                           rmask);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -275,16 +281,96 @@ static int sap_chunks(const Geo& g, int nrhs, int target) {
     const int items = g.ncol * nrhs, maxc = (int)cdiv_u(g.sspb, FT);
     return std::max(1, std::min(maxc, (int)cdiv_u(target, items)));
 }
+// FP16 Krylov storage exists for the FP32 engine only
+template <typename R> constexpr bool kH16 = std::is_same<R, float>::value;
+
+// The level-0 SAP with one workgroup per item (k_sap_block): pairs per thread
+// for this context's geometry and precision, 0 where the form does not apply
+// (non-square lattice, blocks without site pairs along t or spanning a
+// direction, a block tile beyond the LDS budget, fewer than 64 or an uneven
+// split of the pairs over the threads).  AMGS_SAP_BLOCK=0 switches it off.
+template <typename R> static int sap_block_pp(const Ctx& c) {
+    static const bool off = [] {
+        const char* e = std::getenv("AMGS_SAP_BLOCK");
+        return e && e[0] == '0';
+    }();
+    const Geo& g = c.geo[0];
+    if (off || g.nx != g.nt || g.stel % 2 || g.stel >= g.nt || g.sxel >= g.nx || g.sspb % 2 || g.nt % 2 ||
+        c.P.sap_m > SAP_MAXM)
+        return 0;
+    const bool h16 = kH16<R> && c.h16();
+    const size_t site = h16 ? sb_site_bytes<float, true>() : sb_site_bytes<R, false>();
+    if ((size_t)g.sspb * site > SB_TILE_MAX) return 0;
+    const int pairs = g.sspb / 2, ppmax = h16 ? 8 : (std::is_same<R, float>::value ? 4 : 2);
+    // a thread's pairs lie whole rows apart: 2 pairs / pp sites a multiple of stel
+    for (int pp = 1; pp <= ppmax; pp *= 2)
+        if (pairs % pp == 0 && pairs / pp <= SBT && (pairs / pp) % 64 == 0 && (2 * pairs / pp) % g.stel == 0) return pp;
+    return 0;
+}
+
 // workgroup target of the level-0 SAP launches for this batch size: the
-// autotuned value (amgs_autotune) if there is one, else the default
+// autotuned (amgs_autotune) or pinned (amgs_set_tuning) value if there is
+// one, else the default.  A target of 1 is one workgroup per item: the
+// block-per-workgroup kernel where it applies (the default there).
 template <typename R> int Engine<R>::sap_wgs(int nrhs) const {
     for (const auto& t : sap_tuned)
         if (t.first == nrhs) return t.second;
+    // default: the block form when its workgroups fill the chip (at least
+    // 64 Ki threads) or when the chunked form could not split a block either
+    const int pp = sap_block_pp<R>(*c);
+    if (pp > 0) {
+        const Geo& g = c->geo[0];
+        const long long threads = (long long)g.ncol * nrhs * (g.sspb / (2 * pp));
+        if (threads >= 65536 || g.sspb <= FT) return 1;
+    }
     return sap_default_wgs();
 }
 
-// FP16 Krylov storage exists for the FP32 engine only
-template <typename R> constexpr bool kH16 = std::is_same<R, float>::value;
+template <typename R, bool H, int PP>
+static void sap_block_launch(const Ctx& c, int nrhs, const SapArgs<R>& a, cplx<R>* x, int* act, KTimer& kt) {
+    const Geo& g = c.geo[0];
+    const size_t dyn = (size_t)g.sspb * sb_site_bytes<R, H>();
+    static bool attr = false;
+    if (!attr) {
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_sap_block<R, H, PP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)SB_TILE_MAX));
+        attr = true;
+    }
+    const dim3 grid(g.ncol, nrhs), block(g.sspb / (2 * PP));
+    if (act)
+        hipExtLaunchKernelGGL((k_sap_block<R, H, PP>), grid, block, dyn, c.stream, kt.start(), kt.stop(), 0, a, x,
+                              c.P.sap_restarts, act);
+    else
+        hipLaunchKernelGGL((k_sap_block<R, H, PP>), grid, block, dyn, c.stream, a, x, c.P.sap_restarts, act);
+}
+
+// one colour of one sweep by k_sap_block (pp: sap_block_pp)
+template <typename R>
+static void sap_block_colour(const Ctx& c, int nrhs, int pp, bool h16, const SapArgs<R>& a, cplx<R>* x, int* act,
+                             KTimer& kt) {
+    if constexpr (std::is_same<R, float>::value) {
+        if (h16) {
+            switch (pp) {
+                case 1: sap_block_launch<float, true, 1>(c, nrhs, a, x, act, kt); return;
+                case 2: sap_block_launch<float, true, 2>(c, nrhs, a, x, act, kt); return;
+                case 4: sap_block_launch<float, true, 4>(c, nrhs, a, x, act, kt); return;
+                case 8: sap_block_launch<float, true, 8>(c, nrhs, a, x, act, kt); return;
+            }
+        } else {
+            switch (pp) {
+                case 1: sap_block_launch<float, false, 1>(c, nrhs, a, x, act, kt); return;
+                case 2: sap_block_launch<float, false, 2>(c, nrhs, a, x, act, kt); return;
+                case 4: sap_block_launch<float, false, 4>(c, nrhs, a, x, act, kt); return;
+            }
+        }
+    } else {
+        switch (pp) {
+            case 1: sap_block_launch<double, false, 1>(c, nrhs, a, x, act, kt); return;
+            case 2: sap_block_launch<double, false, 2>(c, nrhs, a, x, act, kt); return;
+        }
+    }
+    throw ApiError(AMGS_ERR_ARG, "SAP block kernel: no instance for this shape");
+}
 
 template <typename R, bool H, bool SQ>
 static void sap_arnoldi_a(dim3 grid, hipStream_t s, const SapArgs<R>& a, int j, int k, int* act, KTimer& kt) {
@@ -311,7 +397,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     const R mm = R(c->P.m0 + 2.0);
     const unsigned ntile = cdiv_u(g.ns, FT);
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_zero_int, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, nrhs, skip.as<int>());
+    hipLaunchKernelGGL(k_skip_init, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, nrhs, rmask, skip.as<int>());
     if (nu <= 0) return;
     hipLaunchKernelGGL(k_norm_part<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g.ns, v, part_v.as<double>());
     // r is read only on the blocks of the colour about to be smoothed: the
@@ -320,7 +406,8 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     const unsigned char* mblack = mred + g.nsb;
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                        (double*)nullptr, skip.as<int>(), mred, ucs, mmv);
-    const int nchunk = sap_chunks(g, nrhs, sap_wgs(nrhs)), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
+    const int target = sap_wgs(nrhs), bpp = target == 1 ? sap_block_pp<R>(*c) : 0;
+    const int nchunk = sap_chunks(g, nrhs, target), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
     a.mm = mm;
@@ -358,57 +445,66 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
-            if (h16) {
-                if (sq) hipLaunchKernelGGL((k_sap_init<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
-                else hipLaunchKernelGGL((k_sap_init<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
+            if (bpp > 0) {
+                // the colour's whole block solve, one workgroup per item;
+                // algorithmic bytes: links once, per item the bytes per block
+                // site that the kernel counts
+                KTimer& kb = c->kt[KT_SAP_BLOCK];
+                int* act = kb.begin(s, (double)g.ncol * g.sspb * 2 * sizeof(cplx<R>), (double)g.sspb);
+                sap_block_colour<R>(*c, nrhs, bpp, h16, a, x, act, kb);
             } else {
-                if (sq) hipLaunchKernelGGL((k_sap_init<R, false, true>), grid, dim3(FT), 0, s, a);
-                else hipLaunchKernelGGL((k_sap_init<R, false, false>), grid, dim3(FT), 0, s, a);
-            }
-            for (int k = 0; k < c->P.sap_restarts; k++) {
-                for (int j = 0; j < a.m; j++) {
-                    // algorithmic bytes: links of the colour's sites once; per active
-                    // item Vt_j read, w written, Vt_i (i < j) read for the dots.
-                    // With FP16 slots: w and every slot FP16.
-                    const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
-                    const double pa = h16 ? 16.0 + 8.0 * j : 2 * S * (2 + j);
-                    KTimer& ka = c->kt[KT_SAP_A];
-                    int* act = ka.begin(s, sites * 2 * S, g.sspb * pa, j);
-                    if (h16) {
-                        if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act, ka);
-                        else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act, ka);
-                    } else {
-                        if (sq) sap_arnoldi_a<R, false, true>(grid, s, a, j, k, act, ka);
-                        else sap_arnoldi_a<R, false, false>(grid, s, a, j, k, act, ka);
-                    }
-                    // w and Vt_i (i <= j) read, w written
-                    const double pb = h16 ? 24.0 + 8.0 * j : 2 * S * (3 + j);
-                    KTimer& kb = c->kt[KT_SAP_B];
-                    act = kb.begin(s, 0.0, g.sspb * pb);
-                    if (act) {
-                        if (h16)
-                            hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, kb.start(),
-                                                  kb.stop(), 0, a, j, act);
-                        else
-                            hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, kb.start(),
-                                                  kb.stop(), 0, a, j, act);
-                    } else if (h16) {
-                        hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
-                    } else {
-                        hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
-                    }
+                if (h16) {
+                    if (sq) hipLaunchKernelGGL((k_sap_init<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
+                    else hipLaunchKernelGGL((k_sap_init<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
+                } else {
+                    if (sq) hipLaunchKernelGGL((k_sap_init<R, false, true>), grid, dim3(FT), 0, s, a);
+                    else hipLaunchKernelGGL((k_sap_init<R, false, false>), grid, dim3(FT), 0, s, a);
                 }
-                const bool last = k + 1 == c->P.sap_restarts;
-                cplx<R>* xacc = last ? x : (cplx<R>*)nullptr;
-                if (h16) hipLaunchKernelGGL((k_sap_update<R, kH16<R>>), grid, dim3(FT), 0, s, a, xacc);
-                else hipLaunchKernelGGL((k_sap_update<R, false>), grid, dim3(FT), 0, s, a, xacc);
-                if (!last) {
-                    if (h16) {
-                        if (sq) hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
-                        else hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
-                    } else {
-                        if (sq) hipLaunchKernelGGL((k_sap_resid<R, false, true>), grid, dim3(FT), 0, s, a);
-                        else hipLaunchKernelGGL((k_sap_resid<R, false, false>), grid, dim3(FT), 0, s, a);
+                for (int k = 0; k < c->P.sap_restarts; k++) {
+                    for (int j = 0; j < a.m; j++) {
+                        // algorithmic bytes: links of the colour's sites once; per active
+                        // item Vt_j read, w written, Vt_i (i < j) read for the dots.
+                        // With FP16 slots: w and every slot FP16.
+                        const double S = sizeof(cplx<R>), sites = (double)g.ncol * g.sspb;
+                        const double pa = h16 ? 16.0 + 8.0 * j : 2 * S * (2 + j);
+                        KTimer& ka = c->kt[KT_SAP_A];
+                        int* act = ka.begin(s, sites * 2 * S, g.sspb * pa, j);
+                        if (h16) {
+                            if (sq) sap_arnoldi_a<R, kH16<R>, true>(grid, s, a, j, k, act, ka);
+                            else sap_arnoldi_a<R, kH16<R>, false>(grid, s, a, j, k, act, ka);
+                        } else {
+                            if (sq) sap_arnoldi_a<R, false, true>(grid, s, a, j, k, act, ka);
+                            else sap_arnoldi_a<R, false, false>(grid, s, a, j, k, act, ka);
+                        }
+                        // w and Vt_i (i <= j) read, w written
+                        const double pb = h16 ? 24.0 + 8.0 * j : 2 * S * (3 + j);
+                        KTimer& kb = c->kt[KT_SAP_B];
+                        act = kb.begin(s, 0.0, g.sspb * pb);
+                        if (act) {
+                            if (h16)
+                                hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, kb.start(),
+                                                      kb.stop(), 0, a, j, act);
+                            else
+                                hipExtLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, kb.start(),
+                                                      kb.stop(), 0, a, j, act);
+                        } else if (h16) {
+                            hipLaunchKernelGGL((k_sap_arnoldi_b<R, kH16<R>>), grid, dim3(FT), 0, s, a, j, act);
+                        } else {
+                            hipLaunchKernelGGL((k_sap_arnoldi_b<R, false>), grid, dim3(FT), 0, s, a, j, act);
+                        }
+                    }
+                    const bool last = k + 1 == c->P.sap_restarts;
+                    cplx<R>* xacc = last ? x : (cplx<R>*)nullptr;
+                    if (h16) hipLaunchKernelGGL((k_sap_update<R, kH16<R>>), grid, dim3(FT), 0, s, a, xacc);
+                    else hipLaunchKernelGGL((k_sap_update<R, false>), grid, dim3(FT), 0, s, a, xacc);
+                    if (!last) {
+                        if (h16) {
+                            if (sq) hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, true>), grid, dim3(FT), 0, s, a);
+                            else hipLaunchKernelGGL((k_sap_resid<R, kH16<R>, false>), grid, dim3(FT), 0, s, a);
+                        } else {
+                            if (sq) hipLaunchKernelGGL((k_sap_resid<R, false, true>), grid, dim3(FT), 0, s, a);
+                            else hipLaunchKernelGGL((k_sap_resid<R, false, false>), grid, dim3(FT), 0, s, a);
+                        }
                     }
                 }
             }
@@ -454,19 +550,23 @@ template <typename R> void Engine<R>::kcycle0(int nrhs, const cplx<R>* eta, cplx
     coarse(c->P.cycle == 0 ? CM_CYCLE : CM_KC_SOLVE, 1, nrhs, eta1.as<cplx<R>>(), n1, psi1.as<cplx<R>>(), n1);
     if (tvcs[0] != 0) {  // one test-vector set per configuration
         hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc, tvcs[0]);
+                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc, tvcs[0], rmask);
     } else if constexpr (std::is_same<R, float>::value) {
         const dim3 gp(ntile, cdiv_u(nrhs, 8));
         const int nv = g.ntest, dofc = c->geo[1].dof, nsc = c->geo[1].ns;
         const c32* pc = (const c32*)psi1.as<c32>();
-        if (nv <= 8) hipLaunchKernelGGL(k_prolong0_g<8>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
-        else if (nv <= 12) hipLaunchKernelGGL(k_prolong0_g<12>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
-        else if (nv <= 16) hipLaunchKernelGGL(k_prolong0_g<16>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc);
-        else hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                                (const cplx<R>*)psi1.as<cplx<R>>(), psi, dofc, nsc, acc);
+        if (nv <= 8)
+            hipLaunchKernelGGL(k_prolong0_g<8>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc, rmask);
+        else if (nv <= 12)
+            hipLaunchKernelGGL(k_prolong0_g<12>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc, rmask);
+        else if (nv <= 16)
+            hipLaunchKernelGGL(k_prolong0_g<16>, gp, dim3(FT), 0, s, g, nv, tv[0], pc, psi, dofc, nsc, nrhs, acc, rmask);
+        else
+            hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
+                               (const cplx<R>*)psi1.as<cplx<R>>(), psi, dofc, nsc, acc, (size_t)0, rmask);
     } else {
         hipLaunchKernelGGL(k_prolong0<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, g.ntest, tv[0],
-                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc);
+                           (const cplx<R>*)psi1.as<cplx<R>>(), psi, c->geo[1].dof, c->geo[1].ns, acc, (size_t)0, rmask);
     }
     if (c->P.nu2 > 0) sap0(nrhs, eta, psi, c->P.nu2);
     HIP_CHECK(hipGetLastError());
@@ -718,9 +818,6 @@ static void outer_alloc(Ctx& c, int cap) {
     if (c.mixed()) {
         c.oeta32.alloc(sizeof(c32) * cap * n0);
         c.opsi32.alloc(sizeof(c32) * cap * n0);
-    } else {
-        c.oeta64.alloc(sizeof(c64) * cap * n0);
-        c.opsi64.alloc(sizeof(c64) * cap * n0);
     }
     const int ntile = (int)cdiv_u(c.geo[0].ns, FT);
     // per-tile partial sums of the final true residual check
@@ -762,58 +859,11 @@ static void outer_alloc(Ctx& c, int cap) {
     s.hcap = hcap;
 }
 
-// preconditioner Z_j = K(V_j) for all rhs of the batch
-// The multigrid cycle at level 0 on the fixed staging buffers (oeta32 ->
-// opsi32 in mixed precision, oeta64 -> opsi64 in FP64): the part of the
-// preconditioner that is captured into a HIP graph.
-static void precond_body(Ctx& c, int nrhs, c32* out32) {
-    if (c.mixed()) c.e32.kcycle0(nrhs, c.oeta32.as<c32>(), out32 ? out32 : c.opsi32.as<c32>());
-    else c.e64.kcycle0(nrhs, c.oeta64.as<c64>(), c.opsi64.as<c64>());
-}
-
-// Executable graph of precond_body for this batch size and output (captured
-// on first use; the outer FGMRES's buffers are fixed for the context's life).
-static hipGraphExec_t precond_graph(Ctx& c, int nrhs, c32* out32) {
-    for (auto& g : c.pgraphs)
-        if (g.nrhs == nrhs && g.out == out32) return g.exec;
-    hipGraph_t graph = nullptr;
-    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    try {
-        precond_body(c, nrhs, out32);
-    } catch (...) {
-        if (hipStreamEndCapture(c.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
-        throw;
-    }
-    HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
-    hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIP_CHECK(e);
-    c.pgraphs.push_back({nrhs, out32, exec});
-    return exec;
-}
-
-// eta32_ready: the FP32 staging input already holds vj (written by the
-// kernel that produced vj); zj32: in the mixed modes, the FP32 Z_j that
-// receives the result instead of zj
+// Z_j = K(V_j) for all rhs of the batch.  eta32_ready: the FP32 staging
+// input already holds V_j (written by the kernel that produced it); zj32: in
+// the mixed modes, the FP32 Z_j that receives the result instead of zj.
 static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_ready, c32* zj32) {
     const size_t n0 = c.nvec(0), tot = nrhs * n0;
-    // graph replay (kernel timers record eager launches only)
-    if (c.graph && c.outer_prec == 0 && !c.kt[KT_SAP_A].on && !c.kt[KT_SAP_B].on && !c.kt[KT_COARSE].on) {
-        if (!c.mixed())
-            HIP_CHECK(hipMemcpyAsync(c.oeta64.p, vj, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
-        else if (!eta32_ready)
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, vj,
-                               c.oeta32.as<c32>());
-        // mixed modes: the graph of this Z_j writes it directly
-        HIP_CHECK(hipGraphLaunch(precond_graph(c, nrhs, c.mixed() ? zj32 : nullptr), c.stream));
-        if (!c.mixed())
-            HIP_CHECK(hipMemcpyAsync(zj, c.opsi64.p, sizeof(c64) * tot, hipMemcpyDeviceToDevice, c.stream));
-        else if (!zj32)
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot,
-                               (const c32*)c.opsi32.as<c32>(), zj);
-        return;
-    }
     if (c.outer_prec == 1) {
         // FGMRES_SAP::preconditioner (include/sap.h:212-224): zero, one SAP sweep
         hipLaunchKernelGGL(k_zero<double>, dim3(cdiv_u(tot, 256)), dim3(256), 0, c.stream, tot, zj);
@@ -836,6 +886,39 @@ static void outer_precond(Ctx& c, int nrhs, const c64* vj, c64* zj, bool eta32_r
         c.e64.kcycle0(nrhs, vj, zj);
     }
 }
+
+// Executable graph of one outer restart cycle for this batch size and outer
+// preconditioner, captured from `body` on first use (every buffer of the
+// outer solve is fixed for the context's life, and no kernel of the cycle
+// takes the restart index).
+template <typename F> static hipGraphExec_t cycle_graph(Ctx& c, int nrhs, F&& body) {
+    for (auto& g : c.pgraphs)
+        if (g.nrhs == nrhs && g.outer_prec == c.outer_prec) return g.exec;
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        if (hipStreamEndCapture(c.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(e);
+    c.pgraphs.push_back({nrhs, c.outer_prec, exec});
+    return exec;
+}
+
+// The outer solve's per-rhs active flags gate the K-cycle's work (SAP, the
+// coarse kernel, restriction and prolongation skip a converged rhs) for the
+// lifetime of the guard.
+struct RhsMask {
+    Ctx& c;
+    RhsMask(Ctx& ctx, const int* m) : c(ctx) { c.e64.rmask = c.e32.rmask = m; }
+    ~RhsMask() { c.e64.rmask = c.e32.rmask = nullptr; }
+};
 
 // The outer solve's per-rhs active flags on the host.  The copy goes to
 // pinned memory and the host spins on an event behind it: this wait happens
@@ -894,6 +977,47 @@ static void outer_solve(Ctx& c, int nrhs) {
     const bool progress = pe && pe[0] && pe[0] != '0';
     const auto tstart = std::chrono::steady_clock::now();
     int step = 0;
+    const RhsMask mask(c, st.active);
+    // Arnoldi step j (src/fgmres.cpp:20-49): Z_j = K(V_j), w = D Z_j, its
+    // Hessenberg column and Givens rotation, the break test, V_{j+1}.  Every
+    // kernel skips an rhs whose flag the break test cleared.
+    auto arnoldi_step = [&](int j) {
+        outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride, eta32,
+                      Z32 ? Z32 + (size_t)j * vstride : nullptr);
+        if (Z32)
+            hipLaunchKernelGGL(k_outer_a<float>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
+                               (const c64*)c.U64.as<c64>(), (const c32*)(Z32 + (size_t)j * vstride), (const c64*)V, w,
+                               st, j, vstride);
+        else
+            hipLaunchKernelGGL(k_outer_a<double>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
+                               (const c64*)c.U64.as<c64>(), (const c64*)(Z + (size_t)j * vstride), (const c64*)V, w,
+                               st, j, vstride);
+        hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
+        hipLaunchKernelGGL(k_outer_b, g2, dim3(FT), 0, s, g.ns, (const c64*)V, w, st, j, vstride);
+        hipLaunchKernelGGL(k_outer_fin, dim3(nrhs), dim3(FT), 0, s, st, j, tol);
+        if (j + 1 < M)
+            hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)w, V + (size_t)(j + 1) * vstride,
+                               st, j + 1, eta32 ? oeta32 : nullptr);
+    };
+    auto cycle_start = [&]() {
+        hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0, eta32 ? oeta32 : nullptr);
+    };
+    // end of a cycle: y from the triangular system, x += Z y, r = b - D x
+    // (src/fgmres.cpp:52-59, 61-65)
+    auto cycle_end = [&]() {
+        hipLaunchKernelGGL(k_outer_backsolve, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs);
+        if (Z32) hipLaunchKernelGGL(k_outer_update<float>, g2, dim3(FT), 0, s, g.ns, (const c32*)Z32, x, st, vstride);
+        else hipLaunchKernelGGL(k_outer_update<double>, g2, dim3(FT), 0, s, g.ns, (const c64*)Z, x, st, vstride);
+        hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
+                           (const c64*)x, r, st);
+    };
+    // graph mode: a whole restart cycle is one graph launch and the host
+    // reads the flags once per cycle; a batch that converges inside a cycle
+    // runs its remaining steps as launches that skip every rhs.  Eager
+    // launches (and the per-launch kernel timers, which resolve after every
+    // step) read them after every step and leave the cycle early.
+    const bool timers = std::any_of(std::begin(c.kt), std::end(c.kt), [](const KTimer& kt) { return kt.on; });
+    const bool whole_cycle = c.graph && !timers && !verbose;
     hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
                        (const c64*)x, r, st);
     for (int k = 0; k < c.P.out_restarts; k++) {
@@ -908,47 +1032,41 @@ static void outer_solve(Ctx& c, int nrhs) {
             std::fflush(stderr);
         }
         if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
-        hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)r, V, st, 0, eta32 ? oeta32 : nullptr);
-        for (int j = 0; j < M; j++) {
-            outer_precond(c, nrhs, V + (size_t)j * vstride, Z + (size_t)j * vstride, eta32,
-                          Z32 ? Z32 + (size_t)j * vstride : nullptr);
-            if (Z32)
-                hipLaunchKernelGGL(k_outer_a<float>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
-                                   (const c64*)c.U64.as<c64>(), (const c32*)(Z32 + (size_t)j * vstride),
-                                   (const c64*)V, w, st, j, vstride);
-            else
-                hipLaunchKernelGGL(k_outer_a<double>, dim3(st.ntile_a, nrhs), dim3(FT), 0, s, g, mm,
-                                   (const c64*)c.U64.as<c64>(), (const c64*)(Z + (size_t)j * vstride),
-                                   (const c64*)V, w, st, j, vstride);
-            hipLaunchKernelGGL(k_outer_hcol, dim3(nrhs), dim3(FT), 0, s, st, j);
-            hipLaunchKernelGGL(k_outer_b, g2, dim3(FT), 0, s, g.ns, (const c64*)V, w, st, j, vstride);
-            hipLaunchKernelGGL(k_outer_fin, dim3(nrhs), dim3(FT), 0, s, st, j, tol);
-            if (j + 1 < M)
-                hipLaunchKernelGGL(k_outer_scale, g2, dim3(FT), 0, s, g.ns, (const c64*)w, V + (size_t)(j + 1) * vstride,
-                                   st, j + 1, eta32 ? oeta32 : nullptr);
-            read_flags(c, st.active, nrhs, flags);
-            for (KTimer& kt : c.kt) kt.resolve(s);
-            step++;
-            if (verbose) {
-                int nh = 0;
-                double h = 0, nb = 0;
-                HIP_CHECK(hipMemcpy(&nh, st.nhist, sizeof(int), hipMemcpyDeviceToHost));
-                if (nh > 0 && nh <= st.hcap) HIP_CHECK(hipMemcpy(&h, st.hist + nh - 1, sizeof(double), hipMemcpyDeviceToHost));
-                HIP_CHECK(hipMemcpy(&nb, st.normb, sizeof(double), hipMemcpyDeviceToHost));
-                const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
-                int act = 0;
-                for (int f : flags) act += f != 0;
-                std::fprintf(stderr, "[amgs] cycle %d step %d (total %d): |g|/|b| rhs0 = %.3e, active %d/%d, %.1f ms\n", k, j,
-                             step, nb > 0 ? h / nb : 0.0, act, nrhs, el);
-                std::fflush(stderr);
+        if (whole_cycle) {
+            HIP_CHECK(hipGraphLaunch(cycle_graph(c, nrhs,
+                                                 [&]() {
+                                                     cycle_start();
+                                                     for (int j = 0; j < M; j++) arnoldi_step(j);
+                                                     cycle_end();
+                                                 }),
+                                     s));
+            step += M;
+        } else {
+            cycle_start();
+            for (int j = 0; j < M; j++) {
+                arnoldi_step(j);
+                read_flags(c, st.active, nrhs, flags);
+                for (KTimer& kt : c.kt) kt.resolve(s);
+                step++;
+                if (verbose) {
+                    int nh = 0;
+                    double h = 0, nb = 0;
+                    HIP_CHECK(hipMemcpy(&nh, st.nhist, sizeof(int), hipMemcpyDeviceToHost));
+                    if (nh > 0 && nh <= st.hcap)
+                        HIP_CHECK(hipMemcpy(&h, st.hist + nh - 1, sizeof(double), hipMemcpyDeviceToHost));
+                    HIP_CHECK(hipMemcpy(&nb, st.normb, sizeof(double), hipMemcpyDeviceToHost));
+                    const double el =
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+                    int act = 0;
+                    for (int f : flags) act += f != 0;
+                    std::fprintf(stderr, "[amgs] cycle %d step %d (total %d): |g|/|b| rhs0 = %.3e, active %d/%d, %.1f ms\n",
+                                 k, j, step, nb > 0 ? h / nb : 0.0, act, nrhs, el);
+                    std::fflush(stderr);
+                }
+                if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
             }
-            if (std::none_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) break;
+            cycle_end();
         }
-        hipLaunchKernelGGL(k_outer_backsolve, dim3(cdiv_u(nrhs, 64)), dim3(64), 0, s, st, nrhs);
-        if (Z32) hipLaunchKernelGGL(k_outer_update<float>, g2, dim3(FT), 0, s, g.ns, (const c32*)Z32, x, st, vstride);
-        else hipLaunchKernelGGL(k_outer_update<double>, g2, dim3(FT), 0, s, g.ns, (const c64*)Z, x, st, vstride);
-        hipLaunchKernelGGL(k_outer_resid, g2, dim3(FT), 0, s, g, mm, (const c64*)c.U64.as<c64>(), (const c64*)b,
-                           (const c64*)x, r, st);
         if (k + 1 == c.P.out_restarts)
             hipLaunchKernelGGL(k_outer_start, dim3(nrhs), dim3(FT), 0, s, st, k + 1, tol, M);
     }
@@ -1307,6 +1425,7 @@ template <typename R> static int autotune_sap(Ctx& c, Engine<R>& e, int nrhs, do
                        f32 ? (c32*)v.p : nullptr, f32 ? nullptr : (c64*)v.p);
     const int maxc = (int)cdiv_u(g.sspb, FT);
     std::vector<int> cand;
+    if (sap_block_pp<R>(c) > 0) cand.push_back(1);  // one workgroup per item (k_sap_block)
     for (int t : {512, 1024, 1536, 2048, 2560, 3072, 4096}) {
         const int nch = sap_chunks(g, nrhs, t);
         bool dup = false;

@@ -73,6 +73,7 @@ template <typename R> struct CArgs {
     cplx<R>* xchg;
     size_t xchg_n, x_top;
     int* err;  // set when a member barrier waits beyond its bound
+    const int* rmask;  // per-rhs flags of the calling outer solve (0: skip that rhs); null: none
     // algorithmic stencil flops of this launch (member 0 of each rhs adds
     // them), or null: 8 flops per complex multiply-add of every level
     // operator and Schwarz-block stencil applied
@@ -1229,6 +1230,7 @@ template <typename R> __device__ __forceinline__ void c_run(const CArgs<R>& A, C
 // pointer (scalar loads; a copy of the parameter would land in scratch).
 template <typename R> __device__ __forceinline__ void coarse_main(const CArgs<R>& A, CLds& S, char* dyn_lds) {
     const int rhs = blockIdx.x / A.grp, member = blockIdx.x - rhs * A.grp;
+    if (A.rmask && !A.rmask[rhs]) return;  // the whole group skips a finished rhs
     CK<R> K{A, A.ws + (size_t)blockIdx.x * A.ws_per_rhs, S, reinterpret_cast<cplx<R>*>(dyn_lds),
             A.prof && member == 0 ? A.prof + (size_t)rhs * 32 : nullptr, rhs, member};
     PhaseTimer Ttot(K.prof);

@@ -23,6 +23,7 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
         a.wcs[l] = tvcs[l];
     }
     a.mmv = mmv;
+    a.rmask = rmask;
     // one workgroup group per configuration: the arguments go through LDS
     const bool cfg = mmv != nullptr;
     a.mode = mode;

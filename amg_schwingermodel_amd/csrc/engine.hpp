@@ -141,7 +141,7 @@ struct KTimer {
         tag.clear();
     }
 };
-enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_COARSE = 2, KT_CLASSES = 3 };
+enum { KT_SAP_A = 0, KT_SAP_B = 1, KT_COARSE = 2, KT_SAP_BLOCK = 3, KT_CLASSES = 4 };
 
 // ---------------------------------------------------------------------------
 // per-precision engine: fine SAP, K-cycle(0), coarse kernel arguments
@@ -159,6 +159,9 @@ template <typename R> struct Engine {
     // links[l] (0 with one configuration), and m0 + 2 per configuration
     size_t ucs = 0, tvcs[MAXL] = {}, lkcs[MAXL] = {};
     const R* mmv = nullptr;
+    // per-rhs flags of the outer solve that calls the K-cycle (0: that rhs has
+    // converged, its preconditioner work is skipped); null outside it
+    const int* rmask = nullptr;
     // fine SAP workspace
     DevBuf Vt, Vh, xl, r, st_mem, skip, part_r, part_v, zero, rsig;
     SapState st{};
@@ -241,17 +244,15 @@ struct Ctx {
     int hflags_cap = 0;
     hipEvent_t evf = nullptr;
     KTimer kt[KT_CLASSES];
-    // HIP-graph replay of the level-0 preconditioner, one executable graph per
-    // batch size; FP64 inputs/outputs are staged through fixed buffers
+    // HIP-graph replay of the outer FGMRES restart cycle (m Arnoldi steps with
+    // their preconditioner applications, the update and the new residual):
+    // one executable graph per batch size
     bool graph = false;
-    // keyed by batch size and output buffer (mixed modes: one per FP32 Z_j)
     struct PGraph {
-        int nrhs;
-        const void* out;
+        int nrhs, outer_prec;
         hipGraphExec_t exec;
     };
     std::vector<PGraph> pgraphs;
-    DevBuf oeta64, opsi64;
     void clear_graphs() {
         for (auto& g : pgraphs) (void)hipGraphExecDestroy(g.exec);
         pgraphs.clear();

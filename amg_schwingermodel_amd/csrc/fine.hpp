@@ -562,6 +562,42 @@ __device__ void sap_finalize_step(const SapState& st, int item, double hn2, int 
     }
 }
 
+// Hessenberg column j of a block GMRES by modified Gram-Schmidt algebra on
+// the step's inner products of the stored vectors, L.dq: d[i] = <W, Vt_i>
+// (i <= j, W = D_B Vt_j unscaled), d[j+1+k] = <Vt_j, Vt_k> (k < j).  Sets the
+// Gram row j (G_jk = s_j s_k <Vt_j, Vt_k>), Hcol (H_ij = <w, V_i> - sum_{k<i}
+// H_kj G_ik with w = s_j W) and the coefficients eta_i = s_i H_ij on the Vt_i.
+__device__ __forceinline__ void sap_hessenberg_column(ItemLds& L, int M, int j) {
+    const c64* d = L.dq;
+    for (int kk = 0; kk < j; kk++) L.G[j * (M + 1) + kk] = (L.s[j] * L.s[kk]) * d[j + 1 + kk];
+    for (int i = 0; i <= j; i++) {
+        c64 h = (L.s[i] * L.s[j]) * d[i];
+        for (int kk = 0; kk < i; kk++) h -= L.Hcol[kk] * conj(L.G[i * (M + 1) + kk]);
+        L.Hcol[i] = h;
+    }
+    for (int i = 0; i <= j; i++) L.eta[i] = L.s[i] * L.Hcol[i];
+}
+
+// End of a cycle: y from the rotated triangular system (src/fgmres.cpp:52-59,
+// 107-115) over maxIt columns, as coefficients eta_i = s_i y_i on the Vt_i.
+__device__ __forceinline__ void sap_backsolve(ItemLds& L, int M, int maxIt) {
+    for (int i = maxIt - 1; i >= 0; i--) {
+        c64 e = L.gpost[i];
+        for (int q = i + 1; q < maxIt; q++) e -= L.R[i * M + q] * L.eta[q];
+        L.eta[i] = cdiv(e, L.R[i * M + i]);
+    }
+    for (int i = 0; i < maxIt; i++) L.eta[i] = L.s[i] * L.eta[i];
+}
+
+// FP16 storage: power-of-two scale of the next cycle's stored Vt_0 from the
+// cycle's final residual estimate spread over the block's 2 sspb entries
+// (entries of order 1); 0 when the estimate gives none.
+__device__ __forceinline__ double sap_next_sig0(const ItemLds& L, int maxIt, int sspb) {
+    const double est = sqrt(norm2(L.gpre[maxIt]));
+    const double want = sqrt(2.0 * sspb) / est;
+    return est > 0 && want < 1e30 ? exp2(rint(log2(want))) : 0.0;
+}
+
 // Wave-reduce v for quantity q and park it in tmp[wave][q] (no private arrays
 // with run-time indices).
 __device__ __forceinline__ void park(c64 v, int q, c64* tmp, int nqmax) {
@@ -757,16 +793,7 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT) k_sap_arnold
         const SapState& st = a.st;
         const int active = !L.done && L.jbreak < 0;
         if (active) {
-            const c64* d = L.dq;
-            // Gram row j: G_jk = s_j s_k <Vt_j, Vt_k>, k < j
-            for (int kk = 0; kk < j; kk++) L.G[j * (M + 1) + kk] = (L.s[j] * L.s[kk]) * d[j + 1 + kk];
-            // <w, V_i> with w = s_j W (step A stores W = D_B Vt_j unscaled)
-            for (int i = 0; i <= j; i++) {
-                c64 h = (L.s[i] * L.s[j]) * d[i];
-                for (int kk = 0; kk < i; kk++) h -= L.Hcol[kk] * conj(L.G[i * (M + 1) + kk]);
-                L.Hcol[i] = h;
-            }
-            for (int i = 0; i <= j; i++) L.eta[i] = L.s[i] * L.Hcol[i];  // coefficients on the Vt_i
+            sap_hessenberg_column(L, M, j);
             if (t.chunk == 0) {
                 c64* Hr = st.Hraw + (size_t)t.item * (M + 1) * M;
                 for (int i = 0; i <= j; i++) Hr[i * M + j] = L.Hcol[i];
@@ -875,18 +902,10 @@ template <typename R, bool H> __global__ void __launch_bounds__(FT, (H ? 8 : 1))
                 sap_finalize_step(a.st, t.item, L.dr[0], M, M - 1, a.tol, t.chunk == 0, L, H ? a.sig : 1.0);
                 if (L.broke) maxIt = M;
             }
-            for (int i = maxIt - 1; i >= 0; i--) {
-                c64 e = L.gpost[i];
-                for (int q = i + 1; q < maxIt; q++) e -= L.R[i * M + q] * L.eta[q];
-                L.eta[i] = cdiv(e, L.R[i * M + i]);
-            }
-            for (int i = 0; i < maxIt; i++) L.eta[i] = L.s[i] * L.eta[i];  // coefficients on the Vt_i
+            sap_backsolve(L, M, maxIt);
             if (H && t.chunk == 0) {
-                // scale of the next cycle's stored Vt_0: the residual estimate
-                // spread over the block's 2 sspb entries, brought to order 1
-                const double est = sqrt(norm2(L.gpre[maxIt]));
-                const double want = sqrt(2.0 * a.g.sspb) / est;
-                if (est > 0 && want < 1e30) a.st.sig0[t.item] = exp2(rint(log2(want)));
+                const double sg = sap_next_sig0(L, maxIt, a.g.sspb);
+                if (sg > 0) a.st.sig0[t.item] = sg;
             }
         }
         L.maxIt = maxIt;
@@ -1032,9 +1051,10 @@ __global__ void __launch_bounds__(FT) k_norm_part(int ns, const cplx<R>* __restr
 template <typename R, int NV>
 __global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* __restrict__ w,
                                                   const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
-                                                  int dofc, int nsc, size_t wcs = 0) {
+                                                  int dofc, int nsc, size_t wcs = 0, const int* rmask = nullptr) {
     __shared__ cplx<R> red[4 * 2 * NV];
     const int X = blockIdx.x, rhs = blockIdx.y;
+    if (rmask && !rmask[rhs]) return;
     w += rhs * wcs;
     const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
     cplx<R> acc[2 * NV];
@@ -1070,10 +1090,12 @@ __global__ void __launch_bounds__(FT) k_restrict0(Geo g, int nv, const cplx<R>* 
 template <typename R>
 __global__ void __launch_bounds__(FT) k_prolong0(Geo g, int nv, const cplx<R>* __restrict__ w,
                                                  const cplx<R>* __restrict__ vc, cplx<R>* __restrict__ out,
-                                                 int dofc, int nsc, int accumulate, size_t wcs = 0) {
+                                                 int dofc, int nsc, int accumulate, size_t wcs = 0,
+                                                 const int* rmask = nullptr) {
     const int n = blockIdx.x * FT + threadIdx.x;
     if (n >= g.ns) return;
     const int rhs = blockIdx.y;
+    if (rmask && !rmask[rhs]) return;
     w += rhs * wcs;
     const size_t off = (size_t)rhs * g.ns * 2, nvec = (size_t)g.ns * 2;
     const cplx<R>* c = vc + (size_t)rhs * nsc * dofc + (size_t)agg_block_of(g, n) * dofc;
@@ -1099,9 +1121,10 @@ __global__ void __launch_bounds__(FT) k_prolong0(Geo g, int nv, const cplx<R>* _
 template <int NV>
 __global__ void __launch_bounds__(FT) k_restrict0_g(Geo g, int nv, const c32* __restrict__ w,
                                                     const c32* __restrict__ v, c32* __restrict__ out, int dofc,
-                                                    int nsc, int nrhs) {
+                                                    int nsc, int nrhs, const int* __restrict__ rmask) {
     __shared__ c32 red[4 * 2 * 2 * NV];
     const int X = blockIdx.x, rA = 2 * blockIdx.y, rB = rA + 1 < nrhs ? rA + 1 : rA;
+    if (rmask && !rmask[rA] && !rmask[rB]) return;
     const size_t nvec = (size_t)g.ns * 2;
     c32 acc[2][2 * NV];
 #pragma unroll
@@ -1144,7 +1167,7 @@ __global__ void __launch_bounds__(FT) k_restrict0_g(Geo g, int nv, const c32* __
 template <int NV>
 __global__ void __launch_bounds__(FT) k_prolong0_g(Geo g, int nv, const c32* __restrict__ w,
                                                    const c32* __restrict__ vc, c32* __restrict__ out, int dofc,
-                                                   int nsc, int nrhs, int accumulate) {
+                                                   int nsc, int nrhs, int accumulate, const int* __restrict__ rmask) {
     const int n = blockIdx.x * FT + threadIdx.x;
     if (n >= g.ns) return;
     const size_t nvec = (size_t)g.ns * 2;
@@ -1158,6 +1181,7 @@ __global__ void __launch_bounds__(FT) k_prolong0_g(Geo g, int nv, const c32* __r
     const size_t X = (size_t)agg_block_of(g, n);
     const int r1 = min(nrhs, 8 * (int)blockIdx.y + 8);
     for (int rhs = 8 * blockIdx.y; rhs < r1; rhs++) {
+        if (rmask && !rmask[rhs]) continue;
         const c32* c = vc + (size_t)rhs * nsc * dofc + X * dofc;
         c32 o0 = czero<float>(), o1 = czero<float>();
 #pragma unroll
@@ -1233,11 +1257,12 @@ template <typename R> __global__ void k_zero(size_t n, cplx<R>* out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = czero<R>();
 }
-// int flags to zero (a kernel rather than a memset node, so that a captured
-// K-cycle replays correctly)
-__global__ void k_zero_int(int n, int* out) {
+// Per-rhs SAP skip flags at the start of a smoother application: 0, or 1
+// where the caller's mask marks the rhs as finished (a kernel rather than a
+// memset node, so that a captured K-cycle replays correctly)
+__global__ void k_skip_init(int n, const int* __restrict__ mask, int* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = 0;
+    if (i < n) out[i] = mask && mask[i] == 0 ? 1 : 0;
 }
 
 }  // namespace amgs

@@ -54,6 +54,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "time/RHS to 1e-10 residual, 512² 3-lvl: mixed+batched vs plain FP64 GPU"
+# kernel classes with per-launch timing: SAP step A, step B, the coarse
+# kernel, the block-per-workgroup SAP kernel
+KCLASSES = (0, 1, 2, 3)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS, FP64_PEAK_TFS = 157.3, 78.6
 UNEVEN_RESTARTS = 5  # outer FGMRES(20) restarts of the uneven batch (members below critical use all)
@@ -214,14 +217,15 @@ def main():
     ms_rhs = ms_step / a.rhs
     # roofline pass: the same batched solve with every launch of SAP step A
     # bracketed by HIP events on the library stream (eager launches)
-    for kc in (0, 1, 2):
+    for kc in KCLASSES:
         sm.kernel_timing(kc, True)
     ms_roof, _, _ = timed(sm, a.rhs, 1, 0, f"mixed {a.rhs} rhs, kernel timing")
     n_launch, k_ms, k_bytes = sm.kernel_stats(0)
     nb_launch, kb_ms, kb_bytes = sm.kernel_stats(1)
     nc_launch, kc_ms, _, kc_flops = sm.kernel_stats_ex(2)
+    ns_launch, ks_ms, ks_bytes = sm.kernel_stats(3)
     per_j = [sm.kernel_stats_ex(0, j) for j in range(5)]
-    for kc in (0, 1, 2):
+    for kc in KCLASSES:
         sm.kernel_timing(kc, False)
     sap_wgs, coarse_grp = sm.tuning(a.rhs)
     log(f"mixed batched: setup {setup_mixed:.0f} ms, solve {ms_step:.1f} ms for {a.rhs} rhs -> {ms_rhs:.2f} ms/rhs, "
@@ -287,34 +291,51 @@ def main():
     if "plain_fp64" in out:
         out["speedup_mixed_batched_vs_plain_fp64"] = round(out["plain_fp64"]["ms_per_rhs"] * a.rhs / ms_best, 2)
 
-    # ---- roofline: SAP Arnoldi step A over the kernel-timing solve ----------
-    avg_us = k_ms * 1e3 / max(n_launch, 1)
-    bytes_per_launch = k_bytes / max(n_launch, 1)
-    achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    # ---- roofline: the level-0 SAP over the kernel-timing solve ------------
     traffic = traffic_b = None
     tr_path = os.path.join(ROOT, "profiles", "sap_arnoldi_a_traffic.json")
-    if os.path.exists(tr_path):
-        tr = json.load(open(tr_path))
-        traffic = tr.get("hbm_bytes_per_launch")
-        traffic_b = tr.get("step_b", {}).get("hbm_bytes_per_launch")
-    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                       "kernel": "k_sap_arnoldi_a<float,J> (SAP block stencil + Gram dots, all J)",
-                       "launches": n_launch, "avg_launch_us": round(avg_us, 2),
-                       "algorithmic_bytes_per_launch": round(bytes_per_launch)}
-    # step B (orthogonalisation against the stored basis + norm) from the same
-    # timing solve, the second-largest level-0 kernel
-    achieved_b = kb_bytes / (kb_ms * 1e-3) / 1e9 if kb_ms > 0 else 0.0
-    out["roofline_sap_step_b"] = {"bound": "hbm", "achieved": round(achieved_b, 1), "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": round(achieved_b / HBM_PEAK_GBS, 4), "traffic": traffic_b,
-                                  "kernel": "k_sap_arnoldi_b<float,H> (W -= sum H_ij Vt_i, |W|^2, Vt_j+1 stored)",
-                                  "launches": nb_launch, "avg_launch_us": round(kb_ms * 1e3 / max(nb_launch, 1), 2),
-                                  "algorithmic_bytes_per_launch": round(kb_bytes / max(nb_launch, 1))}
-    out["roofline"]["per_J"] = [
-        {"J": j, "launches": nj, "avg_launch_us": round(msj * 1e3 / max(nj, 1), 2),
-         "algorithmic_bytes_per_launch": round(bj / max(nj, 1)),
-         "frac": round(bj / (msj * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if msj > 0 else None}
-        for j, (nj, msj, bj, _) in enumerate(per_j)]
+    if ns_launch > 0:
+        # one workgroup per (block, rhs) item: the colour's whole block GMRES
+        # per launch; bytes counted by the kernel (links once, per item and
+        # block site the slots, x_B and b it moves)
+        tr_path = os.path.join(ROOT, "profiles", "sap_block_traffic.json")
+        if os.path.exists(tr_path):
+            traffic = json.load(open(tr_path)).get("hbm_bytes_per_launch")
+        achieved = ks_bytes / (ks_ms * 1e-3) / 1e9 if ks_ms > 0 else 0.0
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "kernel": "k_sap_block<float,FP16> (one SAP colour: every block's GMRES(5) x 5 restarts, "
+                                     "one workgroup per block and rhs)",
+                           "launches": ns_launch, "avg_launch_us": round(ks_ms * 1e3 / max(ns_launch, 1), 2),
+                           "algorithmic_bytes_per_launch": round(ks_bytes / max(ns_launch, 1))}
+    else:
+        avg_us = k_ms * 1e3 / max(n_launch, 1)
+        bytes_per_launch = k_bytes / max(n_launch, 1)
+        achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+        if os.path.exists(tr_path):
+            tr = json.load(open(tr_path))
+            traffic = tr.get("hbm_bytes_per_launch")
+            traffic_b = tr.get("step_b", {}).get("hbm_bytes_per_launch")
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "kernel": "k_sap_arnoldi_a<float,J> (SAP block stencil + Gram dots, all J)",
+                           "launches": n_launch, "avg_launch_us": round(avg_us, 2),
+                           "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+        # step B (orthogonalisation against the stored basis + norm) from the
+        # same timing solve, the second-largest level-0 kernel
+        achieved_b = kb_bytes / (kb_ms * 1e-3) / 1e9 if kb_ms > 0 else 0.0
+        out["roofline_sap_step_b"] = {"bound": "hbm", "achieved": round(achieved_b, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(achieved_b / HBM_PEAK_GBS, 4),
+                                      "traffic": traffic_b,
+                                      "kernel": "k_sap_arnoldi_b<float,H> (W -= sum H_ij Vt_i, |W|^2, Vt_j+1 stored)",
+                                      "launches": nb_launch,
+                                      "avg_launch_us": round(kb_ms * 1e3 / max(nb_launch, 1), 2),
+                                      "algorithmic_bytes_per_launch": round(kb_bytes / max(nb_launch, 1))}
+        out["roofline"]["per_J"] = [
+            {"J": j, "launches": nj, "avg_launch_us": round(msj * 1e3 / max(nj, 1), 2),
+             "algorithmic_bytes_per_launch": round(bj / max(nj, 1)),
+             "frac": round(bj / (msj * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if msj > 0 else None}
+            for j, (nj, msj, bj, _) in enumerate(per_j)]
     # the persistent coarse-level kernel: latency-bound (one nested solve per
     # rhs); its counted stencil flops (8 per complex multiply-add of every level
     # operator and Schwarz-block stencil, once per rhs) over its device time

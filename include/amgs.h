@@ -125,12 +125,17 @@ int amgs_solve(amgs_ctx* c, int nrhs, const double* b, double* x, int* iters, do
                double* hist, int hist_cap, int* nhist);
 int amgs_solve_dev(amgs_ctx* c, int nrhs, const void* b_dev, void* x_dev, int* iters, double* relres);
 
-/* HIP-graph replay of the preconditioner (new): with enable != 0, later
- * solves capture the level-0 multigrid cycle (restriction, the persistent
- * coarse-level kernel, prolongation, SAP; several hundred launches) once per
- * batch size into a HIP graph and replay it in every outer iteration.  Results
- * are identical to the eager launches.  Graphs are rebuilt after a new set-up;
- * solves with kernel timing enabled (amgs_kernel_timing) launch eagerly. */
+/* HIP-graph replay of the outer restart cycle (new): with enable != 0, later
+ * solves capture one whole FGMRES restart cycle - its m Arnoldi steps with
+ * their multigrid preconditioner applications (restriction, the persistent
+ * coarse-level kernel, prolongation, SAP; several hundred launches each), the
+ * update and the new residual - once per batch size into a HIP graph and
+ * replay it in every cycle; the host waits once per cycle, never inside one.
+ * The break test stays on the device: a right-hand side that converges inside
+ * a cycle is skipped by every later kernel.  Results are identical to the
+ * eager launches (which read the flags after every Arnoldi step and leave the
+ * cycle early).  Graphs are rebuilt after a new set-up; solves with kernel
+ * timing enabled (amgs_kernel_timing) launch eagerly. */
 int amgs_set_graph(amgs_ctx* c, int enable);
 
 /* Autotuner (new): times one level-0 SAP sweep for a set of launch shapes

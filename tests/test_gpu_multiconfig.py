@@ -39,21 +39,35 @@ def cvec(n, seed):
     return r.standard_normal(n) + 1j * r.standard_normal(n)
 
 
+GEOMS = {
+    # chunked level-0 SAP launches (8x8-site Schwarz blocks)
+    "32": (32, [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)]),
+    # one workgroup per (32x32-site block, rhs): per-configuration links and masses inside it
+    "64_blockwise": (64, [(0, 4, 4, 4, 2, 2), (1, 2, 2, 4, 2, 2)]),
+}
+
+
+@pytest.mark.parametrize("geom", list(GEOMS))
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED16])
-def test_members_equal_their_own_contexts(prec):
-    nx = nt = 32
-    rows = [(0, 4, 4, 4, 4, 4), (1, 2, 2, 4, 2, 2)]
+def test_members_equal_their_own_contexts(prec, geom):
+    nx = nt = GEOMS[geom][0]
+    rows = GEOMS[geom][1]
     cfgs = [(2.0, -0.10, 11), (3.0, -0.05, 12), (1.0, 0.02, 13)]
     Us = [amgs.gauge_metropolis(nx, nt, beta, 30, seed)[0] for beta, _, seed in cfgs]
     m0s = [m0 for _, m0, _ in cfgs]
     B = np.stack([cvec(nx * nt * 2, 800 + i) for i in range(3)])
     sm = amgs.Solver(amgs.default_params(nx, nt, rows, precision=prec), np.stack(Us), m0s=m0s)
     assert amgs.lib().amgs_num_configs(sm.h) == 3
+    blockwise = geom == "64_blockwise"
+    if blockwise:
+        sm.set_tuning(3, sap_wgs=1)
     sm.setup()
     X, its, rrs, hists = sm.solve(B)
     Dm = sm.apply_D(0, B, nrhs=3).reshape(3, -1)
     for i, (U, m0) in enumerate(zip(Us, m0s)):
         s1 = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec), U)
+        if blockwise:
+            s1.set_tuning(1, sap_wgs=1)
         s1.setup()
         x1, it1, rr1, h1 = s1.solve(B[i])
         assert np.array_equal(Dm[i], s1.apply_D(0, B[i]))

@@ -442,9 +442,10 @@ def test_solve_multi_uneven_configurations():
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
 @pytest.mark.parametrize("case", ["32_3lvl", "16x32_3lvl"])
 def test_graph_replay_matches_eager(case, prec):
-    """The level-0 preconditioner replayed from a captured HIP graph gives the
-    eager result bit for bit, over many replays (every outer iteration of
-    several solves), with eager solves in between and for two batch sizes."""
+    """Whole outer restart cycles replayed from captured HIP graphs (the host
+    waits once per cycle, the break test stays on the device) give the eager
+    result bit for bit, over many replays, with eager solves in between and
+    for two batch sizes."""
     s, o, _ = make(case, precision=prec, max_rhs=2)
     B = np.stack([cvec(s.size(0), 400 + i) for i in range(2)])
     xe1, ite1, rre1, _ = s.solve(B[0])
@@ -464,10 +465,13 @@ def test_graph_replay_matches_eager(case, prec):
 
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
 def test_graph_replay_across_restarts(prec):
-    """FGMRES(4): several outer restarts, so every Z_j slot's captured graph is
-    replayed in later cycles, and a 2-rhs batch whose members converge at
-    different steps (one finished, or idle for the rest of a cycle, while the
-    other goes on): bit for bit the eager solve, against the oracle."""
+    """FGMRES(4): several outer restarts, so the captured cycle graph is
+    replayed cycle after cycle, and a 2-rhs batch whose members converge at
+    different steps (one finished, its kernels skipping it for the rest of a
+    cycle and in later cycles, while the other goes on): bit for bit the eager
+    solve (which leaves a cycle when both are done), against the oracle.  The
+    comparison solvers' FGMRES with an SAP-sweep or no preconditioner, graphed
+    in the same context, equal their eager runs too."""
     s, o, _ = make("32_3lvl", common=dict(out_m=4), precision=prec, max_rhs=2)
     # a prolongated coarse vector and its image under D: the oracle needs 11
     # and 9 outer iterations for them (cycles of 4: they stop at different steps)
@@ -488,6 +492,16 @@ def test_graph_replay_across_restarts(prec):
             assert abs(int(ite[i]) - ito) <= 1 and rel(xe[i], xo) < 1e-9
         else:
             assert int(ite[i]) <= ito + 3 and rel(xe[i], xo) < 1e-8
+    # other outer preconditioners get graphs of their own
+    for solver in (s.fgmres_sap, s.gmres):
+        s.set_graph(True)
+        rg = solver(B[1])
+        s.set_graph(False)
+        re_ = solver(B[1])
+        assert rg[1] == re_[1] and np.array_equal(rg[0], re_[0]), solver
+    s.set_graph(True)
+    xg, itg, _, _ = s.solve(B)
+    assert np.array_equal(itg, ite) and np.array_equal(xg, xe)
 
 
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED16])
@@ -662,6 +676,55 @@ def test_full_size_256_three_levels():
         sm.close()
 
 
+BLOCK_CASES = {
+    # 32x32-site Schwarz blocks (one pair of sites per thread), and 64x64 (two)
+    "64_32blk": (64, 64, [(0, 4, 4, 6, 2, 2)], -0.10, 2.0),
+    "128_64blk_3lvl": (128, 128, [(0, 8, 8, 6, 2, 2), (1, 2, 2, 6, 2, 2)], -0.10, 2.0),
+}
+
+
+@pytest.mark.parametrize("case", list(BLOCK_CASES))
+@pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
+def test_sap_block_form_matches_chunked_form(case, prec):
+    """The level-0 SAP with one workgroup per (Schwarz block, rhs) item -- a
+    colour's whole block GMRES in one launch, the new Krylov vector in an LDS
+    tile, W in registers -- against the chunked launches and the oracle: the
+    FP64 smoother within 1e-12 of the chunked form and 1e-9 of the oracle;
+    full solves in each precision with both forms against the oracle."""
+    nx, nt, rows, m0, beta = BLOCK_CASES[case]
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 77)
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=prec, max_rhs=2), U)
+    s.setup()
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    o.setup(draw_order=1)
+    B = np.stack([cvec(s.size(0), 700 + i) for i in range(2)])
+    v, x0 = cvec(s.size(0), 710), cvec(s.size(0), 711)
+    xo, _ = o.sap(0, v, x0, 2)
+    res = {}
+    for wgs in (1, 2048):  # 1: the block form
+        for n in (1, 2):
+            s.set_tuning(n, sap_wgs=wgs)
+        s.kernel_timing(3, True)
+        xs = s.sap(0, v, x0, 2)
+        X, its, rrs, _ = s.solve(B)
+        launches = s.kernel_stats(3)[0]
+        s.kernel_timing(3, False)
+        assert (launches > 0) == (wgs == 1), (wgs, launches)
+        assert rel(xs, xo) < 1e-9, wgs
+        res[wgs] = (xs, X, its)
+        for i in range(2):
+            xoi, ito, _, _ = o.solve(B[i])
+            assert rrs[i] < 1e-10
+            if prec == amgs.PREC_FP64:
+                assert abs(int(its[i]) - ito) <= 1 and rel(X[i], xoi) < 1e-9, (wgs, i)
+            else:
+                assert int(its[i]) <= ito + 3 and rel(X[i], xoi) < 1e-8, (wgs, i)
+    assert rel(res[1][0], res[2048][0]) < 1e-12
+    if prec == amgs.PREC_FP64:
+        assert np.array_equal(res[1][2], res[2048][2]) and rel(res[1][1], res[2048][1]) < 1e-11
+    s.close()
+
+
 def test_autotune_keeps_results_deterministic():
     """The SAP launch-shape autotuner picks one of its candidate workgroup
     targets; afterwards solves converge as before and repeat bit for bit."""
@@ -669,7 +732,8 @@ def test_autotune_keeps_results_deterministic():
     B = np.stack([cvec(s.size(0), 500 + i) for i in range(2)])
     x0, it0, rr0, _ = s.solve(B)
     best, ms = s.autotune(2)
-    assert best in (512, 1024, 1536, 2048, 2560, 3072, 4096) and ms > 0
+    # 1: one workgroup per (block, rhs) item (its own kernel where the geometry allows)
+    assert best in (1, 512, 1024, 1536, 2048, 2560, 3072, 4096) and ms > 0
     x1, it1, rr1, _ = s.solve(B)
     x2, it2, rr2, _ = s.solve(B)
     assert np.array_equal(x1, x2) and np.array_equal(it1, it2)
