@@ -89,6 +89,7 @@ SIGNATURES = {
     "amgs_apply_P": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_apply_Pt": ([_V, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_sap": ([_V, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_int], ctypes.c_int),
+    "amgs_sap_inner": ([_V, ctypes.c_int, _P, _P, ctypes.c_int], ctypes.c_int),
     "amgs_kcycle": ([_V, ctypes.c_int, _P, _P], ctypes.c_int),
     "amgs_sap_dev_noalloc": ([_V, ctypes.c_int, ctypes.c_int, _V, _V, ctypes.c_int], ctypes.c_int),
     "amgs_get_tv": ([_V, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
@@ -355,6 +356,14 @@ class Solver:
         v = _c(v).reshape(-1)
         x = _c(x).reshape(-1).copy()
         self._chk(lib().amgs_sap(self.h, level, nrhs, _p(v), _p(x), nu), "amgs_sap")
+        return x
+
+    def sap_inner(self, v, x, nu, nrhs=1):
+        """Level-0 SAP in the inner precision of a mixed context (FP32, FP16
+        Krylov basis under PREC_MIXED16), in the launch form the tuning selects."""
+        v = _c(v).reshape(-1)
+        x = _c(x).reshape(-1).copy()
+        self._chk(lib().amgs_sap_inner(self.h, nrhs, _p(v), _p(x), nu), "amgs_sap_inner")
         return x
 
     def kcycle(self, eta, nrhs=1):

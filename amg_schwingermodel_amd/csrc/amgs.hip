@@ -36,6 +36,7 @@
 #include "mfma.hpp"
 #include "outer.hpp"
 #include "sap_block.hpp"
+#include "sap_run.hpp"
 #include "setup.hpp"
 
 namespace amgs {
@@ -308,22 +309,88 @@ template <typename R> static int sap_block_pp(const Ctx& c) {
     return 0;
 }
 
+// The streaming level-0 SAP with one workgroup per item (k_sap_run): rows per
+// thread (2, 4, 8 or 16, the fewest that keep a workgroup within SRT threads)
+// for this context's geometry, 0 where the form does not apply (FP64 or FP32
+// basis, non-square lattice, blocks spanning a direction, odd t extent, more
+// than 64 pairs per block row or a row width not dividing 64, fewer than 64
+// threads, a block GMRES restart length above SR_MAXM).  AMGS_SAP_RUN=0
+// switches it off.
+template <typename R> static int sap_run_rw(const Ctx& c) {
+    static const bool off = [] {
+        const char* e = std::getenv("AMGS_SAP_RUN");
+        return e && e[0] == '0';
+    }();
+    const Geo& g = c.geo[0];
+    if (off || !std::is_same<R, float>::value || !c.h16() || g.nx != g.nt || g.stel % 2 || g.stel >= g.nt ||
+        g.sxel >= g.nx || c.P.sap_m > SR_MAXM)
+        return 0;
+    const int TL = g.stel / 2;
+    if (TL > 64 || 64 % TL) return 0;
+    for (int rw = 2; rw <= 16; rw *= 2) {
+        if (g.sxel % rw) continue;
+        const long long nth = (long long)TL * (g.sxel / rw);
+        if (nth <= SRT && nth % 64 == 0) return rw;
+    }
+    return 0;
+}
+
 // workgroup target of the level-0 SAP launches for this batch size: the
 // autotuned (amgs_autotune) or pinned (amgs_set_tuning) value if there is
-// one, else the default.  A target of 1 is one workgroup per item: the
-// block-per-workgroup kernel where it applies (the default there).
+// one, else the default.  Targets 1 and 2 are one workgroup per item: 1 the
+// block-tile kernel (k_sap_block), 2 the streaming kernel (k_sap_run), each
+// where it applies; larger targets split items into chunks.
 template <typename R> int Engine<R>::sap_wgs(int nrhs) const {
     for (const auto& t : sap_tuned)
         if (t.first == nrhs) return t.second;
-    // default: the block form when its workgroups fill the chip (at least
-    // 64 Ki threads) or when the chunked form could not split a block either
+    // default: a one-workgroup-per-item form when its workgroups fill the chip
+    // (at least 64 Ki threads) or when the chunked form could not split a
+    // block either; the streaming form first
+    const Geo& g = c->geo[0];
+    const int rw = sap_run_rw<R>(*c);
+    if (rw > 0) {
+        const long long threads = (long long)g.ncol * nrhs * (g.stel / 2) * (g.sxel / rw);
+        if (threads >= 65536 || g.sspb <= FT) return 2;
+    }
     const int pp = sap_block_pp<R>(*c);
     if (pp > 0) {
-        const Geo& g = c->geo[0];
         const long long threads = (long long)g.ncol * nrhs * (g.sspb / (2 * pp));
         if (threads >= 65536 || g.sspb <= FT) return 1;
     }
     return sap_default_wgs();
+}
+
+template <int RW>
+static void sap_run_launch(const Ctx& c, int nrhs, const SapArgs<float>& a, c32* x, int* act, KTimer& kt) {
+    const Geo& g = c.geo[0];
+    const int nth = (g.stel / 2) * (g.sxel / RW);
+    const size_t dyn = (size_t)nth * SR_LDS_PER_THREAD;
+    static bool attr = false;
+    if (!attr) {
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_sap_run<RW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(SRT * SR_LDS_PER_THREAD)));
+        attr = true;
+    }
+    const dim3 grid(g.ncol, nrhs), block(nth);
+    if (act)
+        hipExtLaunchKernelGGL((k_sap_run<RW>), grid, block, dyn, c.stream, kt.start(), kt.stop(), 0, a, x,
+                              c.P.sap_restarts, act);
+    else
+        hipLaunchKernelGGL((k_sap_run<RW>), grid, block, dyn, c.stream, a, x, c.P.sap_restarts, act);
+}
+
+// one colour of one sweep by k_sap_run (rw: sap_run_rw)
+template <typename R>
+static void sap_run_colour(const Ctx& c, int nrhs, int rw, const SapArgs<R>& a, cplx<R>* x, int* act, KTimer& kt) {
+    if constexpr (std::is_same<R, float>::value) {
+        switch (rw) {
+            case 2: sap_run_launch<2>(c, nrhs, a, x, act, kt); return;
+            case 4: sap_run_launch<4>(c, nrhs, a, x, act, kt); return;
+            case 8: sap_run_launch<8>(c, nrhs, a, x, act, kt); return;
+            case 16: sap_run_launch<16>(c, nrhs, a, x, act, kt); return;
+        }
+    }
+    throw ApiError(AMGS_ERR_ARG, "SAP streaming kernel: no instance for this shape");
 }
 
 template <typename R, bool H, int PP>
@@ -407,6 +474,7 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     hipLaunchKernelGGL(k_resid<R>, dim3(ntile, nrhs), dim3(FT), 0, s, g, mm, U, v, x, r.as<cplx<R>>(),
                        (double*)nullptr, skip.as<int>(), mred, ucs, mmv);
     const int target = sap_wgs(nrhs), bpp = target == 1 ? sap_block_pp<R>(*c) : 0;
+    const int rrw = target == 2 ? sap_run_rw<R>(*c) : 0;
     const int nchunk = sap_chunks(g, nrhs, target), cs = (int)cdiv_u(cdiv_u(g.sspb, nchunk), FT) * FT;
     SapArgs<R> a{};
     a.g = g;
@@ -445,7 +513,13 @@ template <typename R> void Engine<R>::sap0(int nrhs, const cplx<R>* v, cplx<R>* 
     for (int it = 0; it < nu; it++) {
         for (int colour = 0; colour < 2; colour++) {
             a.blocks = (colour == 0 ? c->redl[0] : c->blackl[0]).as<int>();
-            if (bpp > 0) {
+            if (rrw > 0) {
+                // the colour's whole block solve, one workgroup per item,
+                // streaming over block rows; bytes counted by the kernel
+                KTimer& kb = c->kt[KT_SAP_BLOCK];
+                int* act = kb.begin(s, (double)g.ncol * g.sspb * 2 * sizeof(cplx<R>), (double)g.sspb);
+                sap_run_colour<R>(*c, nrhs, rrw, a, x, act, kb);
+            } else if (bpp > 0) {
                 // the colour's whole block solve, one workgroup per item;
                 // algorithmic bytes: links once, per item the bytes per block
                 // site that the kernel counts
@@ -1425,6 +1499,7 @@ template <typename R> static int autotune_sap(Ctx& c, Engine<R>& e, int nrhs, do
                        f32 ? (c32*)v.p : nullptr, f32 ? nullptr : (c64*)v.p);
     const int maxc = (int)cdiv_u(g.sspb, FT);
     std::vector<int> cand;
+    if (sap_run_rw<R>(c) > 0) cand.push_back(2);    // one workgroup per item, streaming (k_sap_run)
     if (sap_block_pp<R>(c) > 0) cand.push_back(1);  // one workgroup per item (k_sap_block)
     for (int t : {512, 1024, 1536, 2048, 2560, 3072, 4096}) {
         const int nch = sap_chunks(g, nrhs, t);
@@ -2093,6 +2168,34 @@ extern "C" int amgs_sap(amgs_ctx* ctx, int level, int nrhs, const double* v, dou
     return host_wrap(ctx, n, v, n, nullptr, [&](c64* a, c64* b) {
         if (level == 0) c.e64.sap0(nrhs, a, b, nu);
         else c.e64.coarse(CM_SAP, level, nrhs, a, c.nvec(level), b, c.nvec(level), nu);
+    }, x);
+    API_END
+}
+
+// amgs_sap at level 0 in the inner precision of a mixed context: the FP32
+// engine's smoother, with the FP16 Krylov basis under AMGS_PREC_MIXED16 (the
+// kernels the mixed solves run, in the form the tuning selects), on host
+// buffers converted to FP32 and back.
+extern "C" int amgs_sap_inner(amgs_ctx* ctx, int nrhs, const double* v, double* x, int nu) {
+    if (!ctx || !v || !x) return AMGS_ERR_ARG;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    single_config(c);
+    need(c.mixed(), AMGS_ERR_STATE, "amgs_sap_inner needs a mixed-precision context");
+    need(c.setup_done, AMGS_ERR_STATE, "amgs_setup first");
+    need(nrhs >= 1 && nrhs <= c.e32.cap, AMGS_ERR_ARG, "bad nrhs");
+    bind_engine(c, c.e32, c.U32, c.tv32, c.links32);
+    const size_t n = c.nvec(0) * nrhs;
+    DevBuf v32, x32;
+    v32.alloc(sizeof(c32) * n);
+    x32.alloc(sizeof(c32) * n);
+    return host_wrap(ctx, n, v, n, nullptr, [&](c64* a, c64* b) {
+        const dim3 grid(cdiv_u(n, 256));
+        hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, c.stream, n, (const c64*)a, v32.as<c32>());
+        hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, c.stream, n, (const c64*)b, x32.as<c32>());
+        c.e32.sap0(nrhs, v32.as<c32>(), x32.as<c32>(), nu);
+        hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(256), 0, c.stream, n, (const c32*)x32.as<c32>(), b);
+        HIP_CHECK(hipGetLastError());
     }, x);
     API_END
 }

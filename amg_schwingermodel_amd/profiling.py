@@ -1,21 +1,28 @@
-"""HBM traffic of the SAP Arnoldi steps A and B, from rocprofv3 hardware counters.
+"""HBM traffic of the level-0 SAP kernels, from rocprofv3 hardware counters.
 
 Two passes of the same short workload, one per counter (FETCH_SIZE, then
 WRITE_SIZE, both in KiB per dispatch), run from the repository root:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \\
-        python3 -m amg_schwingermodel_amd.profiling sample
+        python3 -m amg_schwingermodel_amd.profiling sample [run|block|chunked]
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- \\
-        python3 -m amg_schwingermodel_amd.profiling sample
-    python3 -m amg_schwingermodel_amd.profiling summarize OUT/fetch OUT/write \\
-        profiles/sap_arnoldi_a_traffic.json
+        python3 -m amg_schwingermodel_amd.profiling sample [run|block|chunked]
+    python3 -m amg_schwingermodel_amd.profiling summarize OUT/fetch OUT/write OUT.json [kernel]
 
 The workload first copies a 256 MiB device buffer three times (a plain copy
 kernel of known size: 256 MiB read and 256 MiB written per launch), then runs
 two outer iterations of the mixed-precision batched solve (512x512, 3 levels,
 32 right-hand sides, m0 = -0.12, beta = 3; FP16 SAP Krylov basis, eager
-launches), the configuration of the benchmark's roofline pass.  Each counter is scaled by the
-ratio of the copy's known bytes to what the counter reads for the copy.
+launches), the configuration of the benchmark's roofline pass.  `run`
+(default) runs the level-0 SAP as one workgroup per Schwarz block and rhs
+streaming over block rows (k_sap_run, one launch per colour), `block` as the
+block-tile kernel (k_sap_block), `chunked` as the five-kernel chunked form
+(k_sap_init / arnoldi_a / arnoldi_b / update / resid).  The sample prints the
+algorithmic bytes per launch that the library counts for the kernel.
+
+`summarize` scales each counter by the ratio of the copy's known bytes to what
+the counter reads for the copy, reports every SAP kernel it finds and puts the
+named kernel's (default k_sap_run) bytes per launch at the top.
 """
 import csv
 import glob
@@ -27,9 +34,13 @@ import numpy as np
 
 COPY_BYTES = 256 << 20
 ROWS = [(0, 8, 8, 10, 4, 4), (1, 4, 4, 10, 4, 4)]
+# kernel name prefixes reported by summarize, with the library's timing class
+KERNELS = {"k_sap_run<": 3, "k_sap_block<float": 3, "k_sap_arnoldi_a<float": 0, "k_sap_arnoldi_b<float": 1}
+# sample forms: the level-0 SAP tuning value that selects each
+FORMS = {"run": 2, "block": 1, "chunked": 1024}
 
 
-def sample(nx=512, nrhs=32, m0=-0.12):
+def sample(form="block", nx=512, nrhs=32, m0=-0.12):
     import torch
 
     import amg_schwingermodel_amd as amgs
@@ -45,10 +56,16 @@ def sample(nx=512, nrhs=32, m0=-0.12):
     s = amgs.Solver(amgs.default_params(nx, nx, ROWS, m0=m0, max_rhs=nrhs, precision=amgs.PREC_MIXED16,
                                         out_m=2, out_restarts=1), U)
     s.setup()
-    s.kernel_timing(0, True)
+    s.set_tuning(nrhs, sap_wgs=FORMS[form])
+    for kc in set(KERNELS.values()):
+        s.kernel_timing(kc, True)
     s.solve(B)
-    n, ms, nbytes = s.kernel_stats(0)
-    print(json.dumps({"sap_a_launches": n, "sap_a_alg_bytes_per_launch": nbytes / max(n, 1)}), flush=True)
+    rec = {}
+    for kc in sorted(set(KERNELS.values())):
+        n, ms, nbytes = s.kernel_stats(kc)
+        if n:
+            rec[f"class {kc}"] = {"launches": n, "alg_bytes_per_launch": nbytes / n, "avg_us": ms * 1e3 / n}
+    print(json.dumps({"form": form, "kernels": rec}), flush=True)
     s.close()
 
 
@@ -66,8 +83,9 @@ def _per_kernel(d, counter):
     return out
 
 
-def summarize(fetch_dir, write_dir, out_json):
-    res = {}
+def summarize(fetch_dir, write_dir, out_json, main="k_sap_run<"):
+    res = {"copy": {}}
+    counts = {}
     for key, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
         k = _per_kernel(d, counter)
         copy = [v for name, vals in k.items() if "copy" in name.lower() or "elementwise" in name.lower()
@@ -75,20 +93,21 @@ def summarize(fetch_dir, write_dir, out_json):
         if not copy:
             raise RuntimeError(f"no copy kernel of {COPY_BYTES} bytes found in {d}")
         scale = COPY_BYTES / (np.median(copy) * 1024)
-        sap = [v for name, vals in k.items() if "k_sap_arnoldi_a<float" in name for v in vals]
-        res[key] = {"copy_kib_per_launch": float(np.median(copy)), "scale": float(scale),
-                    "sap_a_launches": len(sap), "sap_a_bytes_per_launch": float(np.mean(sap) * 1024 * scale)}
-    res["hbm_bytes_per_launch"] = res["fetch"]["sap_a_bytes_per_launch"] + res["write"]["sap_a_bytes_per_launch"]
-    res["kernel"] = "k_sap_arnoldi_a<float, J>, all J"
-    # step B of the same run, with the same copy scale
-    b = {}
-    for key, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
-        vals = [v for name, vs in _per_kernel(d, counter).items() if "k_sap_arnoldi_b<float" in name for v in vs]
-        b["launches"] = len(vals)
-        b[f"{key}_bytes_per_launch"] = float(np.mean(vals) * 1024 * res[key]["scale"])
-    b["hbm_bytes_per_launch"] = b["fetch_bytes_per_launch"] + b["write_bytes_per_launch"]
-    b["kernel"] = "k_sap_arnoldi_b<float, true>"
-    res["step_b"] = b
+        res["copy"][key] = {"kib_per_launch": float(np.median(copy)), "scale": float(scale)}
+        for prefix in KERNELS:
+            vals = [v for name, vs in k.items() if prefix in name for v in vs]
+            if vals:
+                c = counts.setdefault(prefix, {})
+                c["launches"] = len(vals)
+                c[f"{key}_bytes_per_launch"] = float(np.mean(vals) * 1024 * scale)
+    for prefix, c in counts.items():
+        if "fetch_bytes_per_launch" in c and "write_bytes_per_launch" in c:
+            c["hbm_bytes_per_launch"] = c["fetch_bytes_per_launch"] + c["write_bytes_per_launch"]
+    res["kernels"] = counts
+    # the main kernel's bytes at the top (what the benchmark's roofline reads)
+    if main in counts and "hbm_bytes_per_launch" in counts[main]:
+        res["kernel"] = main
+        res["hbm_bytes_per_launch"] = counts[main]["hbm_bytes_per_launch"]
     with open(out_json, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res, indent=1))
@@ -96,9 +115,9 @@ def summarize(fetch_dir, write_dir, out_json):
 
 if __name__ == "__main__":
     if sys.argv[1:2] == ["sample"]:
-        sample()
-    elif sys.argv[1:2] == ["summarize"] and len(sys.argv) == 5:
-        summarize(*sys.argv[2:5])
+        sample(*(sys.argv[2:3] or ["run"]))
+    elif sys.argv[1:2] == ["summarize"] and len(sys.argv) in (5, 6):
+        summarize(*sys.argv[2:6])
     else:
         print(__doc__)
         sys.exit(2)

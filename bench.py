@@ -297,15 +297,23 @@ def main():
     if ns_launch > 0:
         # one workgroup per (block, rhs) item: the colour's whole block GMRES
         # per launch; bytes counted by the kernel (links once, per item and
-        # block site the slots, x_B and b it moves)
-        tr_path = os.path.join(ROOT, "profiles", "sap_block_traffic.json")
+        # block site the slots, x_B and b it moves).  Tuning 2: the streaming
+        # kernel (rows in registers, each slot read once per Arnoldi step),
+        # 1: the block-tile kernel
+        if sap_wgs == 2:
+            name = "k_sap_run"
+            desc = ("k_sap_run<16> (one SAP colour: every block's GMRES(5) x 5 restarts streaming over block rows, "
+                    "one 512-thread workgroup per block and rhs, FP16 basis)")
+        else:
+            name = "k_sap_block"
+            desc = ("k_sap_block<float,FP16> (one SAP colour: every block's GMRES(5) x 5 restarts, "
+                    "one workgroup per block and rhs)")
+        tr_path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
         if os.path.exists(tr_path):
             traffic = json.load(open(tr_path)).get("hbm_bytes_per_launch")
         achieved = ks_bytes / (ks_ms * 1e-3) / 1e9 if ks_ms > 0 else 0.0
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_sap_block<float,FP16> (one SAP colour: every block's GMRES(5) x 5 restarts, "
-                                     "one workgroup per block and rhs)",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": desc,
                            "launches": ns_launch, "avg_launch_us": round(ks_ms * 1e3 / max(ns_launch, 1), 2),
                            "algorithmic_bytes_per_launch": round(ks_bytes / max(ns_launch, 1))}
     else:

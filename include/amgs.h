@@ -194,6 +194,12 @@ int amgs_apply_P(amgs_ctx* c, int level, int nrhs, const double* coarse, double*
 int amgs_apply_Pt(amgs_ctx* c, int level, int nrhs, const double* fine, double* coarse);
 /* SAP_C::SAP(v, x, nu) (src/sap.cpp:64-174); x is updated in place. */
 int amgs_sap(amgs_ctx* c, int level, int nrhs, const double* v, double* x, int nu);
+/* SAP_C::SAP(v, x, nu) at level 0 (src/sap.cpp:64-174) in the inner precision
+ * of a mixed-precision context (FP32 arithmetic; FP16 Krylov basis under
+ * AMGS_PREC_MIXED16), the smoother its solves run, in the launch form the
+ * tuning selects; host buffers, x updated in place.  AMGS_ERR_STATE for an
+ * FP64 context or before amgs_setup. */
+int amgs_sap_inner(amgs_ctx* c, int nrhs, const double* v, double* x, int nu);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops): amgs_sap at a
  * coarse level (1 .. levels-2) on device buffers, with the block solves'

@@ -725,6 +725,71 @@ def test_sap_block_form_matches_chunked_form(case, prec):
     s.close()
 
 
+STREAM_CASES = {
+    # block shapes of the streaming level-0 SAP: 16x16 (one run of 2 rows per
+    # 8 lanes), 32x32, 64x64 (4 rows per thread) and 128x128 (16 rows per
+    # thread, the 512^2 benchmark's blocks)
+    "64_16blk": (64, 64, [(0, 4, 4, 6, 4, 4)], -0.10, 2.0),
+    "64_32blk": (64, 64, [(0, 4, 4, 6, 2, 2)], -0.10, 2.0),
+    "128_64blk_3lvl": (128, 128, [(0, 8, 8, 6, 2, 2), (1, 2, 2, 6, 2, 2)], -0.10, 2.0),
+    "256_128blk_3lvl": (256, 256, [(0, 8, 8, 6, 2, 2), (1, 2, 2, 6, 2, 2)], -0.10, 2.0),
+}
+
+
+@pytest.mark.parametrize("case", list(STREAM_CASES))
+def test_sap_streaming_form(case):
+    """The FP16-basis level-0 SAP streaming over block rows (tuning 2: one
+    workgroup per (block, rhs) item, rows in registers, t-neighbours by lane
+    shuffles, the last norm of a cycle from the Gram data) against the
+    block-tile form (1), the chunked launches (2048) and the FP64 oracle.  The
+    three FP16 forms differ only in rounding (order of the inner products, the
+    last norm's evaluation), so their smoother outputs agree with each other
+    and with the oracle to the FP16 level (1e-3; measured 1e-5 .. 2e-4, a
+    wrong neighbour shows as 1e-2 or more).  Restart lengths 1, 2, 5 and 1, 2,
+    5 cycles; then full mixed-precision solves with the streaming form against
+    the oracle."""
+    nx, nt, rows, m0, beta = STREAM_CASES[case]
+    U, _ = amgs.gauge_metropolis(nx, nt, beta, 30, 77)
+    o = O.Oracle(O.default_params(nx, nt, rows, m0=m0, threads=8), U)
+    o.setup(draw_order=1)
+    for m, restarts in ((1, 1), (2, 2), (5, 1), (5, 5)):
+        s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=amgs.PREC_MIXED16, max_rhs=2,
+                                            sap_m=m, sap_restarts=restarts), U)
+        s.setup()
+        V = np.stack([cvec(s.size(0), 720 + i) for i in range(2)])
+        X0 = np.stack([cvec(s.size(0), 730 + i) for i in range(2)])
+        out = {}
+        for wgs in (2, 1, 2048):
+            s.set_tuning(2, sap_wgs=wgs)
+            s.kernel_timing(3, True)
+            out[wgs] = s.sap_inner(V, X0, 2, nrhs=2).reshape(2, -1)
+            launches = s.kernel_stats(3)[0]
+            s.kernel_timing(3, False)
+            assert (launches > 0) == (wgs in (1, 2)), (wgs, launches)
+        po = O.default_params(nx, nt, rows, m0=m0, threads=8, sap_m=m, sap_restarts=restarts)
+        oo = O.Oracle(po, U)
+        for i in range(2):
+            xo, _ = oo.sap(0, V[i], X0[i], 2)
+            for wgs in (2, 1, 2048):
+                assert rel(out[wgs][i], xo) < 1e-3, (m, restarts, wgs, i)
+            assert rel(out[2][i], out[1][i]) < 1e-3 and rel(out[2][i], out[2048][i]) < 1e-3, (m, restarts, i)
+        s.close()
+    s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=m0, precision=amgs.PREC_MIXED16, max_rhs=2), U)
+    s.setup()
+    for n in (1, 2):
+        s.set_tuning(n, sap_wgs=2)
+    B = np.stack([cvec(s.size(0), 740 + i) for i in range(2)])
+    s.kernel_timing(3, True)
+    X, its, rrs, _ = s.solve(B)
+    assert s.kernel_stats(3)[0] > 0
+    s.kernel_timing(3, False)
+    for i in range(2):
+        xo, ito, _, _ = o.solve(B[i])
+        assert rrs[i] < 1e-10
+        assert int(its[i]) <= ito + 3 and rel(X[i], xo) < 1e-8, (i, its[i], ito)
+    s.close()
+
+
 def test_autotune_keeps_results_deterministic():
     """The SAP launch-shape autotuner picks one of its candidate workgroup
     targets; afterwards solves converge as before and repeat bit for bit."""
@@ -733,7 +798,8 @@ def test_autotune_keeps_results_deterministic():
     x0, it0, rr0, _ = s.solve(B)
     best, ms = s.autotune(2)
     # 1: one workgroup per (block, rhs) item (its own kernel where the geometry allows)
-    assert best in (1, 512, 1024, 1536, 2048, 2560, 3072, 4096) and ms > 0
+    # 2: one workgroup per item streaming over block rows (k_sap_run)
+    assert best in (1, 2, 512, 1024, 1536, 2048, 2560, 3072, 4096) and ms > 0
     x1, it1, rr1, _ = s.solve(B)
     x2, it2, rr2, _ = s.solve(B)
     assert np.array_equal(x1, x2) and np.array_equal(it1, it2)
