@@ -364,11 +364,11 @@ template <int RW>
 static void sap_run_launch(const Ctx& c, int nrhs, const SapArgs<float>& a, c32* x, int* act, KTimer& kt) {
     const Geo& g = c.geo[0];
     const int nth = (g.stel / 2) * (g.sxel / RW);
-    const size_t dyn = (size_t)nth * SR_LDS_PER_THREAD;
+    const size_t dyn = (size_t)nth * sr_lds_per_thread<RW>();
     static bool attr = false;
     if (!attr) {
         HIP_CHECK(hipFuncSetAttribute((const void*)k_sap_run<RW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(SRT * SR_LDS_PER_THREAD)));
+                                      (int)(SRT * sr_lds_per_thread<RW>())));
         attr = true;
     }
     const dim3 grid(g.ncol, nrhs), block(nth);

@@ -46,6 +46,31 @@ constexpr size_t SR_LDS_PER_THREAD = 2 * sizeof(f16x8) + 2 * sizeof(Pair4<float>
 // registers of longer cycles do not fit the 1024-thread workgroup)
 constexpr int SR_MAXM = 5;
 
+// Global accesses as a wave-uniform base plus a 32-bit byte offset: the
+// loads and stores then take the scalar-base + vector-offset form and no lane
+// holds a 64-bit address (every offset here is below 2^31).
+template <typename T> __device__ __forceinline__ T sr_ld(const void* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T> __device__ __forceinline__ void sr_st(void* base, unsigned off, const T& v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+typedef float sr_f4 __attribute__((ext_vector_type(4)));
+// both spins of the sites n, n+1 (n even) of an FP32 vector [ns][2]
+__device__ __forceinline__ Pair4<float> sr_ld_pair(const c32* base, unsigned n) {
+    const sr_f4 a = sr_ld<sr_f4>(base, 16u * n), b = sr_ld<sr_f4>(base, 16u * n + 16u);
+    Pair4<float> p;
+    p.v[0] = c32{a.x, a.y};
+    p.v[1] = c32{a.z, a.w};
+    p.v[2] = c32{b.x, b.y};
+    p.v[3] = c32{b.z, b.w};
+    return p;
+}
+__device__ __forceinline__ void sr_st_pair(c32* base, unsigned n, const Pair4<float>& p) {
+    sr_st<sr_f4>(base, 16u * n, sr_f4{p.v[0].re, p.v[0].im, p.v[1].re, p.v[1].im});
+    sr_st<sr_f4>(base, 16u * n + 16u, sr_f4{p.v[2].re, p.v[2].im, p.v[3].re, p.v[3].im});
+}
+
 __device__ __forceinline__ Pair4<float> sr_unpack(const f16x8 h) {
     Pair4<float> p;
     p.v[0] = c32{float(h.s0), float(h.s1)};
@@ -112,11 +137,10 @@ __device__ __forceinline__ c32 sr_shfl_down(c32 v) {
 struct SrLinks {
     c32 f00, f01, f10, f11, b00, b10, b01, b11;
     __device__ __forceinline__ SrLinks(const c32* __restrict__ Ut, unsigned n, unsigned nt, bool ml0, bool mx_l) {
-        using F4 = float __attribute__((ext_vector_type(4)));
-        const F4* U4 = reinterpret_cast<const F4*>(Ut);
-        const F4 ua = U4[n], ub = U4[n + 1u];
+        const sr_f4 ua = sr_ld<sr_f4>(Ut, 16u * n), ub = sr_ld<sr_f4>(Ut, 16u * n + 16u);
         const unsigned l0 = ml0 ? n - 1u : n, xl = mx_l ? n - nt : n;
-        const c32 ul0 = Ut[2u * l0], ux0 = Ut[2u * xl + 1u], ux1 = Ut[2u * xl + 3u];
+        const c32 ul0 = sr_ld<c32>(Ut, 16u * l0), ux0 = sr_ld<c32>(Ut, 16u * xl + 8u),
+                  ux1 = sr_ld<c32>(Ut, 16u * xl + 24u);
         f00 = c32{ua.x, ua.y};
         f01 = c32{ua.z, ua.w};
         f10 = c32{ub.x, ub.y};
@@ -161,35 +185,45 @@ __device__ __forceinline__ void sr_stencil32(const SrLinks& K, float mm, const P
                                  o.v[3]);
 }
 
-// W rows of the thread, FP16 as stored; row k by a run-time index through a
-// switch, so the array stays in registers.
+// W rows of the thread, FP16 as stored: the first WR in registers (row k by
+// a run-time index through a switch, so the array stays in registers), the
+// others in LDS (row k at wl[(k - WR) * nth]), which keeps the 16-row form
+// within its registers.
+template <int RW> constexpr int sr_wreg() { return RW < 8 ? RW : 8; }
+template <int RW> constexpr size_t sr_lds_per_thread() {
+    return SR_LDS_PER_THREAD + (size_t)(RW - sr_wreg<RW>()) * sizeof(f16x8);
+}
 template <int RW> struct SrW {
     static_assert(RW >= 2 && RW <= 16, "rows per thread");
-    f16x8 h[RW];
+    static constexpr int WR = sr_wreg<RW>();
+    f16x8 h[WR];
+    f16x8* wl;  // this thread's LDS rows
+    unsigned nth;
     __device__ __forceinline__ f16x8 get(int k) const {
+        if (WR < RW && k >= WR) return wl[(unsigned)(k - WR) * nth];
         f16x8 r = h[0];
         switch (k) {
 #define AMGS_SR_GET(i) \
     case i:            \
-        if constexpr (i < RW) r = h[i]; \
+        if constexpr (i < WR) r = h[i]; \
         break;
             AMGS_SR_GET(1) AMGS_SR_GET(2) AMGS_SR_GET(3) AMGS_SR_GET(4) AMGS_SR_GET(5) AMGS_SR_GET(6) AMGS_SR_GET(7)
-            AMGS_SR_GET(8) AMGS_SR_GET(9) AMGS_SR_GET(10) AMGS_SR_GET(11) AMGS_SR_GET(12) AMGS_SR_GET(13)
-            AMGS_SR_GET(14) AMGS_SR_GET(15)
 #undef AMGS_SR_GET
         }
         return r;
     }
     __device__ __forceinline__ void set(int k, const f16x8 v) {
+        if (WR < RW && k >= WR) {
+            wl[(unsigned)(k - WR) * nth] = v;
+            return;
+        }
         switch (k) {
 #define AMGS_SR_SET(i)                   \
     case i:                              \
-        if constexpr (i < RW) h[i] = v; \
+        if constexpr (i < WR) h[i] = v; \
         break;
             AMGS_SR_SET(0) AMGS_SR_SET(1) AMGS_SR_SET(2) AMGS_SR_SET(3)
             AMGS_SR_SET(4) AMGS_SR_SET(5) AMGS_SR_SET(6) AMGS_SR_SET(7)
-            AMGS_SR_SET(8) AMGS_SR_SET(9) AMGS_SR_SET(10) AMGS_SR_SET(11)
-            AMGS_SR_SET(12) AMGS_SR_SET(13) AMGS_SR_SET(14) AMGS_SR_SET(15)
 #undef AMGS_SR_SET
         }
     }
@@ -212,11 +246,9 @@ struct SrItem {
     const SapState& st;  // unused state (the scalar helpers store nothing here)
     __device__ __forceinline__ unsigned n(int k) const { return n0 + (unsigned)k * nt; }
     __device__ __forceinline__ int bi(int run, int side) const { return (run * 2 + side) * TL + l; }
-    __device__ __forceinline__ f16x8 slot_ld(int i, unsigned n) const {
-        return *reinterpret_cast<const f16x8*>(Vh0 + i * sst + n);
-    }
+    __device__ __forceinline__ f16x8 slot_ld(int i, unsigned n) const { return sr_ld<f16x8>(Vh0 + i * sst, 8u * n); }
     __device__ __forceinline__ void slot_st(int i, unsigned n, const f16x8 h) const {
-        *reinterpret_cast<f16x8*>(Vh0 + i * sst + n) = h;
+        sr_st<f16x8>(Vh0 + i * sst, 8u * n, h);
     }
     // the x-neighbour rows of row k lie in the block
     __device__ __forceinline__ bool xlo(int k) const { return rho > 0 || k > 0; }
@@ -293,7 +325,7 @@ __device__ __forceinline__ bool sr_step(const SrItem& I, SrW<RW>& W, int M, doub
         load(v0, 0);
         load(v1, RW - 1);
         I.b16[I.bi(I.rho, 0)] = form(W.h[0], v0, I.n(0));
-        I.b16[I.bi(I.rho, 1)] = form(W.h[RW - 1], v1, I.n(RW - 1));
+        I.b16[I.bi(I.rho, 1)] = form(W.get(RW - 1), v1, I.n(RW - 1));
         __syncthreads();
         bytes += 8.0 * (J + 1);  // Vt_0..Vt_{J-1} read once, Vt_J written
     }
@@ -326,7 +358,7 @@ __device__ __forceinline__ bool sr_step(const SrItem& I, SrW<RW>& W, int M, doub
             un = J == 0 ? W.get(k + 1) : form(W.get(k + 1), vn, I.n(k + 1));
         } else {
             if (k + 2 == RW)  // the run's last row, formed before the pass
-                un = J == 0 ? W.h[RW - 1] : I.b16[I.bi(I.rho, 1)];
+                un = J == 0 ? W.get(RW - 1) : I.b16[I.bi(I.rho, 1)];
             else  // the next run's first row
                 un = I.rho + 1 < I.nruns ? I.b16[I.bi(I.rho + 1, 0)] : uc;
             if constexpr (J > 0) I.slot_st(J + 1, I.n(RW - 1), un);
@@ -410,7 +442,7 @@ __device__ __forceinline__ double sr_last_norm2(const ItemLds& L, int M, int j) 
 // One colour of one SAP sweep for every item: x_B = b_B, then `restarts`
 // cycles of GMRES(m) on D_B x_B = b_B; after the last cycle x_B is added to
 // xacc (src/sap.cpp:117-125).  Grid (colour blocks, rhs); block TL sxel / RW
-// threads; dynamic LDS SR_LDS_PER_THREAD bytes per thread.  act (may be null):
+// threads; dynamic LDS sr_lds_per_thread<RW>() bytes per thread.  act (may be null):
 // the algorithmic bytes per block site moved by each item, summed over items.
 template <int RW>
 __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restrict__ xacc, int restarts, int* act) {
@@ -450,36 +482,42 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
     const size_t off = (size_t)rhs * g.ns * 2;
     const c32* __restrict__ bg = a.rglob + off;
     c32* __restrict__ xl = a.xl + off;
+    c32* __restrict__ xa0 = xacc + off;
     const double sigd = (double)a.sig;
     double bytes = 0;  // algorithmic bytes per block site (accounting)
     const bool ml0 = l > 0, mr1t = l + 1 < TL;
     SrW<RW> W;
+    W.wl = reinterpret_cast<f16x8*>(sr_dyn + blockDim.x * SR_LDS_PER_THREAD) + tid;
+    W.nth = blockDim.x;
     // update coefficients of the cycle's x_B update (L.eta after the
     // back-substitution), as FP32
     c32 cf[SR_MAXM];
     int mi = 0;
-    // x_B at row k for the residual pass: b (first cycle, x0 = b) or
-    // x_B + sum_{i<mi} eta_i Vt_i (a restart); the slots of all M restart
-    // steps are loaded together (those at mi <= i < M, from earlier cycles or
-    // never written, are not used; the basis holds M + 1 slots, so i < M keeps
-    // every load inside it)
-    auto xrow = [&](int k, auto first) {
-        const unsigned n = I.n(k);
+    // The residual pass reads per row x_B (or b in the first cycle, x0 = b)
+    // and the slots of all M restart steps (those at mi <= i < M, from
+    // earlier cycles or never written, are not used; the basis holds M + 1
+    // slots, so i < M keeps every load inside it), loaded one row ahead.
+    struct SrX {
         Pair4<float> x;
-        if constexpr (decltype(first)::value) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) x.v[e] = bg[2u * n + e];
-        } else {
-            f16x8 v[SR_MAXM];
-#pragma unroll
-            for (int e = 0; e < 4; e++) x.v[e] = xl[2u * n + e];
+        f16x8 v[SR_MAXM];
+    };
+    auto xload = [&](SrX& r, int k, auto first) {
+        const unsigned n = I.n(k < RW ? k : RW - 1);
+        r.x = sr_ld_pair(first.value ? bg : xl, n);
+        if constexpr (!decltype(first)::value) {
 #pragma unroll
             for (int i = 0; i < SR_MAXM; i++)
-                if (i < M) v[i] = I.slot_ld(i, n);
+                if (i < M) r.v[i] = I.slot_ld(i, n);
+        }
+    };
+    // x_B (first cycle: b; else x_B + sum_{i<mi} eta_i Vt_i) of a loaded row
+    auto xfin = [&](const SrX& r, auto first) {
+        Pair4<float> x = r.x;
+        if constexpr (!decltype(first)::value) {
 #pragma unroll
             for (int i = 0; i < SR_MAXM; i++) {
                 if (i < mi) {
-                    const Pair4<float> vi = sr_unpack(v[i]);
+                    const Pair4<float> vi = sr_unpack(r.v[i]);
 #pragma unroll
                     for (int e = 0; e < 4; e++) cmad(x.v[e], cf[i], vi.v[e]);
                 }
@@ -487,16 +525,18 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
         }
         return x;
     };
-    // x_B from xrow stored, then Vt_0 = sg (b_B - D_B x_B) stored (slot 0, W
-    // rows, the run's boundary rows in LDS) in one pass over the rows, the
-    // x_B boundary rows exchanged through LDS first.  Returns the thread's
-    // |stored Vt_0|^2 (and |b|^2 in qb on the first cycle).
+    auto xstore = [&](int k, const Pair4<float>& x) { sr_st_pair(xl, I.n(k), x); };
+    // x_B stored, then Vt_0 = sg (b_B - D_B x_B) stored (slot 0, W rows, the
+    // run's boundary rows in LDS) in one pass over the rows, the x_B boundary
+    // rows exchanged through LDS first.  Returns the thread's |stored Vt_0|^2
+    // (and |b|^2 in qb on the first cycle).
     auto resid_pass = [&](auto first, float sg, double& qb) {
+        constexpr bool FIRST = decltype(first)::value;
         auto xst = [&](int k) {
-            const Pair4<float> x = xrow(k, first);
-            const unsigned n = I.n(k);
-#pragma unroll
-            for (int e = 0; e < 4; e++) xl[2u * n + e] = x.v[e];
+            SrX r;
+            xload(r, k, first);
+            const Pair4<float> x = xfin(r, first);
+            xstore(k, x);
             return x;
         };
         I.b32[I.bi(rho, 0)] = xst(0);
@@ -510,9 +550,8 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
             const unsigned n = I.n(k);
             const bool mx_l = I.xlo(k), mx_r = I.xhi(k, RW);
             const SrLinks K(I.Ut, n, I.nt, ml0, mx_l);
-            Pair4<float> bk;
-#pragma unroll
-            for (int e = 0; e < 4; e++) bk.v[e] = bg[2u * n + e];
+            // b of the row (x_B itself in the first cycle)
+            const Pair4<float> bk = FIRST ? xc : sr_ld_pair(bg, n);
             Pair4<float> xn;
             if (k + 2 < RW) xn = xst(k + 1);
             else if (k + 2 == RW) xn = I.b32[I.bi(rho, 1)];
@@ -525,7 +564,7 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 r.v[e] = bk.v[e] - o.v[e];
-                if constexpr (decltype(first)::value) qb += double(norm2(bk.v[e]));
+                if constexpr (FIRST) qb += double(norm2(bk.v[e]));
             }
             const f16x8 h = sr_pack(r, sg);
             I.slot_st(0, n, h);
@@ -599,13 +638,22 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
 #pragma unroll
             for (int i = 0; i < SR_MAXM; i++) cf[i] = cvt<float>(L.eta[i]);
             if (last) {
-                // x += x_B + sum_i eta_i Vt_i
-#pragma unroll 2
-                for (int k = 0; k < RW; k++) {
+                // x += x_B + sum_i eta_i Vt_i, two rows' loads in flight
+#pragma unroll 1
+                for (int k = 0; k < RW; k += 2) {
+                    SrX A, B;
+                    xload(A, k, Restart{});
+                    xload(B, k + 1, Restart{});
+                    const Pair4<float> xa = xfin(A, Restart{}), xb = xfin(B, Restart{});
                     const unsigned n = I.n(k);
-                    const Pair4<float> x = xrow(k, Restart{});
+                    Pair4<float> ya = sr_ld_pair(xa0, n), yb = sr_ld_pair(xa0, n + I.nt);
 #pragma unroll
-                    for (int e = 0; e < 4; e++) xacc[off + 2u * n + e] += x.v[e];
+                    for (int e = 0; e < 4; e++) {
+                        ya.v[e] += xa.v[e];
+                        yb.v[e] += xb.v[e];
+                    }
+                    sr_st_pair(xa0, n, ya);
+                    sr_st_pair(xa0, n + I.nt, yb);
                 }
                 bytes += 8.0 * mi + 16.0 * 3;
             } else {
@@ -635,8 +683,11 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
 #pragma unroll 1
             for (int k = 0; k < RW; k++) {
                 const unsigned n = I.n(k);
+                Pair4<float> y = sr_ld_pair(xa0, n);
+                const Pair4<float> xb = sr_ld_pair(xl, n);
 #pragma unroll
-                for (int e = 0; e < 4; e++) xacc[off + 2u * n + e] += xl[2u * n + e];
+                for (int e = 0; e < 4; e++) y.v[e] += xb.v[e];
+                sr_st_pair(xa0, n, y);
             }
             bytes += 16.0 * 3;
         }
