@@ -1226,8 +1226,10 @@ template <typename R> __device__ __forceinline__ void c_run(const CArgs<R>& A, C
 }
 
 // The arguments travel by value in the kernel-argument segment (no per-launch
-// copy, capturable in a HIP graph) and are read in place through the segment
-// pointer (scalar loads; a copy of the parameter would land in scratch).
+// copy, so the launch can be captured in a HIP graph); each workgroup copies
+// them to LDS, where they are read on demand (read in place through the
+// segment pointer, the kernel kept many of them in scalar registers and
+// spilled several hundred of those: 10 % slower per launch).
 template <typename R> __device__ __forceinline__ void coarse_main(const CArgs<R>& A, CLds& S, char* dyn_lds) {
     const int rhs = blockIdx.x / A.grp, member = blockIdx.x - rhs * A.grp;
     if (A.rmask && !A.rmask[rhs]) return;  // the whole group skips a finished rhs
@@ -1279,18 +1281,10 @@ template <typename R> __device__ __forceinline__ void coarse_main(const CArgs<R>
     Ttot.lap(31);
 }
 
-template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R> Aval) {
-    (void)Aval;
-    const CArgs<R>& A = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
-    __shared__ CLds S;
-    extern __shared__ __align__(16) char dyn_lds[];
-    coarse_main(A, S, dyn_lds);
-}
-
 // Several gauge configurations in one launch (right-hand side r belongs to
-// configuration r): each workgroup copies the arguments to LDS with its
-// configuration's coarse links, test vectors and mass term in place.
-template <typename R> __global__ void __launch_bounds__(CT) k_coarse_cfg(const CArgs<R> Aval) {
+// configuration r, mmv non-null): the copy of the arguments gets the
+// workgroup's configuration's coarse links, test vectors and mass term.
+template <typename R> __global__ void __launch_bounds__(CT) k_coarse(const CArgs<R> Aval) {
     (void)Aval;
     const CArgs<R>& Ak = *(const CArgs<R>*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ CLds S;
@@ -1300,7 +1294,7 @@ template <typename R> __global__ void __launch_bounds__(CT) k_coarse_cfg(const C
     for (int i = threadIdx.x; i < (int)(sizeof(CArgs<R>) / 4); i += CT)
         reinterpret_cast<int*>(&Ac)[i] = reinterpret_cast<const int*>(&Ak)[i];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && Ak.mmv) {
         const int cfg = blockIdx.x / Ak.grp;
         for (int l = 1; l < MAXL; l++) {
             if (Ac.lv[l].links) Ac.lv[l].links += cfg * Ak.lcs[l];

@@ -24,15 +24,14 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     }
     a.mmv = mmv;
     a.rmask = rmask;
-    // one workgroup group per configuration: the arguments go through LDS
-    const bool cfg = mmv != nullptr;
     a.mode = mode;
     a.level = level;
     a.nu = nu;
     // coarsest GMRES vectors in dynamic LDS if they fit beside the static part
     const size_t ncrs = c->nvec(c->L - 1);
     const size_t dyn = (size_t)(a.crs_m + 4) * ncrs * sizeof(cplx<R>);
-    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024 - (cfg ? sizeof(CArgs<R>) : 0);
+    // (the kernel's static LDS: CLds and its copy of the arguments)
+    const size_t lds_cap = 160 * 1024 - sizeof(CLds) - 1024 - sizeof(CArgs<R>);
     a.crs_lds = dyn <= lds_cap ? 1 : 0;
     size_t dyn_bytes = a.crs_lds ? dyn : 0;
     // SAP block vectors of a level share that region (never live at the same time)
@@ -67,15 +66,11 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
         }
     }
     if (dyn_bytes > 0) {
-        static bool attr_set[2][2] = {{false, false}, {false, false}};
-        bool& done = attr_set[sizeof(R) == 8][cfg];
+        static bool attr_set[2] = {false, false};
+        bool& done = attr_set[sizeof(R) == 8];
         if (!done) {
-            if (cfg)
-                HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse_cfg<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)(160 * 1024 - sizeof(CLds) - sizeof(CArgs<R>) - 1024)));
-            else
-                HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)(160 * 1024 - sizeof(CLds))));
+            HIP_CHECK(hipFuncSetAttribute((const void*)k_coarse<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(160 * 1024 - sizeof(CLds) - sizeof(CArgs<R>) - 1024)));
             done = true;
         }
     }
@@ -85,14 +80,8 @@ void Engine<R>::coarse(int mode, int level, int nrhs, const cplx<R>* in, size_t 
     a.cnt = nullptr;
     if (kt.begin(c->stream, 0.0, 0.0)) {
         a.cnt = kt.flop_counter();
-        if (cfg)
-            hipExtLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, kt.start(),
-                                  kt.stop(), 0, a);
-        else
-            hipExtLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, kt.start(),
-                                  kt.stop(), 0, a);
-    } else if (cfg) {
-        hipLaunchKernelGGL(k_coarse_cfg<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
+        hipExtLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, kt.start(), kt.stop(), 0,
+                              a);
     } else {
         hipLaunchKernelGGL(k_coarse<R>, dim3(nrhs * a.grp), dim3(CT), dyn_bytes, c->stream, a);
     }

@@ -350,7 +350,7 @@ def main():
     achieved_c = kc_flops / (kc_ms * 1e-3) / 1e12 if kc_ms > 0 else 0.0
     out["roofline_coarse"] = {"bound": "latency", "achieved": round(achieved_c, 3), "peak": FP32_PEAK_TFS,
                               "unit": "TFLOP/s", "frac": round(achieved_c / FP32_PEAK_TFS, 5), "traffic": None,
-                              "kernel": "k_coarse<float> (K-cycle levels >= 1: FGMRES, SAP, coarsest GMRES)",
+                              "kernel": "k_coarse<float> (K-cycle levels >= 1: FGMRES, SAP, coarsest GMRES; arguments copied to LDS)",
                               "launches": nc_launch, "avg_launch_us": round(kc_ms * 1e3 / max(nc_launch, 1), 1),
                               "stencil_flops_per_launch": round(kc_flops / max(nc_launch, 1)),
                               "workgroups_per_launch": a.rhs * coarse_grp,
