@@ -1153,7 +1153,7 @@ static void outer_solve(Ctx& c, int nrhs) {
         HIP_CHECK(hipMemcpy(h.data(), pb.p, 32 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         std::fprintf(stderr, "[amgs profile %s, rhs 0, ms at 100 MHz]", tag);
         for (int i = 0; i < 32; i++)
-            if (h[i]) std::fprintf(stderr, " %d:%.3f", i, (i == 7 || i == 11 || i == 15 || i == 19) ? (double)h[i] : h[i] * 1e-5);
+            if (h[i]) std::fprintf(stderr, " %d:%.3f", i, (i == 7 || i == 11 || i == 15 || i == 16 || i == 19) ? (double)h[i] : h[i] * 1e-5);
         std::fprintf(stderr, "\n");
     };
     dump(c.e64.prof, "fp64");

@@ -99,8 +99,10 @@ template <int M> struct FgScal {
 };
 struct WaveScal {
     c64 R[KC_MAXM][KC_MAXM];
+    c64 G[KC_MAXM][KC_MAXM];  // Gram rows of the cycle's basis: G[i][k] = <V_i, V_k>, k < i
     c64 g[KC_MAXM + 1], cn[KC_MAXM], sn[KC_MAXM], eta[KC_MAXM];
     c64 col[KC_MAXM + 1];
+    c64 dq[2 * KC_MAXM + 2];  // quantities of a step's inner-product pass
     double bsc[2];  // |b_B|, |r_B| broadcast to the workgroup (g_block_gmres_2x2)
     int brk;
 };
@@ -585,6 +587,57 @@ __device__ __forceinline__ void w_dlocal(const CArgs<R>& A, int l, int blk, cons
     }
 }
 
+// Modified Gram-Schmidt of block GMRES step j (src/fgmres.cpp:27-34) from
+// one pass of inner products instead of j + 1 dependent ones.  For w = D_B V_j
+// and the stored unit vectors V_0..V_j, MGS projects one direction at a time,
+// h_i = <w - sum_{k<i} h_k V_k, V_i> = d_i - sum_{k<i} h_k conj(G_ik) with
+// d_i = <w, V_i> and G_ik = <V_i, V_k>, and each projection takes |h_i|^2
+// off the squared norm: |w - sum_i h_i V_i|^2 = |w|^2 - sum_i |h_i|^2.
+// Pass quantity q (one wavefront: lane partial sums over e = lane + 64 t in
+// order, then the wave butterfly): q <= j: d_q; j < q <= 2j: G_{j,q-j-1};
+// q = 2j + 1: |w|^2.
+template <typename R, typename VF>
+__device__ __forceinline__ c64 w_gs_quantity(int q, int j, const cplx<R>* w, VF V, int nb) {
+    const cplx<R>* a = q <= j || q == 2 * j + 1 ? w : V(j);
+    const cplx<R>* b = q <= j ? V(q) : q == 2 * j + 1 ? w : V(q - j - 1);
+    return w_dot(a, b, nb);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// One wavefront, after the pass quantities are in S.dq: the column h_0..h_j
+// into S.col (lane i forms h_i; h_k is final once the lanes below it are),
+// the Gram row j into S.G; returns |w - sum_i h_i V_i|^2 (wave-uniform).
+__device__ __forceinline__ double w_gs_column(WaveScal& S, int j) {
+    const int lane = threadIdx.x & 63;
+    c64 d = {0.0, 0.0};
+    if (lane <= j) d = S.dq[lane];
+    double u2 = S.dq[2 * j + 1].re;
+    for (int k = 0; k <= j; k++) {
+        const c64 hk = {readlane_d(d.re, k), readlane_d(d.im, k)};
+        u2 -= norm2(hk);
+        if (lane > k && lane <= j) {
+            const c64 gik = lane == j ? S.dq[j + 1 + k] : S.G[lane][k];
+            d -= hk * conj(gik);
+        }
+    }
+    if (lane <= j) S.col[lane] = d;
+    if (lane < j) S.G[j][lane] = S.dq[j + 1 + lane];
+    return u2 > 0 ? u2 : 0.0;
+}
+// Element e of V_{j+1} = (w - sum_i h_i V_i) / |.| (w is stored in V_{j+1});
+// hn = |.| from the algebra (0: the difference is stored unscaled).
+template <typename R, typename VF>
+__device__ __forceinline__ void w_gs_form(const WaveScal& S, VF V, int j, double hn, int e) {
+    cplx<R>* w = V(j + 1);
+    c64 u = cvt<double>(w[e]);
+    for (int i = 0; i <= j; i++) u -= S.col[i] * cvt<double>(V(i)[e]);
+    w[e] = cvt<R>(hn > 0 ? (1.0 / hn) * u : u);
+}
+
 // Givens rotation of column col[0..j+1] with stored cn/sn (src/fgmres.cpp:90-104);
 // col, cn, sn live in LDS; one thread executes it.  col[j] becomes the
 // diagonal of R; returns cn_j, sn_j.
@@ -602,7 +655,8 @@ __device__ __forceinline__ void givens_col(c64* col, const c64* cn, const c64* s
 }
 
 // GMRES(m, restarts, tol) on D_B for one Schwarz block, x0 = b_B; then
-// x[B] += x_B.  Executed by one wavefront.  True modified Gram-Schmidt.
+// x[B] += x_B.  Executed by one wavefront; modified Gram-Schmidt from one
+// pass of inner products per step (w_gs_column).
 template <typename R>
 __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob, cplx<R>* x) {
     const CLevel<R>& L = A.lv[l];
@@ -650,20 +704,16 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
             T.lap(25);
             T.count(19);
             cplx<R>* w = V(j + 1);
-            for (int i = 0; i <= j; i++) {
-                const c64 h = w_dot(w, V(i), nb);
-                if (lane == 0) S.col[i] = h;
-                const cplx<R> hr = cvt<R>(h);
-                const cplx<R>* vi = V(i);
-                for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
-                wave_sync();
+            for (int q = 0; q <= 2 * j + 1; q++) {
+                const c64 v = w_gs_quantity<R>(q, j, w, V, nb);
+                if (lane == 0) S.dq[q] = v;
             }
+            wave_sync();
+            const double hn = sqrt(w_gs_column(S, j));
+            wave_sync();
             T.lap(26);
-            const double hn = sqrt(w_dot(w, w, nb).re);
-            if (hn > 0) {
-                const R ih = R(1.0 / hn);
-                for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
-            }
+            for (int e = lane; e < nb; e += 64) w_gs_form<R>(S, V, j, hn, e);
+            wave_sync();
             T.lap(27);
             if (lane == 0) {
                 S.col[j + 1] = c64{hn, 0.0};
@@ -759,8 +809,10 @@ __device__ __forceinline__ void wg_dlocal_2x2(R mm, int D, const float __attribu
 // member of a grouped launch that owns one block of the colour): the block's
 // links are staged in LDS once and serve every stencil of the block's
 // GMRES(m, restarts), which the workgroup's threads share one (row, link
-// block) partial sum each; the Krylov vector operations and the scalars stay
-// on wavefront 0 in the order of w_block_gmres.  Needs the FP32 engine's
+// block) partial sum each; a step's inner products are spread over the
+// wavefronts, each computed as w_block_gmres computes it, and the column,
+// the new vector and the scalars stay on wavefront 0 in the order of
+// w_block_gmres, so both give the same bits.  Needs the FP32 engine's
 // column-pair links and sap_wg[l].
 template <typename R>
 __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int l, int blk, const cplx<R>* rglob,
@@ -780,6 +832,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
     auto V = [&](int j) { return base + (size_t)j * nb; };
     cplx<R>* xb = base + (size_t)(m + 1) * nb;
     cplx<R>* bb = base + (size_t)(m + 2) * nb;
+    PhaseTimer T(K.prof);
     // stage the links; gather b_B; x_B = b_B
     {
         const int msz = D * D, ksz = g.ns * msz;
@@ -810,6 +863,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
         }
     }
     __syncthreads();
+    T.lap(18);
     const double normb = S.bsc[0];
     double err = S.bsc[1];
     if (normb == 0.0) return;  // zero block right-hand side: x_B = 0
@@ -821,22 +875,22 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
             if (lane == 0) S.g[0] = c64{err, 0.0};
         }
         for (int j = 0; j < m; j++) {
+            T.lap(17);
             wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1), cnt);
+            T.lap(5);
+            T.count(16);
+            // the pass quantities over the wavefronts, then the column on wave 0
+            for (int q = wv; q <= 2 * j + 1; q += CW) {
+                const c64 v = w_gs_quantity<R>(q, j, V(j + 1), V, nb);
+                if (lane == 0) S.dq[q] = v;
+            }
+            __syncthreads();
             if (wv == 0) {
-                cplx<R>* w = V(j + 1);
-                for (int i = 0; i <= j; i++) {
-                    const c64 h = w_dot(w, V(i), nb);
-                    if (lane == 0) S.col[i] = h;
-                    const cplx<R> hr = cvt<R>(h);
-                    const cplx<R>* vi = V(i);
-                    for (int e = lane; e < nb; e += 64) w[e] -= hr * vi[e];
-                    wave_sync();
-                }
-                const double hn = sqrt(w_dot(w, w, nb).re);
-                if (hn > 0) {
-                    const R ih = R(1.0 / hn);
-                    for (int e = lane; e < nb; e += 64) w[e] = ih * w[e];
-                }
+                const double hn = sqrt(w_gs_column(S, j));
+                wave_sync();
+                // V_{j+1} from the unrotated column, then the rotation
+                for (int e = lane; e < nb; e += 64) w_gs_form<R>(S, V, j, hn, e);
+                wave_sync();
                 if (lane == 0) {
                     S.col[j + 1] = c64{hn, 0.0};
                     c64 cnj, snj;
@@ -852,6 +906,7 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
                 }
             }
             __syncthreads();
+            T.lap(6);
             if (S.brk) {
                 maxIt = j + 1;
                 break;
@@ -917,6 +972,7 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
                     wg = true;
                     if (mem < L.g.ncol) {
                         g_block_gmres_2x2<R>(A, K, l, list[mem], r, x);
+                        T.lap(22);
                         if (xs && threadIdx.x < 64) {
                             for (int e = threadIdx.x; e < nbs; e += 64) {
                                 const int m = e / D, d = e - m * D;
@@ -950,7 +1006,7 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
                 }
             }
             __syncthreads();
-            T.lap(20);
+            T.lap(wg ? 23 : 20);
             // the residual after the last colour of the last sweep and its
             // tolerance check decide nothing: x is final
             if (colour == 1 && it + 1 == nu) return;
