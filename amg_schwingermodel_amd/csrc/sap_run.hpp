@@ -131,24 +131,38 @@ __device__ __forceinline__ c32 sr_shfl_down(c32 v) {
     return c32{__shfl_down(v.re, 1u, 64), __shfl_down(v.im, 1u, 64)};
 }
 
-// The links a pair's stencil reads: both links of n and n+1, the t link of
-// n-1 and the x links of the pair one row down (masked-out hops read the
-// pair's own, unused).
+// The links a pair's stencil reads: both links of n and n+1 (the two loads
+// of a row, ua and ub), the t link of n-1 and the x links of the pair one row
+// down.  The t link of n-1 is the left lane's second site's (a lane shuffle)
+// and the x links one row down are the thread's previous row's own (carried
+// in SrXCarry); lanes and rows whose hop is masked out get a value they do
+// not use.
 struct SrLinks {
     c32 f00, f01, f10, f11, b00, b10, b01, b11;
-    __device__ __forceinline__ SrLinks(const c32* __restrict__ Ut, unsigned n, unsigned nt, bool ml0, bool mx_l) {
-        const sr_f4 ua = sr_ld<sr_f4>(Ut, 16u * n), ub = sr_ld<sr_f4>(Ut, 16u * n + 16u);
-        const unsigned l0 = ml0 ? n - 1u : n, xl = mx_l ? n - nt : n;
-        const c32 ul0 = sr_ld<c32>(Ut, 16u * l0), ux0 = sr_ld<c32>(Ut, 16u * xl + 8u),
-                  ux1 = sr_ld<c32>(Ut, 16u * xl + 24u);
+    __device__ __forceinline__ SrLinks(const sr_f4 ua, const sr_f4 ub, c32 ux0, c32 ux1) {
         f00 = c32{ua.x, ua.y};
         f01 = c32{ua.z, ua.w};
         f10 = c32{ub.x, ub.y};
         f11 = c32{ub.z, ub.w};
-        b00 = conj(ul0);
+        b00 = conj(sr_shfl_up(f10));
         b10 = conj(f00);
         b01 = conj(ux0);
         b11 = conj(ux1);
+    }
+};
+// x links (direction 1) of the pair one row down, for the next row: loaded
+// for the run's first row (its own pair's where the row below lies outside
+// the block), then taken from each row's own links.
+struct SrXCarry {
+    c32 x0, x1;
+    __device__ __forceinline__ SrXCarry(const c32* __restrict__ Ut, unsigned n0, unsigned nt, bool below) {
+        const unsigned xl = below ? n0 - nt : n0;
+        x0 = sr_ld<c32>(Ut, 16u * xl + 8u);
+        x1 = sr_ld<c32>(Ut, 16u * xl + 24u);
+    }
+    __device__ __forceinline__ void next(const sr_f4 ua, const sr_f4 ub) {
+        x0 = c32{ua.z, ua.w};
+        x1 = c32{ub.z, ub.w};
     }
 };
 
@@ -189,7 +203,7 @@ __device__ __forceinline__ void sr_stencil32(const SrLinks& K, float mm, const P
 // a run-time index through a switch, so the array stays in registers), the
 // others in LDS (row k at wl[(k - WR) * nth]), which keeps the 16-row form
 // within its registers.
-template <int RW> constexpr int sr_wreg() { return RW < 8 ? RW : 8; }
+template <int RW> constexpr int sr_wreg() { return RW < 8 ? RW : RW == 8 ? 8 : 6; }
 template <int RW> constexpr size_t sr_lds_per_thread() {
     return SR_LDS_PER_THREAD + (size_t)(RW - sr_wreg<RW>()) * sizeof(f16x8);
 }
@@ -342,12 +356,13 @@ __device__ __forceinline__ bool sr_step(const SrItem& I, SrW<RW>& W, int M, doub
     // row k+2 go to vp.  Three register sets take the three roles in turn (the
     // row loop runs three rows per trip), so no copy of a set waits for its
     // loads.
+    SrXCarry X(I.Ut, I.n(0), I.nt, I.xlo(0));
     auto row = [&](int k, const f16x8 (&vc)[JV], const f16x8 (&vn)[JV], f16x8 (&vp)[JV]) {
         const unsigned n = I.n(k);
         const bool mx_l = I.xlo(k), mx_r = I.xhi(k, RW);
         // the row's links first: vector-memory counters retire in order, so
         // the stencil's wait for them then leaves the prefetch below in flight
-        const SrLinks K(I.Ut, n, I.nt, ml0, mx_l);
+        const sr_f4 ua = sr_ld<sr_f4>(I.Ut, 16u * n), ub = sr_ld<sr_f4>(I.Ut, 16u * n + 16u);
         // The same memory operations on every path (a clamped prefetch row; a
         // store of the value at hand into slot J+1, written later before any
         // read or never read, where no row is formed): the compiler's waits
@@ -365,6 +380,8 @@ __device__ __forceinline__ bool sr_step(const SrItem& I, SrW<RW>& W, int M, doub
         }
         f16x4 tl, tr;
         sr_tnb(uc, tl, tr);
+        const SrLinks K(ua, ub, X.x0, X.x1);
+        X.next(ua, ub);
         Pair4<float> o, c;
         sr_stencil16(K, I.mm, uc, un, up, tl, tr, mx_r, mx_l, ml0, mr1t, o, c);
         // W as stored in FP16 (the Hessenberg column belongs to these values)
@@ -545,11 +562,12 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
         Pair4<float> xc = I.b32[I.bi(rho, 0)];
         Pair4<float> xp = rho > 0 ? I.b32[I.bi(rho - 1, 1)] : xc;
         double qr = 0;
+        SrXCarry X(I.Ut, I.n(0), I.nt, I.xlo(0));
 #pragma unroll 1
         for (int k = 0; k < RW; k++) {
             const unsigned n = I.n(k);
             const bool mx_l = I.xlo(k), mx_r = I.xhi(k, RW);
-            const SrLinks K(I.Ut, n, I.nt, ml0, mx_l);
+            const sr_f4 ua = sr_ld<sr_f4>(I.Ut, 16u * n), ub = sr_ld<sr_f4>(I.Ut, 16u * n + 16u);
             // b of the row (x_B itself in the first cycle)
             const Pair4<float> bk = FIRST ? xc : sr_ld_pair(bg, n);
             Pair4<float> xn;
@@ -558,6 +576,8 @@ __global__ void __launch_bounds__(SRT) k_sap_run(SapArgs<float> a, c32* __restri
             else xn = rho + 1 < I.nruns ? I.b32[I.bi(rho + 1, 0)] : xc;
             const Spin2<float> tl{sr_shfl_up(xc.v[2]), sr_shfl_up(xc.v[3])};
             const Spin2<float> tr{sr_shfl_down(xc.v[0]), sr_shfl_down(xc.v[1])};
+            const SrLinks K(ua, ub, X.x0, X.x1);
+            X.next(ua, ub);
             Pair4<float> o;
             sr_stencil32(K, mm, xc, xn, xp, tl, tr, mx_r, mx_l, ml0, mr1t, o);
             Pair4<float> r;
