@@ -23,6 +23,7 @@ constexpr int CW = CT / 64;    // wavefronts per coarse workgroup
 constexpr int MAXL = 5;        // maximum number of levels
 constexpr int KC_MAXM = 8;     // max restart length of K-cycle FGMRES and SAP GMRES
 constexpr int CRS_MAXM = 24;   // max restart length of the coarsest GMRES
+static_assert(KC_MAXM < 64 && CRS_MAXM < 64, "one lane per Hessenberg row in the column algebra");
 constexpr int GRP_MAX = 8;     // max workgroups per right-hand side
 
 enum CMode { CM_KC_SOLVE = 0, CM_SAP = 1, CM_DOP = 2, CM_RESTRICT = 3, CM_PROLONG = 4, CM_CYCLE = 5 };
@@ -1113,40 +1114,32 @@ __device__ __forceinline__ void fg_run(const CArgs<R>& A, CK<R>& K, int l, FgSca
             c_dop(A, l, zj, w, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
             T.lap(pb + 0);
             // one pass: <w, V_i> (i <= j) and <V_j, V_k> (k < j); nq dots,
-            // CT/nq threads per dot, partial sums combined in fixed order
-            const int nq = 2 * j + 1, per = CT / nq;
-            {
-                const int q = threadIdx.x / per, sub = threadIdx.x - q * per;
-                c64 acc = {0, 0};
-                if (q < nq) {
-                    const cplx<R>* a = q <= j ? w : V(j);
-                    const cplx<R>* bv = q <= j ? V(q) : V(q - j - 1);
-                    for (int e = sub; e < n; e += per) acc += mulc(cvt<double>(a[e]), cvt<double>(bv[e]));
-                }
-                K.S.red[threadIdx.x] = acc;
-                __syncthreads();
-                if (threadIdx.x < nq) {
-                    c64 s = K.S.red[threadIdx.x * per];
-                    for (int i = 1; i < per; i++) s += K.S.red[threadIdx.x * per + i];
-                    S.dq[threadIdx.x] = s;
-                }
-                __syncthreads();
+            // one wavefront per dot (lane partial sums in element order, then
+            // the wave butterfly)
+            const int nq = 2 * j + 1;
+            for (int q = threadIdx.x >> 6; q < nq; q += CW) {
+                const cplx<R>* a = q <= j ? w : V(j);
+                const cplx<R>* bv = q <= j ? V(q) : V(q - j - 1);
+                const c64 v = w_dot(a, bv, n);
+                if ((threadIdx.x & 63) == 0) S.dq[q] = v;
             }
+            __syncthreads();
             T.lap(pb + 1);
             if (threadIdx.x < 64) {
-                // H_ij = <w,V_i> - sum_{k<i} H_kj conj(G_ik), one wave, k over lanes
+                // H_ij = <w,V_i> - sum_{k<i} H_kj conj(G_ik): lane i forms row i,
+                // H_kj is final once the lanes below k are (m <= 64)
                 const int lane = threadIdx.x;
-                for (int i = 0; i <= j; i++) {
-                    c64 t = {0, 0};
-                    for (int kk = lane; kk < i; kk += 64) {
-                        const c64 Gik = (i == j) ? S.dq[j + 1 + kk] : S.G[i][kk];
-                        t += S.col[kk] * conj(Gik);
+                c64 d = {0.0, 0.0};
+                if (lane <= j) d = S.dq[lane];
+                for (int k = 0; k <= j; k++) {
+                    const c64 hk = {readlane_d(d.re, k), readlane_d(d.im, k)};
+                    if (lane > k && lane <= j) {
+                        const c64 gik = lane == j ? S.dq[j + 1 + k] : S.G[lane][k];
+                        d -= hk * conj(gik);
                     }
-                    t = wave_sum(t);
-                    if (lane == 0) S.col[i] = S.dq[i] - t;
-                    wave_sync();
                 }
-                for (int kk = lane; kk < j; kk += 64) S.G[j][kk] = S.dq[j + 1 + kk];
+                if (lane <= j) S.col[lane] = d;
+                if (lane < j) S.G[j][lane] = S.dq[j + 1 + lane];
             }
             __syncthreads();
             T.lap(pb + 2);
