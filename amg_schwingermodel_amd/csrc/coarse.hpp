@@ -103,8 +103,10 @@ struct WaveScal {
     c64 G[KC_MAXM][KC_MAXM];  // Gram rows of the cycle's basis: G[i][k] = <V_i, V_k>, k < i
     c64 g[KC_MAXM + 1], cn[KC_MAXM], sn[KC_MAXM], eta[KC_MAXM];
     c64 col[KC_MAXM + 1];
+    c64 rcol[KC_MAXM + 1];    // the column as rotated (w_gs_rotate)
     c64 dq[2 * KC_MAXM + 2];  // quantities of a step's inner-product pass
     double bsc[2];  // |b_B|, |r_B| broadcast to the workgroup (g_block_gmres_2x2)
+    double hn;      // |w - sum h_i V_i| of the step, broadcast (g_block_gmres_2x2)
     int brk;
 };
 constexpr int VIN_BYTES = 16384;  // LDS staging of stencil inputs
@@ -655,6 +657,23 @@ __device__ __forceinline__ void givens_col(c64* col, const c64* cn, const c64* s
     col[j] = conj(cnj) * a + conj(snj) * b;
 }
 
+// Step j's Givens rotation and residual estimate (src/fgmres.cpp:35-49) on a
+// copy of the column (col stays unrotated for w_gs_form); one lane.
+__device__ __forceinline__ void w_gs_rotate(WaveScal& S, int j, double hn, double gtol) {
+    for (int i = 0; i <= j; i++) S.rcol[i] = S.col[i];
+    S.rcol[j + 1] = c64{hn, 0.0};
+    c64 cnj, snj;
+    givens_col(S.rcol, S.cn, S.sn, j, cnj, snj);
+    for (int i = 0; i <= j; i++) S.R[i][j] = S.rcol[i];
+    const c64 gj = S.g[j];
+    const c64 gn = -(snj * gj);
+    S.cn[j] = cnj;
+    S.sn[j] = snj;
+    S.g[j] = conj(cnj) * gj;
+    S.g[j + 1] = gn;
+    S.brk = sqrt(norm2(gn)) < gtol;
+}
+
 // GMRES(m, restarts, tol) on D_B for one Schwarz block, x0 = b_B; then
 // x[B] += x_B.  Executed by one wavefront; modified Gram-Schmidt from one
 // pass of inner products per step (w_gs_column).
@@ -716,19 +735,7 @@ __device__ __forceinline__ void w_block_gmres(const CArgs<R>& A, CK<R>& K, int l
             for (int e = lane; e < nb; e += 64) w_gs_form<R>(S, V, j, hn, e);
             wave_sync();
             T.lap(27);
-            if (lane == 0) {
-                S.col[j + 1] = c64{hn, 0.0};
-                c64 cnj, snj;
-                givens_col(S.col, S.cn, S.sn, j, cnj, snj);
-                for (int i = 0; i <= j; i++) S.R[i][j] = S.col[i];
-                const c64 gj = S.g[j];
-                const c64 gn = -(snj * gj);
-                S.cn[j] = cnj;
-                S.sn[j] = snj;
-                S.g[j] = conj(cnj) * gj;
-                S.g[j + 1] = gn;
-                S.brk = sqrt(norm2(gn)) < A.sap_gtol * normb;
-            }
+            if (lane == 0) w_gs_rotate(S, j, hn, A.sap_gtol * normb);
             wave_sync();
             T.lap(28);
             if (S.brk) {
@@ -886,25 +893,19 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
                 if (lane == 0) S.dq[q] = v;
             }
             __syncthreads();
+            T.lap(24);
             if (wv == 0) {
                 const double hn = sqrt(w_gs_column(S, j));
-                wave_sync();
-                // V_{j+1} from the unrotated column, then the rotation
+                if (lane == 0) S.hn = hn;
+            }
+            __syncthreads();
+            T.lap(25);
+            // V_{j+1} from the column on wavefront 0 while wavefront 1 rotates a copy
+            if (wv == 0) {
+                const double hn = S.hn;
                 for (int e = lane; e < nb; e += 64) w_gs_form<R>(S, V, j, hn, e);
-                wave_sync();
-                if (lane == 0) {
-                    S.col[j + 1] = c64{hn, 0.0};
-                    c64 cnj, snj;
-                    givens_col(S.col, S.cn, S.sn, j, cnj, snj);
-                    for (int i = 0; i <= j; i++) S.R[i][j] = S.col[i];
-                    const c64 gj = S.g[j];
-                    const c64 gn = -(snj * gj);
-                    S.cn[j] = cnj;
-                    S.sn[j] = snj;
-                    S.g[j] = conj(cnj) * gj;
-                    S.g[j + 1] = gn;
-                    S.brk = sqrt(norm2(gn)) < A.sap_gtol * normb;
-                }
+            } else if (wv == 1 && lane == 0) {
+                w_gs_rotate(S, j, S.hn, A.sap_gtol * normb);
             }
             __syncthreads();
             T.lap(6);
