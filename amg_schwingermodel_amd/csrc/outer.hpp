@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
             a1[s] = vj[2 * n + 1];
         }
     }
-    for (int i = 0; i <= j; i++) {
+    for (int i = 0; i < j; i++) {
         c64 pw = {0, 0}, pv = {0, 0};
         const c64* vi = V + (size_t)i * vstride + off;
 #pragma unroll
@@ -151,7 +151,14 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
             }
         }
         park(pw, i, red, NQ);
-        if (i < j) park(pv, j + 1 + i, red, NQ);
+        park(pv, j + 1 + i, red, NQ);
+    }
+    {  // i = j: V_j is at hand in a0, a1
+        c64 pw = {0, 0};
+#pragma unroll
+        for (int s = 0; s < OUT_SPT; s++)
+            if (live[s]) pw += mulc(o0[s], a0[s]) + mulc(o1[s], a1[s]);
+        park(pw, j, red, NQ);
     }
     __syncthreads();
     if (threadIdx.x < nq) {

@@ -245,8 +245,9 @@ def main():
 
     def concurrent_solve(nl, rows, Ul, m0, nrhs, split, bd, xd, tag):
         """The nrhs right-hand sides as `split` concurrent sub-batches, one
-        context and stream each: one sub-batch's VALU-bound SAP step A
-        overlaps another's HBM-bound step B.  Returns (ms per step, record)."""
+        context and stream each: one sub-batch's latency-bound coarse kernel
+        and SAP phases overlap another's HBM-bound passes.  Returns (ms per
+        step, ok, record)."""
         # sub-batch sizes as equal as possible
         sizes = [nrhs // split + (1 if i < nrhs % split else 0) for i in range(split)]
         first = [sum(sizes[:i]) for i in range(split)]
