@@ -276,22 +276,24 @@ __device__ __forceinline__ void stencil_row(LP links, int ksz, int msz, int D, i
 }
 
 template <typename R, typename LP, typename VP>
-__device__ __forceinline__ void dop_rows_l(const CArgs<R>& A, int l, LP links, VP src, cplx<R>* out) {
-    const CLevel<R>& L = A.lv[l];
-    const Geo& g = L.g;
-    const int D = g.dof, n = g.ns * D;
+__device__ __forceinline__ cplx<R> dop_row(const CArgs<R>& A, const Geo& g, LP links, VP src, int e) {
+    const int D = g.dof;
     const int msz = D * D, ksz = g.ns * msz;
-    for (int e = threadIdx.x; e < n; e += CT) {
-        const int x = e / D, r = e - x * D;
-        const int xr0 = nb_right(x, 0, g.nx, g.nt), xr1 = nb_right(x, 1, g.nx, g.nt);
-        const int xl0 = nb_left(x, 0, g.nx, g.nt), xl1 = nb_left(x, 1, g.nx, g.nt);
-        const R f1 = wraps_right(x, 0, g.nt) ? R(-1) : R(1);
-        const R f3 = wraps_left(x, 0, g.nt) ? R(-1) : R(1);
-        cplx<R> acc = A.mm * ld<R>(src + e);
-        stencil_row<R>(links, ksz, msz, D, x, r, src + x * D, src + xr0 * D, src + xr1 * D, src + xl0 * D, src + xl1 * D,
-                       f1, R(1), f3, R(1), acc);
-        out[e] = acc;
-    }
+    const int x = e / D, r = e - x * D;
+    const int xr0 = nb_right(x, 0, g.nx, g.nt), xr1 = nb_right(x, 1, g.nx, g.nt);
+    const int xl0 = nb_left(x, 0, g.nx, g.nt), xl1 = nb_left(x, 1, g.nx, g.nt);
+    const R f1 = wraps_right(x, 0, g.nt) ? R(-1) : R(1);
+    const R f3 = wraps_left(x, 0, g.nt) ? R(-1) : R(1);
+    cplx<R> acc = A.mm * ld<R>(src + e);
+    stencil_row<R>(links, ksz, msz, D, x, r, src + x * D, src + xr0 * D, src + xr1 * D, src + xl0 * D, src + xl1 * D,
+                   f1, R(1), f3, R(1), acc);
+    return acc;
+}
+template <typename R, typename LP, typename VP>
+__device__ __forceinline__ void dop_rows_l(const CArgs<R>& A, int l, LP links, VP src, cplx<R>* out) {
+    const Geo& g = A.lv[l].g;
+    const int n = g.ns * g.dof;
+    for (int e = threadIdx.x; e < n; e += CT) out[e] = dop_row<R>(A, g, links, src, e);
 }
 template <typename R, typename VP>
 __device__ __forceinline__ void dop_rows(const CArgs<R>& A, int l, VP src, cplx<R>* out) {
@@ -315,6 +317,43 @@ __device__ __forceinline__ void c_dop(const CArgs<R>& A, int l, const cplx<R>* i
         dop_rows<R>(A, l, (const cplx<R>*)stage, out);
     } else {
         dop_rows<R>(A, l, as_global(in), out);
+    }
+    __syncthreads();
+}
+
+// r = v - D_l x on the rows of Schwarz block blk only (the rows of every
+// other block keep what they hold), with the bits of c_dop and the
+// subtraction after it: D_l x is stored (Dx) and subtracted after a barrier,
+// so no contraction joins the two.
+template <typename R>
+__device__ __forceinline__ void c_resid_block(const CArgs<R>& A, int l, int blk, const cplx<R>* v, const cplx<R>* x,
+                                              cplx<R>* Dx, cplx<R>* r, cplx<R>* stage, int stage_cap) {
+    const Geo& g = A.lv[l].g;
+    const int D = g.dof, nb = g.sspb * D, n = g.ns * D;
+    if (A.cnt && threadIdx.x == 0 && blockIdx.x % A.grp == 0)
+        atomicAdd(A.cnt, 8ull * 5 * D * (unsigned long long)nb);
+    __syncthreads();
+    // x read as c_dop reads it (the staged copy where it fits), so that the
+    // rows come from the same code
+    auto rows = [&](auto src) {
+        for (int q = threadIdx.x; q < nb; q += CT) {
+            const int m = q / D, d = q - m * D;
+            const int e = sap_site(g, blk, m) * D + d;
+            Dx[e] = dop_row<R>(A, g, as_global(A.lv[l].links), src, e);
+        }
+    };
+    if (stage && n <= stage_cap) {
+        for (int e = threadIdx.x; e < n; e += CT) stage[e] = x[e];
+        __syncthreads();
+        rows((const cplx<R>*)stage);
+    } else {
+        rows(as_global(x));
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb; q += CT) {
+        const int m = q / D, d = q - m * D;
+        const int e = sap_site(g, blk, m) * D + d;
+        r[e] = v[e] - Dx[e];
     }
     __syncthreads();
 }
@@ -957,9 +996,25 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
     cplx<R>* r = K.W + L.sR;
     cplx<R>* Dx = K.W + L.sDx;
     const double vn = sqrt(c_dot(K.S, v, v, n).re);
-    c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
-    for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
     const int grp = A.grp, mem = K.member;
+    // With one block per member (the workgroup form) a member reads the
+    // residual only on the block it solves next: the residual before the
+    // first red colour and those before black colours are formed on that
+    // block alone; the one after a black colour stays whole for the sweep's
+    // tolerance check.
+    bool own = false;
+    if constexpr (sizeof(R) == 4) own = A.sap_wg[l] != 0;
+    auto resid = [&](const int* next) {
+        if (own) {
+            if (mem < L.g.ncol)
+                c_resid_block(A, l, next[mem], v, x, Dx, r, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            else __syncthreads();
+        } else {
+            c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+            for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+        }
+    };
+    resid(L.red);
     cplx<R>* xs = grp > 1 ? A.xchg + (size_t)K.rhs * A.xchg_n : nullptr;
     const int D = L.g.dof, nbs = L.g.sspb * D;
     for (int it = 0; it < nu; it++) {
@@ -1012,8 +1067,12 @@ template <typename R> __device__ __forceinline__ void c_sap(const CArgs<R>& A, C
             // the residual after the last colour of the last sweep and its
             // tolerance check decide nothing: x is final
             if (colour == 1 && it + 1 == nu) return;
-            c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
-            for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+            if (colour == 0) {
+                resid(L.black);
+            } else {  // whole: the sweep's tolerance check reads it
+                c_dop(A, l, x, Dx, (cplx<R>*)K.S.vin, VIN_BYTES / (int)sizeof(cplx<R>));
+                for (int e = threadIdx.x; e < n; e += CT) r[e] = v[e] - Dx[e];
+            }
             T.lap(21);
         }
         const double err = sqrt(c_dot(K.S, r, r, n).re);
