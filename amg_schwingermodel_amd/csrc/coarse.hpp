@@ -825,11 +825,11 @@ __host__ __device__ inline size_t wg_block_lds_bytes(int dof, int m) {
 template <typename R>
 __device__ __forceinline__ void wg_dlocal_2x2(R mm, int D, const float __attribute__((ext_vector_type(4)))* Lk,
                                               cplx<R>* part, const cplx<R>* in, cplx<R>* out,
-                                              unsigned long long* cnt = nullptr) {
+                                              unsigned long long* cnt = nullptr, bool synced = false) {
     using V4 = float __attribute__((ext_vector_type(4)));
     const int nb = 4 * D, nu = 3 * nb, np = D / 2, u = threadIdx.x;
     if (cnt && u == 0) atomicAdd(cnt, 8ull * 3 * D * (unsigned long long)nb);
-    __syncthreads();  // in is complete
+    if (!synced) __syncthreads();  // in is complete (synced: the caller's barrier saw to it)
     if (u < nu) {
         const int q = u / nb, e = u - q * nb, m = e / D;
         const V4* vp = (const V4*)(in + (q == 0 ? m : q == 1 ? (m ^ 1) : (m ^ 2)) * D);
@@ -923,7 +923,8 @@ __device__ __noinline__ void g_block_gmres_2x2(const CArgs<R>& A, CK<R>& K, int 
         }
         for (int j = 0; j < m; j++) {
             T.lap(17);
-            wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1), cnt);
+            // after step j-1's closing barrier V(j) is complete
+            wg_dlocal_2x2<R>(A.mm, D, Lk, part, V(j), V(j + 1), cnt, j > 0);
             T.lap(5);
             T.count(16);
             // the pass quantities over the wavefronts, then the column on wave 0
