@@ -66,7 +66,7 @@ template <typename R> struct CArgs {
     // counter bar[rhs], zero at launch).  x_top: workspace offset of the
     // member's private copy of the solve's output.
     int grp;
-    // several gauge configurations (k_coarse_cfg): configuration r's links and
+    // several gauge configurations (mmv non-null): configuration r's links and
     // test vectors at lcs[l], wcs[l] entries past the first's, m0 + 2 = mmv[r]
     size_t lcs[MAXL], wcs[MAXL];
     const R* mmv;
