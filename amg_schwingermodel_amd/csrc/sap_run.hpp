@@ -201,8 +201,10 @@ __device__ __forceinline__ void sr_stencil32(const SrLinks& K, float mm, const P
 
 // W rows of the thread, FP16 as stored: the first WR in registers (row k by
 // a run-time index through a switch, so the array stays in registers), the
-// others in LDS (row k at wl[(k - WR) * nth]), which keeps the 16-row form
-// within its registers.
+// others in LDS (row k at wl[(k - WR) * nth]).  The 16-row form keeps six in
+// registers: with more, its row loops hold spill reloads, and a reload waits
+// (vector-memory counters retire in order) for every load issued before it,
+// the slot prefetch included.
 template <int RW> constexpr int sr_wreg() { return RW < 8 ? RW : RW == 8 ? 8 : 6; }
 template <int RW> constexpr size_t sr_lds_per_thread() {
     return SR_LDS_PER_THREAD + (size_t)(RW - sr_wreg<RW>()) * sizeof(f16x8);
