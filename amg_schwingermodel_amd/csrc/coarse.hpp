@@ -142,16 +142,19 @@ template <typename R> struct CK {
 };
 
 // Barrier of the grp members of one right-hand side.  Every thread makes its
-// global stores visible device-wide (agent-scope release: the members may sit
-// on different XCDs, whose L2 caches are not coherent with each other), one
-// thread counts the arrival and waits for the episode's total, and every
-// thread then acquires at agent scope before it reads what the others wrote.
-// The wait is bounded: past the bound the error flag is raised and the member
-// goes on (the host reports the error), so a launch always drains.
+// Member barrier.  Every wavefront first waits until its own global stores
+// (the published blocks) have reached its XCD's L2; after the workgroup
+// barrier thread 0 alone releases at agent scope (one L2 write-back: the
+// members may sit on different XCDs, whose L2 caches are not coherent with
+// each other), counts the arrival, waits for the episode's total and
+// acquires at agent scope (one invalidate of the CU's L1 and the XCD's L2),
+// and the closing workgroup barrier orders every wavefront's later reads
+// after it.  The wait is bounded: past the bound the error flag is raised and
+// the member goes on (the host reports the error), so a launch always drains.
 constexpr unsigned MEMBER_SPIN_LIMIT = 1u << 22;
 template <typename R> __device__ __forceinline__ void member_sync(CK<R>& K) {
     const CArgs<R>& A = K.A;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned target = (unsigned)A.grp * ++K.S.bar_epi;
@@ -168,7 +171,6 @@ template <typename R> __device__ __forceinline__ void member_sync(CK<R>& K) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // zero the member-barrier arrival counters before a grouped launch (a kernel,
