@@ -646,12 +646,6 @@ __device__ __forceinline__ c64 w_gs_quantity(int q, int j, const cplx<R>* w, VF 
     const cplx<R>* b = q <= j ? V(q) : q == 2 * j + 1 ? w : V(q - j - 1);
     return w_dot(a, b, nb);
 }
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
 // One wavefront, after the pass quantities are in S.dq: the column h_0..h_j
 // into S.col (lane i forms h_i; h_k is final once the lanes below it are),
 // the Gram row j into S.G; returns |w - sum_i h_i V_i|^2 (wave-uniform).

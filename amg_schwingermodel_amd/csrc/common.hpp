@@ -162,6 +162,14 @@ template <typename R> __device__ __forceinline__ cplx<R> wave_sum(cplx<R> v) {
     return {wave_sum(v.re), wave_sum(v.im)};
 }
 
+// lane l's value of v (l wave-uniform), for every lane
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
 // Block-wide sum of NQ complex values per thread; every thread gets the
 // totals in out[].  scratch: >= NQ * nwaves complex entries of LDS.
 template <int NQ, typename R>

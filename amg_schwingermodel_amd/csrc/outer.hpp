@@ -164,50 +164,47 @@ __global__ void __launch_bounds__(FT) k_outer_a(Geo g, double mm, const c64* __r
     if (threadIdx.x < nq) {
         c64 r = red[threadIdx.x];
         for (int wv = 1; wv < FT / 64; wv++) r += red[wv * NQ + threadIdx.x];
-        st.qa[((size_t)rhs * st.ntile_a + tile) * (2 * st.M + 1) + threadIdx.x] = r;
+        st.qa[((size_t)rhs * (2 * st.M + 1) + threadIdx.x) * st.ntile_a + tile] = r;
     }
-}
-
-// Sum over tiles of quantity q of base[t*stride + q], all threads of the
-// workgroup in a fixed order; the total lands in out.
-__device__ __forceinline__ void tile_sum(const c64* base, int stride, int ntile, int q, c64* red, c64* out) {
-    c64 v = {0, 0};
-    for (int t = threadIdx.x; t < ntile; t += FT) v += base[(size_t)t * stride + q];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        c64 r = red[0];
-        for (int wv = 1; wv < FT / 64; wv++) r += red[wv];
-        *out = r;
-    }
-    __syncthreads();
 }
 
 // Hessenberg column j by one-pass MGS: H_ij = <w,V_i> - sum_{k<i} H_kj <V_k,V_i>.
 // Stores the unrotated column in R[.][j] temporarily (rotated in k_outer_fin).
+static_assert(OUT_MAXM < 64, "one lane per Hessenberg row in k_outer_hcol");
 __global__ void __launch_bounds__(FT) k_outer_hcol(OuterState st, int j) {
     __shared__ c64 d[2 * OUT_MAXM + 1];
-    __shared__ c64 col[OUT_MAXM + 1];
-    __shared__ c64 red[FT / 64];
+    __shared__ c64 Gs[OUT_MAXM][OUT_MAXM];  // Gram rows 1..j-1 of the cycle
     const int rhs = blockIdx.x;
     if (!st.active[rhs]) return;
     const int nq = 2 * j + 1, M = st.M;
-    for (int q = 0; q < nq; q++)
-        tile_sum(st.qa + (size_t)rhs * st.ntile_a * (2 * M + 1), 2 * M + 1, st.ntile_a, q, red, &d[q]);
-    if (threadIdx.x != 0) return;
+    // the tile partials of quantity q lie contiguous: one wavefront per
+    // quantity, lane sums over tiles in order, then the wave butterfly
+    const c64* base = st.qa + (size_t)rhs * (2 * M + 1) * st.ntile_a;
+    const int lane = threadIdx.x & 63;
+    for (int q = threadIdx.x >> 6; q < nq; q += FT / 64) {
+        c64 v = {0, 0};
+        for (int t = lane; t < st.ntile_a; t += 64) v += base[(size_t)q * st.ntile_a + t];
+        v = wave_sum(v);
+        if (lane == 0) d[q] = v;
+    }
     c64* G = st.G + (size_t)rhs * (M + 1) * (M + 1);
     c64* Rm = st.R + (size_t)rhs * M * M;
-    for (int i = 0; i <= j; i++) {
-        c64 h = d[i];
-        for (int kk = 0; kk < i; kk++) {
-            const c64 Gik = (i == j) ? d[j + 1 + kk] : G[i * (M + 1) + kk];
-            h -= col[kk] * conj(Gik);
-        }
-        col[i] = h;
-        Rm[i * M + j] = h;
+    for (int e = threadIdx.x; e < j * j; e += FT) {
+        const int i = e / j, k = e - i * j;
+        if (k < i) Gs[i][k] = G[i * (M + 1) + k];
     }
-    for (int kk = 0; kk < j; kk++) G[j * (M + 1) + kk] = d[j + 1 + kk];
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    // lane i forms row i; H_kj is final once the lanes below k are (j < 64)
+    const int l = threadIdx.x;
+    c64 h = {0.0, 0.0};
+    if (l <= j) h = d[l];
+    for (int k = 0; k <= j; k++) {
+        const c64 hk = {readlane_d(h.re, k), readlane_d(h.im, k)};
+        if (l > k && l <= j) h -= hk * conj(l == j ? d[j + 1 + k] : Gs[l][k]);
+    }
+    if (l <= j) Rm[l * M + j] = h;
+    if (l < j) G[j * (M + 1) + l] = d[j + 1 + l];
 }
 
 // w -= sum_i H_ij V_i; partial |w|^2
