@@ -117,6 +117,8 @@ SIGNATURES = {
     "amgs_set_tuning": ([_V, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_solve_multi": ([ctypes.c_int, ctypes.POINTER(_V), _I, ctypes.POINTER(_V), ctypes.POINTER(_V), _I, _P, _I],
                          ctypes.c_int),
+    "amgs_solve_multi_staggered": ([ctypes.c_int, ctypes.POINTER(_V), _I, ctypes.POINTER(_V), ctypes.POINTER(_V), _P,
+                                    _I, _P, _I], ctypes.c_int),
     "amgs_kernel_timing": ([_V, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amgs_kernel_stats": ([_V, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P], ctypes.c_int),
     "amgs_kernel_stats_ex": ([_V, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P, _P],
@@ -190,11 +192,12 @@ def gaussian(n, seed):
     return out
 
 
-def solve_multi(solvers, b_ptrs, x_ptrs, nrhs=None):
+def solve_multi(solvers, b_ptrs, x_ptrs, nrhs=None, start_ms=None):
     """Concurrent solves of independent contexts (amgs_solve_multi): solver i
     solves nrhs[i] right-hand sides from device address b_ptrs[i] into
-    x_ptrs[i] (e.g. torch tensors' data_ptr()).  Returns (status, [iters_i],
-    [relres_i])."""
+    x_ptrs[i] (e.g. torch tensors' data_ptr()).  With start_ms, solver i's
+    first launch waits start_ms[i] milliseconds (amgs_solve_multi_staggered).
+    Returns (status, [iters_i], [relres_i])."""
     k = len(solvers)
     nrhs = list(nrhs) if nrhs is not None else [1] * k
     tot = sum(nrhs)
@@ -205,7 +208,14 @@ def solve_multi(solvers, b_ptrs, x_ptrs, nrhs=None):
     it = np.zeros(tot, np.int32)
     rr = np.zeros(tot)
     st = np.zeros(k, np.int32)
-    rc = lib().amgs_solve_multi(k, ctxs, nr, bp, xp, it.ctypes.data_as(_I), _p(rr), st.ctypes.data_as(_I))
+    if start_ms is None:
+        rc = lib().amgs_solve_multi(k, ctxs, nr, bp, xp, it.ctypes.data_as(_I), _p(rr), st.ctypes.data_as(_I))
+    else:
+        sm = np.ascontiguousarray(start_ms, np.float64)
+        if sm.shape != (k,):
+            raise ValueError("start_ms needs one offset per solver")
+        rc = lib().amgs_solve_multi_staggered(k, ctxs, nr, bp, xp, _p(sm), it.ctypes.data_as(_I), _p(rr),
+                                              st.ctypes.data_as(_I))
     if rc < 0:
         raise AmgsError(f"amgs_solve_multi failed: {rc} (per-context status {st.tolist()})")
     off = np.cumsum([0] + nrhs)
@@ -244,7 +254,12 @@ class Solver:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals may already be gone: the
+        # process then releases the device memory itself
+        try:
+            self.close()
+        except TypeError:
+            pass
 
     def _chk(self, rc, what, ok=(AMGS_OK,)):
         if rc not in ok:

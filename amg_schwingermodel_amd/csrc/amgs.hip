@@ -1633,17 +1633,22 @@ extern "C" int amgs_set_graph(amgs_ctx* ctx, int enable) {
 // ---- uneven multi-configuration batch ----
 // Independent contexts (each its own lattice size, hierarchy, mass and HIP
 // stream) solve concurrently: one host thread per context drives its solve, so
-// the kernels of all configurations overlap on the device.
-extern "C" int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev,
-                                void* const* x_dev, int* iters, double* relres, int* status) {
+// the kernels of all configurations overlap on the device.  A thread may wait
+// a given time before its first launch, to offset the phases of sub-batches
+// that would otherwise run in lockstep.
+extern "C" int amgs_solve_multi_staggered(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev,
+                                          void* const* x_dev, const double* start_ms, int* iters, double* relres,
+                                          int* status) {
     if (nctx < 1 || !ctxs || !nrhs || !b_dev || !x_dev || !iters || !relres || !status) return AMGS_ERR_ARG;
     std::vector<size_t> off(nctx + 1, 0);
     for (int i = 0; i < nctx; i++) {
         if (!ctxs[i] || nrhs[i] < 1) return AMGS_ERR_ARG;
+        if (start_ms && !(start_ms[i] >= 0.0 && start_ms[i] <= 1000.0)) return AMGS_ERR_ARG;
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return AMGS_ERR_ARG;  // one solve per context at a time
         off[i + 1] = off[i] + nrhs[i];
     }
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
     th.reserve(nctx);
     for (int i = 0; i < nctx; i++)
@@ -1652,6 +1657,8 @@ extern "C" int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs
                 status[i] = AMGS_ERR_HIP;
                 return;
             }
+            if (start_ms && start_ms[i] > 0.0)
+                std::this_thread::sleep_until(t0 + std::chrono::duration<double, std::milli>(start_ms[i]));
             status[i] = amgs_solve_dev(ctxs[i], nrhs[i], b_dev[i], x_dev[i], iters + off[i], relres + off[i]);
         });
     for (std::thread& t : th) t.join();
@@ -1661,6 +1668,11 @@ extern "C" int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs
         if (status[i] == AMGS_NOT_CONVERGED) rc = AMGS_NOT_CONVERGED;
     }
     return rc;
+}
+
+extern "C" int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev,
+                                void* const* x_dev, int* iters, double* relres, int* status) {
+    return amgs_solve_multi_staggered(nctx, ctxs, nrhs, b_dev, x_dev, nullptr, iters, relres, status);
 }
 
 // ---- comparison solvers on the fine lattice (FP64) ----

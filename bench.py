@@ -24,7 +24,11 @@ per right-hand side per iteration and the residual reached).
 
 A step is one batched solve of the 32 right-hand sides.  Inputs are resident
 in device memory before the timed region; set-up (test vectors, coarse
-operators) runs before it and is reported separately.
+operators) runs before it and is reported separately.  The batch is timed as
+one 32-rhs solve and as --split concurrent sub-batches (one context and
+stream each, amgs_solve_multi_staggered), sub-batch i starting i*d ms after
+the first with d the fastest of --offsets in one untimed trial each; value is
+the faster form, the offsets' wait included.
 
 Prints one JSON line with "roofline" (the dominant fine-lattice kernel, the
 SAP Arnoldi step A: block Wilson stencil + Gram dot products, timed per launch
@@ -93,6 +97,9 @@ def parse():
                     help="also time the batch as this many concurrent sub-batches of (nearly) equal size "
                          "(amgs_solve_multi, one context and stream each); value is the faster of the two forms "
                          "(1: skip)")
+    ap.add_argument("--offsets", type=lambda s: [float(v) for v in s.split(",") if v], default=[0, 1, 2, 3, 4],
+                    help="start offsets (ms, comma-separated) tried for the concurrent sub-batches: sub-batch i "
+                         "starts i*d ms after the first; the fastest d is kept for the timed steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024 4-level solve beside the value")
@@ -264,10 +271,24 @@ def main():
         xp = [xd[first[i]].data_ptr() for i in range(split)]
         for w in range(a.warmup):
             amgs.solve_multi(ss, bp, xp, sizes)
+        # start offsets: sub-batch i starts i * d ms after sub-batch 0, so that
+        # the sub-batches' phases interleave instead of running in lockstep;
+        # one untimed solve per candidate d, the fastest is kept
+        trials = {}
+        for d in a.offsets:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            amgs.solve_multi(ss, bp, xp, sizes, start_ms=[i * d for i in range(split)])
+            torch.cuda.synchronize()
+            trials[d] = (time.perf_counter() - t) * 1e3
+        d_best = min(trials, key=trials.get) if trials else 0.0
+        starts = [i * d_best for i in range(split)]
+        log(f"{tag} {sizes} rhs concurrent: start offsets " +
+            ", ".join(f"{d:g} ms: {v:.1f} ms" for d, v in trials.items()) + f"; keeping {d_best:g} ms")
         torch.cuda.synchronize()
         t = time.perf_counter()
         for k in range(a.steps):
-            st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, sizes)
+            st, its_s, rr_s = amgs.solve_multi(ss, bp, xp, sizes, start_ms=starts)
             log(f"{tag} {sizes} rhs concurrent: step {k + 1}/{a.steps} done ({(time.perf_counter() - t):.2f}s)")
         torch.cuda.synchronize()
         ms_split = (time.perf_counter() - t) * 1e3 / a.steps
@@ -278,6 +299,8 @@ def main():
             s.close()
         ok = rr_s.max() < 1e-10 and all(v == 0 for v in st.tolist())
         return ms_split, ok, {"sub_batches": sizes, "ms_per_rhs": round(ms_split / nrhs, 3),
+                              "start_offsets_ms": starts,
+                              "offset_trials_ms": {f"{d:g}": round(v, 2) for d, v in trials.items()},
                               "iterations_min": int(its_s.min()), "iterations_max": int(its_s.max()),
                               "relres_max": float(rr_s.max()), "status": st.tolist()}
 

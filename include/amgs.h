@@ -169,6 +169,17 @@ int amgs_set_tuning(amgs_ctx* c, int nrhs, int sap_wgs, int coarse_grp);
 int amgs_solve_multi(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev, void* const* x_dev,
                      int* iters, double* relres, int* status);
 
+/* amgs_solve_multi with staggered starts (new, a scheduling form with no
+ * reference counterpart): context i's host thread waits start_ms[i]
+ * milliseconds (0 to 1000; a null start_ms means no wait) before its first
+ * launch.  Equal-sized sub-batches of one workload otherwise run their
+ * phases in lockstep, so that their bandwidth-bound passes (the outer Arnoldi
+ * steps) contend with each other while the latency-bound coarse kernels leave
+ * the memory idle; an offset of a few milliseconds interleaves them.  The
+ * results do not depend on the offsets (the contexts share no data). */
+int amgs_solve_multi_staggered(int nctx, amgs_ctx* const* ctxs, const int* nrhs, const void* const* b_dev,
+                               void* const* x_dev, const double* start_ms, int* iters, double* relres, int* status);
+
 /* ---- component entry points (the reference's building blocks, used by the
  *      parity tests and by callers that assemble their own solvers) ---- */
 

@@ -439,6 +439,50 @@ def test_solve_multi_uneven_configurations():
         s.close()
 
 
+def test_solve_multi_staggered_starts():
+    """Sub-batches of one workload as concurrent contexts with staggered
+    starts (amgs_solve_multi_staggered, graph replay, FP16 SAP basis): the
+    same iterations and solutions, bit for bit, as simultaneous starts, for
+    several offsets; offsets outside 0..1000 ms are rejected."""
+    import torch
+
+    nx = nt = 64
+    rows = [(0, 4, 4, 10, 4, 4), (1, 2, 2, 10, 2, 2)]
+    U, _ = amgs.gauge_metropolis(nx, nt, 3.0, 30, 77)
+    sizes = [3, 2]
+    solvers = []
+    for k in sizes:
+        s = amgs.Solver(amgs.default_params(nx, nt, rows, m0=-0.10, precision=amgs.PREC_MIXED16, max_rhs=k), U)
+        s.setup()
+        s.set_graph(True)
+        solvers.append(s)
+    b = np.stack([cvec(nx * nt * 2, 900 + r) for r in range(sum(sizes))])
+    bd = torch.from_numpy(b.view(np.float64).copy()).to("cuda:0")
+    bp = [bd[0].data_ptr(), bd[sizes[0]].data_ptr()]
+
+    def run(start_ms):
+        xd = torch.zeros_like(bd)
+        torch.cuda.synchronize()
+        st, its, rrs = amgs.solve_multi(solvers, bp, [xd[0].data_ptr(), xd[sizes[0]].data_ptr()], sizes,
+                                        start_ms=start_ms)
+        torch.cuda.synchronize()
+        assert list(st) == [amgs.AMGS_OK] * 2
+        return np.concatenate(its), np.concatenate(rrs), xd.cpu().numpy()
+
+    its0, rr0, x0 = run(None)
+    assert np.all(rr0 < 1e-10)
+    for start in ([0.0, 0.0], [0.0, 2.5], [4.0, 0.0]):
+        its1, rr1, x1 = run(start)
+        assert np.array_equal(its1, its0), start
+        assert np.array_equal(rr1, rr0), start
+        assert np.array_equal(x1, x0), start
+    for bad in ([0.0, -1.0], [0.0, 2000.0], [float("nan"), 0.0]):
+        with pytest.raises(amgs.AmgsError):
+            run(bad)
+    for s in solvers:
+        s.close()
+
+
 @pytest.mark.parametrize("prec", [amgs.PREC_FP64, amgs.PREC_MIXED, amgs.PREC_MIXED16])
 @pytest.mark.parametrize("case", ["32_3lvl", "16x32_3lvl"])
 def test_graph_replay_matches_eager(case, prec):
