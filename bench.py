@@ -30,10 +30,11 @@ stream each, amgs_solve_multi_staggered), sub-batch i starting i*d ms after
 the first with d the fastest of --offsets in one untimed trial each; value is
 the faster form, the offsets' wait included.
 
-Prints one JSON line with "roofline" (the dominant fine-lattice kernel, the
-SAP Arnoldi step A: block Wilson stencil + Gram dot products, timed per launch
-with HIP events on the library stream during the timed solves, also per
-Arnoldi index J), "roofline_coarse" (the persistent coarse-level kernel: its
+Prints one JSON line with "roofline" (the dominant kernel, timed per launch
+with HIP events on the library stream during an eager pass of the batched
+solve: the streaming level-0 SAP kernel k_sap_run, one colour's block solves
+per launch, where that form runs; else the chunked SAP Arnoldi step A, block
+Wilson stencil + Gram dot products, also per Arnoldi index J), "roofline_coarse" (the persistent coarse-level kernel: its
 counted stencil flops over its launch times), "roofline_coarse_mfma" (the
 coarse operator over the 32 right-hand sides as MFMA block-GEMMs) and
 "cpu_baseline" (the oracle, a C restatement of the reference path, timed on
@@ -95,8 +96,8 @@ def parse():
                     help="1: replay the level-0 preconditioner from a captured HIP graph in the timed solves")
     ap.add_argument("--split", type=int, default=2,
                     help="also time the batch as this many concurrent sub-batches of (nearly) equal size "
-                         "(amgs_solve_multi, one context and stream each); value is the faster of the two forms "
-                         "(1: skip)")
+                         "(amgs_solve_multi_staggered, one context and stream each, starts offset by --offsets); "
+                         "value is the faster of the two forms (1: skip)")
     ap.add_argument("--offsets", type=lambda s: [float(v) for v in s.split(",") if v], default=[0, 1, 2, 3, 4],
                     help="start offsets (ms, comma-separated) tried for the concurrent sub-batches: sub-batch i "
                          "starts i*d ms after the first; the fastest d is kept for the timed steps")
@@ -502,10 +503,17 @@ def main():
             est = t_solve * 1e3
             what = f"the full solve to 1e-10 ({it_cpu} outer iterations, relres {rr_cpu:.2e}, {t_solve:.1f}s)"
         log(f"oracle: setup {t_setup:.1f}s, {it_cpu} outer iterations in {t_solve:.1f}s -> {est / 1e3:.2f}s/rhs")
+        # the host's model name, with its family/model numbers (a virtualised
+        # host may report only the brand in the name)
         cpu = "unknown"
         try:
-            cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
-        except (OSError, IndexError):
+            info = {}
+            for l in open("/proc/cpuinfo"):
+                k, _, v = l.partition(":")
+                info.setdefault(k.strip(), v.strip())
+            cpu = f"{info['model name']} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, " \
+                  f"{os.cpu_count()} logical CPUs visible)"
+        except (OSError, KeyError):
             pass
         out["cpu_baseline"] = {"value": round(est, 1), "unit": "ms/RHS", "cores": threads, "cpu": cpu, "kind": "port",
                                "sample": f"oracle (C restatement of the reference path, OpenMP over SAP blocks) "
