@@ -22,5 +22,15 @@ def test_more_than_one_gpu_exits_with_message():
 def test_help_lists_flags():
     r = run("--help")
     assert r.returncode == 0
-    for flag in ("--steps", "--warmup", "--split", "--no-large", "--no-alt", "--no-cpu", "--no-plain", "--exact-iters"):
+    for flag in ("--steps", "--warmup", "--split", "--offsets", "--no-large", "--no-alt", "--no-cpu", "--no-plain",
+                 "--exact-iters"):
         assert flag in r.stdout, flag
+
+
+def test_offsets_flag_parses_a_list():
+    """--offsets takes comma-separated milliseconds (the start offsets tried
+    for the concurrent sub-batches); the single-GPU check still comes first."""
+    r = run("--offsets", "0,1.5,3", "--gpus", "2")
+    assert r.returncode == 2 and "one GPU" in r.stderr
+    r = run("--offsets", "zero")
+    assert r.returncode == 2 and "--offsets" in r.stderr
