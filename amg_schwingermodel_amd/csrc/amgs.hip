@@ -1347,6 +1347,10 @@ extern "C" void amgs_destroy(amgs_ctx* ctx) {
     hipStream_t s = c.stream;
     hipEvent_t a = c.ev0, b = c.ev1, f = c.evf;
     int* hf = c.hflags;
+    for (int k = 0; k < 2; k++) {
+        if (c.stage_ev[k]) (void)hipEventDestroy(c.stage_ev[k]);
+        if (c.stage[k]) (void)hipHostFree(c.stage[k]);
+    }
     c.clear_graphs();
     delete ctx;  // frees device buffers
     if (a) (void)hipEventDestroy(a);
@@ -1446,17 +1450,61 @@ static int solve_impl(Ctx& c, int nrhs, int* iters, double* relres, double* hist
     return all ? AMGS_OK : AMGS_NOT_CONVERGED;
 }
 
+// Host buffers of a solve pass through two pinned chunks of STAGE_BYTES: the
+// host's threads copy one chunk while the DMA of the other runs.
+constexpr size_t STAGE_BYTES = size_t(32) << 20;
+static void stage_init(Ctx& c) {
+    if (c.stage[0]) return;
+    for (int k = 0; k < 2; k++) {
+        HIP_CHECK(hipHostMalloc(&c.stage[k], STAGE_BYTES));
+        HIP_CHECK(hipEventCreateWithFlags(&c.stage_ev[k], hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(c.stage_ev[k], c.stream));
+    }
+}
+static void h2d_staged(Ctx& c, void* dst, const void* src, size_t bytes) {
+    stage_init(c);
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        HIP_CHECK(hipEventSynchronize(c.stage_ev[k]));  // the chunk's previous DMA has read it
+        par_copy(c.stage[k], static_cast<const char*>(src) + off, n);
+        HIP_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, c.stage[k], n, hipMemcpyHostToDevice, c.stream));
+        HIP_CHECK(hipEventRecord(c.stage_ev[k], c.stream));
+    }
+}
+static void d2h_staged(Ctx& c, void* dst, const void* src, size_t bytes) {
+    stage_init(c);
+    // chunk i's DMA is queued before chunk i-1 is copied out of its buffer
+    size_t prev_off = 0, prev_n = 0;
+    int k = 0, prev_k = -1;
+    auto drain = [&]() {
+        if (prev_k < 0) return;
+        HIP_CHECK(hipEventSynchronize(c.stage_ev[prev_k]));
+        par_copy(static_cast<char*>(dst) + prev_off, c.stage[prev_k], prev_n);
+    };
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        HIP_CHECK(hipMemcpyAsync(c.stage[k], static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, c.stream));
+        HIP_CHECK(hipEventRecord(c.stage_ev[k], c.stream));
+        drain();
+        prev_off = off;
+        prev_n = n;
+        prev_k = k;
+    }
+    drain();
+}
+
 extern "C" int amgs_solve(amgs_ctx* ctx, int nrhs, const double* b, double* x, int* iters, double* relres, double* hist,
                           int hist_cap, int* nhist) {
     if (!ctx || !b || !x) return AMGS_ERR_ARG;
     API_BEGIN
     Ctx& c = ctx->c;
     need(nrhs >= 1 && nrhs <= c.P.max_rhs, AMGS_ERR_ARG, "nrhs must be in [1, max_rhs]");
-    const size_t n0 = c.nvec(0);
-    for (size_t i = 0; i < 2 * n0 * nrhs; i++) need(std::isfinite(b[i]), AMGS_ERR_NAN, "right-hand side holds NaN/Inf");
-    HIP_CHECK(hipMemcpyAsync(c.ob.p, b, sizeof(c64) * n0 * nrhs, hipMemcpyHostToDevice, c.stream));
+    const size_t n0 = c.nvec(0), bytes = sizeof(c64) * n0 * nrhs;
+    need(all_finite(b, 2 * n0 * nrhs), AMGS_ERR_NAN, "right-hand side holds NaN/Inf");
+    h2d_staged(c, c.ob.p, b, bytes);
     int rc = solve_impl(c, nrhs, iters, relres, hist, hist_cap, nhist);
-    HIP_CHECK(hipMemcpy(x, c.ox.p, sizeof(c64) * n0 * nrhs, hipMemcpyDeviceToHost));
+    d2h_staged(c, x, c.ox.p, bytes);
     return rc;
     API_END
 }

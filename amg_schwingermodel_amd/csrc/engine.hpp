@@ -243,6 +243,9 @@ struct Ctx {
     int* hflags = nullptr;
     int hflags_cap = 0;
     hipEvent_t evf = nullptr;
+    // two pinned chunks that stage host-buffer solves (allocated on first use)
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     KTimer kt[KT_CLASSES];
     // HIP-graph replay of the outer FGMRES restart cycle (m Arnoldi steps with
     // their preconditioner applications, the update and the new residual):

@@ -2,6 +2,7 @@
 // formats, and the test-vector random stream.  No HIP calls here.
 #include "host_util.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,6 +60,23 @@ static inline uint64_t mix64(uint64_t z) {
 static inline double u01(uint64_t seed, uint64_t stream, uint64_t ctr) {
     const uint64_t h = mix64(seed ^ mix64(stream * 0x632be59bd9b4e019ull + ctr));
     return ((h >> 11) + 0.5) * (1.0 / 9007199254740992.0);  // (0, 1)
+}
+
+bool all_finite(const double* p, size_t n) {
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (n > (1u << 16))
+    for (long long i = 0; i < (long long)n; i++) bad |= !std::isfinite(p[i]);
+    return bad == 0;
+}
+
+void par_copy(void* dst, const void* src, size_t bytes) {
+    const size_t B = size_t(1) << 20;
+    const long long nb = (long long)((bytes + B - 1) / B);
+#pragma omp parallel for schedule(static) if (nb > 1)
+    for (long long i = 0; i < nb; i++) {
+        const size_t o = (size_t)i * B;
+        std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(B, bytes - o));
+    }
 }
 
 }  // namespace amgs

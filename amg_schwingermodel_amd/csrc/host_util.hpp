@@ -21,4 +21,9 @@ private:
     Impl* p;
 };
 
+// true if no entry of p[0..n) is NaN or infinite (OpenMP over the entries)
+bool all_finite(const double* p, size_t n);
+// memcpy over the host's threads (1 MiB pieces)
+void par_copy(void* dst, const void* src, size_t bytes);
+
 }  // namespace amgs
